@@ -1,0 +1,254 @@
+"""Host-side mirror of distBVH's collision API over the C ABI.
+
+Names follow the reference: CollisionWorld ~ bvh::collision_world
+(collision_world.hpp:54-114), CollisionObject ~ bvh::collision_object
+(collision_object.hpp:55-283).  Element data is a numpy array (host) or a
+torch CUDA tensor (device, read in place); see include/distbvh.h for layouts.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import call
+
+CLUSTERING = capi.BVH_SPLIT_CLUSTERING
+GEOM_AXIS = capi.BVH_SPLIT_GEOM_AXIS
+ML_GEOM_AXIS = capi.BVH_SPLIT_ML_GEOM_AXIS
+
+
+def _ptr_mem(a, dtype):
+    """(pointer, memory kind, keepalive) of a numpy array or a CUDA tensor."""
+    if a is None:
+        return None, capi.BVH_MEM_HOST, None
+    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+        return C.c_void_p(a.data_ptr()), capi.BVH_MEM_DEVICE, a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return C.c_void_p(arr.ctypes.data), capi.BVH_MEM_HOST, arr
+
+
+class CollisionWorld:
+    """collision_world(od, config)."""
+
+    def __init__(self, overdecomposition=1, morton_bits=63, leaf_size=1, precision="fp64",
+                 sort_results=False, timing=False, stream=None, device=0):
+        cfg = capi.WorldConfig()
+        capi.load().bvh_world_config_init(C.byref(cfg))
+        cfg.morton_bits = morton_bits
+        cfg.leaf_size = leaf_size
+        cfg.precision = capi.BVH_PREC_FP32_OUTWARD if precision == "fp32" else capi.BVH_PREC_FP64
+        cfg.flags = (capi.BVH_FLAG_SORT_RESULTS if sort_results else 0) | \
+                    (capi.BVH_FLAG_TIMING if timing else 0)
+        cfg.stream = stream
+        cfg.device = device
+        h = C.c_void_p()
+        call("bvh_world_create", overdecomposition, C.byref(cfg), C.byref(h))
+        self.handle = h
+        self.objects = []
+        self.od = overdecomposition
+
+    def close(self):
+        if getattr(self, "handle", None):
+            capi.load().bvh_world_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def overdecomposition_factor(self):
+        v = C.c_size_t()
+        call("bvh_world_overdecomposition", self.handle, C.byref(v))
+        return v.value
+
+    def create_collision_object(self):
+        h = C.c_void_p()
+        call("bvh_create_collision_object", self.handle, C.byref(h))
+        o = CollisionObject(self, h)
+        self.objects.append(o)
+        return o
+
+    def num_collision_objects(self):
+        v = C.c_size_t()
+        call("bvh_num_collision_objects", self.handle, C.byref(v))
+        return v.value
+
+    def start_iteration(self):
+        call("bvh_start_iteration", self.handle)
+
+    def finish_iteration(self):
+        call("bvh_finish_iteration", self.handle)
+
+    def synchronize(self):
+        call("bvh_synchronize", self.handle)
+
+    def stage_times(self):
+        names = (C.c_char_p * 64)()
+        ms = (C.c_double * 64)()
+        cnt = C.c_size_t()
+        call("bvh_stage_times", self.handle, names, ms, 64, C.byref(cnt))
+        return [(names[i].decode(), ms[i]) for i in range(min(cnt.value, 64))]
+
+    # stand-alone primitives on device tensors (torch)
+    def sort(self, keys, vals=None):
+        fn = "bvh_sort_u64" if keys.element_size() == 8 else "bvh_sort_u32"
+        call(fn, self.handle, C.c_void_p(keys.data_ptr()),
+             C.c_void_p(vals.data_ptr()) if vals is not None else None, keys.numel())
+
+    def exclusive_scan(self, vals):
+        call("bvh_exclusive_scan_u32", self.handle, C.c_void_p(vals.data_ptr()), vals.numel())
+
+
+class CollisionObject:
+    """collision_object: owned by its world."""
+
+    def __init__(self, world, handle):
+        self.world = world
+        self.handle = handle
+        self._keep = None
+        self.n = 0
+
+    def id(self):
+        v = C.c_size_t()
+        call("bvh_object_id", self.handle, C.byref(v))
+        return v.value
+
+    # ---- set_entity_data -------------------------------------------------------
+    def set_entity_data_poly(self, verts, epsilon=0.0, global_ids=None, split_algorithm=CLUSTERING):
+        """verts: (nv, 3, n) float64 SoA planes (numpy or CUDA tensor)."""
+        nv, _, n = tuple(verts.shape)
+        p, mem, keep = _ptr_mem(verts, np.float64)
+        g, gmem, gkeep = _ptr_mem(global_ids, np.uint64)
+        if global_ids is not None and gmem != mem:
+            raise ValueError("global ids and elements must live in the same memory")
+        self._keep = (keep, gkeep)
+        self.n = n
+        call("bvh_set_entity_data_poly", self.handle, p, nv, n, g, float(epsilon), mem, split_algorithm)
+
+    def set_entity_data_sphere(self, xyzr, global_ids=None, split_algorithm=CLUSTERING):
+        """xyzr: (4, n) float64 planes x, y, z, r."""
+        n = tuple(xyzr.shape)[1]
+        p, mem, keep = _ptr_mem(xyzr, np.float64)
+        g, _, gkeep = _ptr_mem(global_ids, np.uint64)
+        self._keep = (keep, gkeep)
+        self.n = n
+        call("bvh_set_entity_data_sphere", self.handle, p, n, g, mem, split_algorithm)
+
+    def set_entity_data_snapshot(self, kdops, centroids, global_ids=None, split_algorithm=CLUSTERING):
+        """kdops: (n, 26) float64, centroids: (n, 3) float64."""
+        n = tuple(kdops.shape)[0]
+        p, mem, keep = _ptr_mem(kdops, np.float64)
+        c, _, ckeep = _ptr_mem(centroids, np.float64)
+        g, _, gkeep = _ptr_mem(global_ids, np.uint64)
+        self._keep = (keep, ckeep, gkeep)
+        self.n = n
+        call("bvh_set_entity_data_snapshot", self.handle, p, c, n, g, mem, split_algorithm)
+
+    def init_broadphase(self):
+        call("bvh_init_broadphase", self.handle)
+
+    def broadphase(self, other):
+        call("bvh_broadphase", self.handle, other.handle)
+
+    def self_collision(self):
+        call("bvh_self_collision", self.handle)
+
+    def end_phase(self):
+        call("bvh_end_phase", self.handle)
+
+    # ---- results ---------------------------------------------------------------
+    def result_count(self):
+        v = C.c_uint64()
+        call("bvh_result_count", self.handle, C.byref(v))
+        return v.value
+
+    def results(self):
+        """(k, 4) uint64 records {patch_p, element_p, patch_q, element_q}."""
+        n = self.result_count()
+        out = np.zeros((max(n, 1), 4), np.uint64)
+        call("bvh_result_records", self.handle, C.c_void_p(out.ctypes.data), n, capi.BVH_MEM_HOST)
+        return out[:n]
+
+    def for_each_result(self, fn):
+        """for_each_result<detailed_narrowphase_result>(f), collision_object.hpp:177-187."""
+        for r in self.results():
+            fn(r)
+
+    def pairs(self):
+        """(k, 2) input-index pairs."""
+        n = self.result_count()
+        out = np.zeros(max(n, 1), np.uint64)
+        call("bvh_result_pairs", self.handle, C.c_void_p(out.ctypes.data), n, capi.BVH_MEM_HOST)
+        out = out[:n]
+        return np.stack([out >> np.uint64(32), out & np.uint64(0xffffffff)], axis=1)
+
+    def pairs_device(self):
+        p = C.c_void_p()
+        n = C.c_uint64()
+        call("bvh_result_pairs_device", self.handle, C.byref(p), C.byref(n))
+        return p.value, n.value
+
+    def patch_pairs(self):
+        n = C.c_uint64()
+        call("bvh_patch_pair_count", self.handle, C.byref(n))
+        out = np.zeros((max(n.value, 1), 2), np.uint64)
+        call("bvh_patch_pairs", self.handle, C.c_void_p(out.ctypes.data), n.value)
+        return out[:n.value]
+
+    def local_patches(self):
+        od = self.world.od
+        arr = (capi.PatchInfo * od)()
+        call("bvh_local_patches", self.handle, arr, od)
+        return [{"global_id": p.global_id, "size": p.size, "kdop": np.array(p.kdop[:]),
+                 "centroid": np.array(p.centroid[:])} for p in arr]
+
+    def patch_elements(self, k):
+        cnt = C.c_uint64()
+        out = np.zeros(max(self.n, 1), np.uint64)
+        call("bvh_patch_elements", self.handle, k, C.c_void_p(out.ctypes.data), self.n, C.byref(cnt))
+        return out[:cnt.value]
+
+    # ---- stage outputs -----------------------------------------------------------
+    def bounds(self):
+        mn = np.zeros(3)
+        inv = np.zeros(3)
+        call("bvh_object_bounds", self.handle, C.c_void_p(mn.ctypes.data), C.c_void_p(inv.ctypes.data))
+        return mn, inv
+
+    def sorted(self):
+        codes = np.zeros(max(self.n, 1), np.uint64)
+        perm = np.zeros(max(self.n, 1), np.uint32)
+        call("bvh_object_sorted", self.handle, C.c_void_p(codes.ctypes.data), C.c_void_p(perm.ctypes.data),
+             capi.BVH_MEM_HOST)
+        return codes[:self.n], perm[:self.n]
+
+    def splits(self):
+        od = self.world.od
+        out = np.zeros(max(od - 1, 1), np.uint64)
+        call("bvh_object_splits", self.handle, C.c_void_p(out.ctypes.data), max(od - 1, 0))
+        return out[:od - 1]
+
+    def sorted_kdops(self):
+        out = np.zeros((max(self.n, 1), 26))
+        call("bvh_object_sorted_kdops", self.handle, C.c_void_p(out.ctypes.data), capi.BVH_MEM_HOST)
+        return out[:self.n]
+
+    def tree(self):
+        nn = C.c_size_t()
+        root = C.c_size_t()
+        call("bvh_object_tree", self.handle, None, None, None, 0, C.byref(nn), C.byref(root))
+        m = nn.value
+        kd = np.zeros((max(m, 1), 26))
+        ch = np.zeros((max(m, 1), 2), np.int64)
+        rg = np.zeros((max(m, 1), 2), np.uint64)
+        call("bvh_object_tree", self.handle, C.c_void_p(kd.ctypes.data), C.c_void_p(ch.ctypes.data),
+             C.c_void_p(rg.ctypes.data), m, C.byref(nn), C.byref(root))
+        return {"kdop": kd[:m], "children": ch[:m], "range": rg[:m], "root": root.value}

@@ -1,0 +1,657 @@
+// build.hip -- object build on gfx950: element bounds, Morton codes, cluster
+// splits, and the bottom-up tree with refit.
+//
+// Stage map (reference file:line -> kernel):
+//  * per-element 26-DOP + centroid, kdop.hpp:238-288 / TestCommon.hpp:62-73
+//      -> k_prep (centroid, cardinal bounds), k_tree (slabs, in sorted order)
+//  * compute_bounds(min_inv_diag), contact_entity.hpp:136-148
+//      -> k_prep block reduction + last-block finalize
+//  * morton(view, bounds, codes), contact_entity.hpp:157-191, hash.hpp
+//      -> k_morton (+ digit histograms of the sort)
+//  * radix_sorter, util/sort.hpp:75-144 -> sort.hip
+//  * morton_cluster splits, split/cluster.hpp:143-168
+//      -> k_cluster_depth + one 8-bit sort pass + k_cluster_select
+//  * patches (collision_object.cpp:134-141) and the tree
+//      -> k_delta + k_tree (Apetrei-style bottom-up build: each leaf climbs,
+//         the second child to reach a parent merges both children's slabs)
+//      -> k_ropes (escape links for stackless traversal)
+//
+// The tree is the binary radix tree of the augmented keys
+// (patch id, Morton code, sorted position): every patch is a subtree, the
+// levels above the patch subtrees form the patch tree, and duplicate codes
+// are split by position.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace dbvh {
+
+constexpr int kPrepBlock = 256;
+constexpr int kPrepMaxBlocks = 2048;
+
+uint64_t prep_blocks(uint64_t n) {
+  uint64_t b = (n + kPrepBlock - 1) / kPrepBlock;
+  if (b < 1) b = 1;
+  if (b > kPrepMaxBlocks) b = kPrepMaxBlocks;
+  return b;
+}
+
+struct ZeroRegion {
+  uint32_t* p;
+  uint64_t n;
+  uint32_t v;
+};
+struct ZeroList {
+  ZeroRegion r[6];
+  int count;
+};
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = sel_min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = sel_max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Centroid (poly: accumulate(vertices, zeros) / nv) and the cardinal slab
+// bounds; block partials; the last block finalizes min / inv_diag / max.
+template <int KIND, int NV>
+__global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, ZeroList zl) {
+  const uint64_t n = in.n;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // clear the scratch the following stages accumulate into
+  for (int r = 0; r < zl.count; ++r)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < zl.r[r].n; i += stride)
+      zl.r[r].p[i] = zl.r[r].v;
+
+  double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+  double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    double lo[3], hi[3];
+    if constexpr (KIND == kPoly) {
+      double v[NV][3];
+      double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          v[j][c] = in.data[(uint64_t)(3 * j + c) * n + e];
+          s[c] = s[c] + v[j][c];
+        }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) b.centroid[(uint64_t)c * n + e] = s[c] / (double)NV;
+      // cardinal slabs of from_vertices (axes 0..2)
+      double p0 = project<0>(v[0][0], v[0][1], v[0][2]);
+      double p1 = project<1>(v[0][0], v[0][1], v[0][2]);
+      double p2 = project<2>(v[0][0], v[0][1], v[0][2]);
+      lo[0] = p0 - in.eps; hi[0] = p0 + in.eps;
+      lo[1] = p1 - in.eps; hi[1] = p1 + in.eps;
+      lo[2] = p2 - in.eps; hi[2] = p2 + in.eps;
+#pragma unroll
+      for (int j = 1; j < NV; ++j) {
+        p0 = project<0>(v[j][0], v[j][1], v[j][2]);
+        p1 = project<1>(v[j][0], v[j][1], v[j][2]);
+        p2 = project<2>(v[j][0], v[j][1], v[j][2]);
+        lo[0] = sel_min(lo[0], p0 - in.eps); hi[0] = sel_max(hi[0], p0 + in.eps);
+        lo[1] = sel_min(lo[1], p1 - in.eps); hi[1] = sel_max(hi[1], p1 + in.eps);
+        lo[2] = sel_min(lo[2], p2 - in.eps); hi[2] = sel_max(hi[2], p2 + in.eps);
+      }
+    } else if constexpr (KIND == kSphere) {
+      const double x = in.data[e], y = in.data[n + e], z = in.data[2 * n + e], r = in.data[3 * n + e];
+      double p0 = project<0>(x, y, z), p1 = project<1>(x, y, z), p2 = project<2>(x, y, z);
+      lo[0] = p0 - r; hi[0] = p0 + r;
+      lo[1] = p1 - r; hi[1] = p1 + r;
+      lo[2] = p2 - r; hi[2] = p2 + r;
+    } else {
+      const double* k = in.data + e * kSlabs;
+      lo[0] = k[0]; hi[0] = k[1];
+      lo[1] = k[2]; hi[1] = k[3];
+      lo[2] = k[4]; hi[2] = k[5];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = (lo[c] < mn[c]) ? lo[c] : mn[c];
+      mx[c] = (mx[c] < hi[c]) ? hi[c] : mx[c];
+    }
+  }
+  __shared__ double s_part[4][6];
+  __shared__ bool s_last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double a = wave_min(mn[c]);
+    double z = wave_max(mx[c]);
+    if (lane == 0) {
+      s_part[w][c] = a;
+      s_part[w][3 + c] = z;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[6];
+    for (int c = 0; c < 6; ++c) r[c] = s_part[0][c];
+    for (int ww = 1; ww < kPrepBlock / 64; ++ww)
+      for (int c = 0; c < 3; ++c) {
+        r[c] = sel_min(r[c], s_part[ww][c]);
+        r[3 + c] = sel_max(r[3 + c], s_part[ww][3 + c]);
+      }
+    for (int c = 0; c < 6; ++c) b.partials[blockIdx.x * 6 + c] = r[c];
+    __threadfence();
+    unsigned prev = atomicAdd(b.done_counter, 1u);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    double r[6];
+    for (int c = 0; c < 6; ++c) r[c] = __hip_atomic_load(&b.partials[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned blk = 1; blk < gridDim.x; ++blk)
+      for (int c = 0; c < 3; ++c) {
+        double a = __hip_atomic_load(&b.partials[blk * 6 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        double z = __hip_atomic_load(&b.partials[blk * 6 + 3 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r[c] = sel_min(r[c], a);
+        r[3 + c] = sel_max(r[3 + c], z);
+      }
+    Bounds bd;
+    for (int c = 0; c < 3; ++c) {
+      bd.min[c] = r[c];
+      bd.max[c] = r[3 + c];
+      double width = r[3 + c] - r[c];
+      bd.inv[c] = 1.0 / width;
+    }
+    bd.pad = 0.0;
+    *b.bounds = bd;
+    *b.done_counter = 0;  // ready for the next build
+  }
+}
+
+// Morton codes in input order + digit histograms of every sort pass.
+template <int KIND, int BITS>
+__global__ __launch_bounds__(256) void k_morton(ElemInput in, BuildBufs b) {
+  constexpr int NP = (BITS == 63) ? 8 : 4;
+  __shared__ uint32_t sh[NP * 256];
+  for (int i = threadIdx.x; i < NP * 256; i += blockDim.x) sh[i] = 0;
+  __syncthreads();
+  const Bounds bd = *b.bounds;
+  const uint64_t n = in.n;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    double x, y, z;
+    if constexpr (KIND == kPoly) {
+      x = b.centroid[e]; y = b.centroid[n + e]; z = b.centroid[2 * n + e];
+    } else if constexpr (KIND == kSphere) {
+      x = in.data[e]; y = in.data[n + e]; z = in.data[2 * n + e];
+    } else {
+      x = in.centroids[3 * e]; y = in.centroids[3 * e + 1]; z = in.centroids[3 * e + 2];
+    }
+    uint64_t code;
+    if constexpr (BITS == 63) {
+      code = morton63(x, y, z, bd);
+      ((uint64_t*)b.codes0)[e] = code;
+    } else {
+      uint32_t c32 = morton30(x, y, z, bd);
+      ((uint32_t*)b.codes0)[e] = c32;
+      code = c32;
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const uint32_t d = (uint32_t)((code >> (8 * p)) & 255);
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      if (__all(d == d0)) {
+        if (lane_id() == (uint32_t)(__ffsll(__ballot(1)) - 1)) atomicAdd(&sh[p * 256 + d0], (uint32_t)__popcll(__ballot(1)));
+      } else {
+        atomicAdd(&sh[p * 256 + d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NP * 256; i += blockDim.x)
+    if (sh[i]) atomicAdd(&b.hist[i], sh[i]);
+}
+
+// depth(i) = clz(h[i] ^ h[i+1]) (255 for equal codes; the reference uses
+// UINT_MAX, split/cluster.hpp:148) + histogram of the single 8-bit digit.
+template <typename K>
+__global__ __launch_bounds__(256) void k_cluster_depth(const K* __restrict__ codes, uint64_t n,
+                                                       uint32_t* __restrict__ depth, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t sh[256];
+  sh[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t m = n - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const K x = codes[i] ^ codes[i + 1];
+    uint32_t d;
+    if constexpr (sizeof(K) == 8) d = x ? (uint32_t)__clzll((long long)x) : 255u;
+    else d = x ? (uint32_t)__clz((int)x) : 255u;
+    depth[i] = d;
+    atomicAdd(&sh[d], 1u);
+  }
+  __syncthreads();
+  if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+}
+
+// The first c entries of the (depth, index) order, sorted by position
+// (split/cluster.hpp:157-168), by a bitonic sort in LDS; remaining split slots
+// take n - 1 (split/cluster.hpp:101-116).
+constexpr int kSelectMax = 16384;
+__global__ __launch_bounds__(1024) void k_cluster_select(const uint32_t* __restrict__ depth_idx, uint64_t n,
+                                                         uint64_t nsplits, uint32_t* __restrict__ splits) {
+  __shared__ uint32_t s[kSelectMax];
+  const uint64_t c = (n == 0) ? 0 : ((nsplits < n - 1) ? nsplits : n - 1);
+  uint32_t m = 1;
+  while (m < c) m <<= 1;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) s[i] = (i < c) ? depth_idx[i] : 0xffffffffu;
+  __syncthreads();
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint32_t a = s[i], bb = s[ixj];
+          if ((a > bb) == up) {
+            s[i] = bb;
+            s[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const uint32_t fillv = (n == 0) ? 0u : (uint32_t)(n - 1);
+  for (uint64_t i = threadIdx.x; i < nsplits; i += blockDim.x) splits[i] = (i < c) ? s[i] : fillv;
+}
+
+__device__ __forceinline__ uint32_t patch_of(const uint32_t* __restrict__ splits, uint32_t ns, uint32_t i) {
+  // number of split values <= i (patch k covers [splits[k-1], splits[k]))
+  uint32_t lo = 0, hi = ns;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (splits[mid] <= i) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t msb32(uint32_t x) { return 31u - (uint32_t)__clz((int)x); }
+__device__ __forceinline__ uint32_t msb64(uint64_t x) { return 63u - (uint32_t)__clzll((long long)x); }
+
+// patch of every sorted position, the split "height" delta(i) of the
+// augmented key between positions i and i+1, zeroed arrival counters.
+template <typename K>
+__global__ __launch_bounds__(256) void k_delta(const K* __restrict__ codes, uint64_t n, const uint32_t* __restrict__ splits,
+                                               uint32_t nsplits, uint64_t od, BuildBufs b) {
+  const uint64_t total = n > od ? n : od;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i < od) b.patch_root[i] = kEnd;
+    if (i >= n) continue;
+    const uint32_t pi = patch_of(splits, nsplits, (uint32_t)i);
+    b.patch_of_pos[i] = pi;
+    if (i + 1 < n) {
+      b.counters[i] = 0;
+      const uint32_t pj = patch_of(splits, nsplits, (uint32_t)(i + 1));
+      uint32_t d;
+      if (pi != pj) {
+        d = 128u + msb32(pi ^ pj);
+      } else {
+        const uint64_t x = (uint64_t)(codes[i] ^ codes[i + 1]);
+        d = x ? 64u + msb64(x) : msb32((uint32_t)i ^ (uint32_t)(i + 1));
+      }
+      b.delta[i] = (uint8_t)d;
+    }
+  }
+}
+
+// next representable float below / above a finite f (IEEE-754 bit stepping)
+__device__ __forceinline__ float next_down(float f) {
+  if (!(f == f) || f == -__builtin_inff()) return f;
+  if (f == 0.0f) return __uint_as_float(0x80000001u);
+  const uint32_t u = __float_as_uint(f);
+  return __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
+}
+__device__ __forceinline__ float next_up(float f) {
+  if (!(f == f) || f == __builtin_inff()) return f;
+  if (f == 0.0f) return __uint_as_float(0x00000001u);
+  const uint32_t u = __float_as_uint(f);
+  return __uint_as_float(f > 0.0f ? u + 1u : u - 1u);
+}
+__device__ __forceinline__ float round_down_1ulp(double x) { return next_down((float)x); }
+__device__ __forceinline__ float round_up_1ulp(double x) { return next_up((float)x); }
+
+template <int KIND, int NV>
+__device__ __forceinline__ void element_kdop(const ElemInput& in, uint32_t e, double* kd) {
+  const uint64_t n = in.n;
+  if constexpr (KIND == kPoly) {
+    double v[NV][3];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[j][c] = in.data[(uint64_t)(3 * j + c) * n + e];
+    kdop_from_vertices<NV>(v, in.eps, kd);
+  } else if constexpr (KIND == kSphere) {
+    kdop_from_sphere(in.data[e], in.data[n + e], in.data[2 * n + e], in.data[3 * n + e], kd);
+  } else {
+    const double2* k = (const double2*)(in.data + (uint64_t)e * kSlabs);
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      double2 t = k[a];
+      kd[2 * a] = t.x;
+      kd[2 * a + 1] = t.y;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_slabs(Node64* nd, const double* kd) {
+  double2* d = (double2*)nd->s;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) d[a] = make_double2(kd[2 * a], kd[2 * a + 1]);
+}
+__device__ __forceinline__ void load_slabs(const Node64* nd, double* kd) {
+  const double2* d = (const double2*)nd->s;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    double2 t = d[a];
+    kd[2 * a] = t.x;
+    kd[2 * a + 1] = t.y;
+  }
+}
+
+// fp32 outward slabs relative to the object origin: the fp64 slab minus the
+// origin's projection (fp64), rounded to nearest fp32 and moved one ulp out.
+__device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    kf[2 * a] = round_down_1ulp(kd[2 * a] - po[a]);
+    kf[2 * a + 1] = round_up_1ulp(kd[2 * a + 1] - po[a]);
+  }
+}
+
+__device__ __forceinline__ void store_slabs32(Node32* nd, const float* kf) {
+  float4* d = (float4*)nd->s;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) d[q] = make_float4(kf[4 * q], kf[4 * q + 1], kf[4 * q + 2], kf[4 * q + 3]);
+  nd->s[24] = kf[24];
+  nd->s[25] = kf[25];
+}
+__device__ __forceinline__ void load_slabs32(const Node32* nd, float* kf) {
+  const float4* d = (const float4*)nd->s;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    float4 t = d[q];
+    kf[4 * q] = t.x; kf[4 * q + 1] = t.y; kf[4 * q + 2] = t.z; kf[4 * q + 3] = t.w;
+  }
+  kf[24] = nd->s[24];
+  kf[25] = nd->s[25];
+}
+
+__device__ __forceinline__ uint32_t patch_begin(const uint32_t* splits, uint64_t od, uint32_t k) {
+  return k == 0 ? 0u : splits[k - 1];
+}
+__device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t od, uint64_t n, uint32_t k) {
+  return (uint64_t)k == od - 1 ? (uint32_t)n : splits[k];
+}
+
+// Leaf slabs (gathered in sorted order) + bottom-up build with refit.  Each
+// thread starts at its leaf and climbs: a node [l, r] is the left child of
+// internal node r when delta(r) < delta(l-1) (or l == 0), else the right
+// child of internal node l-1.  The agent-scope acq_rel arrival counter makes
+// the first arriver's node visible to the second, which merges both slab sets
+// and continues; the first arriver stops.
+template <int KIND, int NV, bool FP32>
+__global__ __launch_bounds__(256) void k_tree(ElemInput in, BuildBufs b, uint64_t od) {
+  const uint64_t n = in.n;
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t e = b.perm[p];
+  if (e >= n) {
+    atomicOr(b.status, kErrIndex);
+    return;
+  }
+  double kd[kSlabs];
+  element_kdop<KIND, NV>(in, e, kd);
+  const uint32_t pk = b.patch_of_pos[p];
+  b.patch_of_input[e] = pk;
+  const uint32_t leaf = (uint32_t)(n - 1 + p);
+  double po[kAxes];
+  float kf[kSlabs];
+  if constexpr (FP32) {
+    const double ox = b.origin[0], oy = b.origin[1], oz = b.origin[2];
+    proj_all(ox, oy, oz, po);
+    to_fp32_outward(kd, po, kf);
+    Node32* nd = &b.nodes32[leaf];
+    store_slabs32(nd, kf);
+    nd->left = e; nd->right = kEnd; nd->first = (uint32_t)p; nd->last = (uint32_t)p; nd->patch = pk;
+  } else {
+    Node64* nd = &b.nodes[leaf];
+    store_slabs(nd, kd);
+    nd->left = e; nd->right = kEnd; nd->first = (uint32_t)p; nd->last = (uint32_t)p; nd->patch = pk;
+  }
+  const uint32_t* splits = b.splits;
+  if (patch_begin(splits, od, pk) == p && patch_end(splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
+  if (n == 1) {
+    *b.root = leaf;
+    return;
+  }
+  uint32_t node = leaf, l = (uint32_t)p, r = (uint32_t)p, mypatch = pk;
+  for (;;) {
+    if (l == 0 && r == n - 1) {
+      *b.root = node;
+      return;
+    }
+    const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
+    const uint32_t parent = is_left ? r : l - 1;
+    if (FP32) {
+      if (is_left) b.nodes32[parent].left = node;
+      else b.nodes32[parent].right = node;
+    } else {
+      if (is_left) b.nodes[parent].left = node;
+      else b.nodes[parent].right = node;
+    }
+    const uint32_t old = __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;  // first arriver: the sibling finishes the parent
+    uint32_t sib, sfirst, slast, spatch;
+    if constexpr (FP32) {
+      const Node32* pn = &b.nodes32[parent];
+      sib = is_left ? pn->right : pn->left;
+      if (sib >= 2 * n - 1) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      const Node32* sn = &b.nodes32[sib];
+      float ks[kSlabs];
+      load_slabs32(sn, ks);
+#pragma unroll
+      for (int i = 0; i < kAxes; ++i) {
+        kf[2 * i] = (kf[2 * i] < ks[2 * i]) ? kf[2 * i] : ks[2 * i];
+        kf[2 * i + 1] = (kf[2 * i + 1] > ks[2 * i + 1]) ? kf[2 * i + 1] : ks[2 * i + 1];
+      }
+      sfirst = sn->first; slast = sn->last; spatch = sn->patch;
+    } else {
+      const Node64* pn = &b.nodes[parent];
+      sib = is_left ? pn->right : pn->left;
+      if (sib >= 2 * n - 1) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      const Node64* sn = &b.nodes[sib];
+      double ks[kSlabs];
+      load_slabs(sn, ks);
+      kdop_union(kd, ks);
+      sfirst = sn->first; slast = sn->last; spatch = sn->patch;
+    }
+    if (is_left) r = slast;
+    else l = sfirst;
+    mypatch = (mypatch == spatch) ? mypatch : kMixed;
+    if constexpr (FP32) {
+      Node32* nd = &b.nodes32[parent];
+      store_slabs32(nd, kf);
+      nd->first = l; nd->last = r; nd->patch = mypatch;
+    } else {
+      Node64* nd = &b.nodes[parent];
+      store_slabs(nd, kd);
+      nd->first = l; nd->last = r; nd->patch = mypatch;
+    }
+    if (mypatch != kMixed && patch_begin(splits, od, mypatch) == l && patch_end(splits, od, n, mypatch) == r + 1)
+      b.patch_root[mypatch] = parent;
+    node = parent;
+  }
+}
+
+// rope(node [l, r]) = right child of internal node r (or end when r = n-1):
+// the next subtree in depth-first order once this one is done.
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_ropes(BuildBufs b, uint64_t n) {
+  const uint64_t total = 2 * n - 1;
+  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (uint64_t)gridDim.x * blockDim.x) {
+    if constexpr (FP32) {
+      const uint32_t last = b.nodes32[id].last;
+      uint32_t rope = kEnd;
+      if (last < n - 1) rope = b.nodes32[last].right;
+      else if (last != n - 1) atomicOr(b.status, kErrTree);
+      b.nodes32[id].rope = rope < 2 * n - 1 ? rope : kEnd;
+    } else {
+      const uint32_t last = b.nodes[id].last;
+      uint32_t rope = kEnd;
+      if (last < n - 1) rope = b.nodes[last].right;
+      else if (last != n - 1) atomicOr(b.status, kErrTree);
+      b.nodes[id].rope = rope < 2 * n - 1 ? rope : kEnd;
+    }
+  }
+}
+
+// ---- launchers ---------------------------------------------------------------------
+template <int KIND, int NV>
+static void prep_dispatch(const ElemInput& in, const BuildBufs& b, const ZeroList& zl, hipStream_t s) {
+  hipLaunchKernelGGL((k_prep<KIND, NV>), dim3((unsigned)prep_blocks(in.n)), dim3(kPrepBlock), 0, s, in, b, zl);
+}
+
+hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s) {
+  (void)in;
+  ZeroList zl{};
+  zl.count = 0;
+  // b.hist holds the Morton histograms (8*256) followed by the depth
+  // histogram (256)
+  zl.r[zl.count++] = {b.hist, 9 * 256, 0u};
+  // look-back words of the Morton sort (8 passes) and of the depth sort
+  const uint64_t lb_words = (8 + 1) * sort_tiles(in.n > 0 ? in.n : 1) * 256;
+  zl.r[zl.count++] = {b.lookback, lb_words, 0u};
+  if (in.kind == kPoly) {
+    switch (in.nv) {
+      case 1: prep_dispatch<kPoly, 1>(in, b, zl, s); break;
+      case 2: prep_dispatch<kPoly, 2>(in, b, zl, s); break;
+      case 3: prep_dispatch<kPoly, 3>(in, b, zl, s); break;
+      case 4: prep_dispatch<kPoly, 4>(in, b, zl, s); break;
+      case 5: prep_dispatch<kPoly, 5>(in, b, zl, s); break;
+      case 6: prep_dispatch<kPoly, 6>(in, b, zl, s); break;
+      case 7: prep_dispatch<kPoly, 7>(in, b, zl, s); break;
+      case 8: prep_dispatch<kPoly, 8>(in, b, zl, s); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else if (in.kind == kSphere) {
+    prep_dispatch<kSphere, 1>(in, b, zl, s);
+  } else {
+    prep_dispatch<kSnapshot, 1>(in, b, zl, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_morton(const ElemInput& in, const BuildBufs& b, int bits, hipStream_t s) {
+  if (in.n == 0) return hipSuccess;
+  uint64_t blocks = (in.n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  const dim3 g((unsigned)blocks), t(256);
+  if (bits == 63) {
+    if (in.kind == kPoly) hipLaunchKernelGGL((k_morton<kPoly, 63>), g, t, 0, s, in, b);
+    else if (in.kind == kSphere) hipLaunchKernelGGL((k_morton<kSphere, 63>), g, t, 0, s, in, b);
+    else hipLaunchKernelGGL((k_morton<kSnapshot, 63>), g, t, 0, s, in, b);
+  } else {
+    if (in.kind == kPoly) hipLaunchKernelGGL((k_morton<kPoly, 30>), g, t, 0, s, in, b);
+    else if (in.kind == kSphere) hipLaunchKernelGGL((k_morton<kSphere, 30>), g, t, 0, s, in, b);
+    else hipLaunchKernelGGL((k_morton<kSnapshot, 30>), g, t, 0, s, in, b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsplits, hipStream_t s) {
+  if (nsplits == 0) return hipSuccess;
+  if (nsplits > (uint64_t)kSelectMax) return hipErrorInvalidValue;
+  hipError_t e;
+  if (n >= 2 && nsplits > 0) {
+    const uint64_t m = n - 1;
+    uint64_t blocks = (m + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    uint32_t* hist = b.hist + 8 * 256;
+    if (bits == 63)
+      hipLaunchKernelGGL(k_cluster_depth<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, s,
+                         (const uint64_t*)b.codes_a, n, b.depth0, hist);
+    else
+      hipLaunchKernelGGL(k_cluster_depth<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, s,
+                         (const uint32_t*)b.codes_a, n, b.depth0, hist);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    SortArgs<uint32_t> a{};
+    a.k0 = b.depth0; a.ka = b.depth_a; a.kb = b.depth_b;
+    a.v0 = nullptr; a.va = b.depth_idx; a.vb = b.depth_idx_b;
+    a.n = m; a.ntiles = sort_tiles(m);
+    a.ctl = b.ctl + 1;
+    a.lookback = b.lookback + 8 * sort_tiles(n) * 256;
+    e = launch_sort_passes<uint32_t>(a, 1, hist, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_cluster_select, dim3(1), dim3(1024), 0, s, (const uint32_t*)b.depth_idx, n, nsplits, b.splits);
+  return hipGetLastError();
+}
+
+template <int KIND, int NV>
+static void tree_dispatch(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
+  const unsigned blocks = (unsigned)((in.n + 255) / 256);
+  if (fp32) hipLaunchKernelGGL((k_tree<KIND, NV, true>), dim3(blocks), dim3(256), 0, s, in, b, od);
+  else hipLaunchKernelGGL((k_tree<KIND, NV, false>), dim3(blocks), dim3(256), 0, s, in, b, od);
+}
+
+hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, int fp32, hipStream_t s) {
+  const uint64_t n = in.n;
+  if (n == 0) return hipSuccess;
+  const uint64_t total = n > od ? n : od;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  const uint32_t nsplits = (uint32_t)(od - 1);
+  if (bits == 63)
+    hipLaunchKernelGGL(k_delta<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       (const uint64_t*)b.codes_a, n, (const uint32_t*)b.splits, nsplits, od, b);
+  else
+    hipLaunchKernelGGL(k_delta<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       (const uint32_t*)b.codes_a, n, (const uint32_t*)b.splits, nsplits, od, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (in.kind == kPoly) {
+    switch (in.nv) {
+      case 1: tree_dispatch<kPoly, 1>(in, b, od, fp32, s); break;
+      case 2: tree_dispatch<kPoly, 2>(in, b, od, fp32, s); break;
+      case 3: tree_dispatch<kPoly, 3>(in, b, od, fp32, s); break;
+      case 4: tree_dispatch<kPoly, 4>(in, b, od, fp32, s); break;
+      case 5: tree_dispatch<kPoly, 5>(in, b, od, fp32, s); break;
+      case 6: tree_dispatch<kPoly, 6>(in, b, od, fp32, s); break;
+      case 7: tree_dispatch<kPoly, 7>(in, b, od, fp32, s); break;
+      case 8: tree_dispatch<kPoly, 8>(in, b, od, fp32, s); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else if (in.kind == kSphere) {
+    tree_dispatch<kSphere, 1>(in, b, od, fp32, s);
+  } else {
+    tree_dispatch<kSnapshot, 1>(in, b, od, fp32, s);
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  uint64_t rb = (2 * n - 1 + 255) / 256;
+  if (rb > 4096) rb = 4096;
+  if (fp32) hipLaunchKernelGGL(k_ropes<true>, dim3((unsigned)rb), dim3(256), 0, s, b, n);
+  else hipLaunchKernelGGL(k_ropes<false>, dim3((unsigned)rb), dim3(256), 0, s, b, n);
+  return hipGetLastError();
+}
+
+}  // namespace dbvh

@@ -1,0 +1,898 @@
+// capi.hip -- the C ABI (include/distbvh.h): worlds, collision objects and
+// the stream-ordered orchestration of the device stages.
+//
+// Reference call stacks this replaces (single process, one GPU):
+//  * collision_object::set_entity_data_clustering (collision_object.hpp:89-131)
+//    + set_entity_data_impl (collision_object.cpp:108-146)
+//      -> prep -> morton -> sort -> cluster -> tree
+//  * init_broadphase (collision_object.cpp:148-215): the patch tree is part of
+//    the object tree, so this only validates state
+//  * broadphase + narrowphase (collision_object.cpp:248-397)
+//      -> patch broadphase kernel + element narrowphase kernel, results in
+//         the initiating object (impl.cpp:328-348)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/distbvh.h"
+#include "kernels.hpp"
+
+using namespace dbvh;
+
+namespace {
+
+thread_local std::string g_err;
+
+bvh_status fail(bvh_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return fail(_e == hipErrorOutOfMemory ? BVH_E_OOM : BVH_E_HIP,                        \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                        \
+  } while (0)
+
+#define BVH_TRY(expr)                    \
+  do {                                   \
+    bvh_status _s = (expr);              \
+    if (_s != BVH_OK) return _s;         \
+  } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  bvh_status ensure(size_t want) {
+    if (want <= bytes && p) return BVH_OK;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    size_t b = want ? want : 256;
+    b = (b + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(BVH_E_OOM, "hipMalloc(" + std::to_string(b) + "): " + hipGetErrorString(e));
+    }
+    bytes = b;
+    return BVH_OK;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Stage {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct bvh_object_s;
+
+struct bvh_world_s {
+  size_t od = 1;
+  bvh_world_config cfg{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<std::unique_ptr<bvh_object_s>> objects;
+  // timing
+  std::vector<Stage> stages;
+  std::vector<hipEvent_t> event_pool;
+  size_t events_used = 0;
+  // primitive-call scratch
+  DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
+  unsigned long long* h_count = nullptr;  // pinned
+  bool iteration_open = false;
+
+  hipEvent_t take_event() {
+    if (events_used == event_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      event_pool.push_back(e);
+    }
+    return event_pool[events_used++];
+  }
+  void reset_stages() {
+    stages.clear();
+    events_used = 0;
+  }
+  void stage_begin(const char* name) {
+    if (!(cfg.flags & BVH_FLAG_TIMING)) return;
+    Stage s{name, take_event(), take_event()};
+    if (s.a && s.b) {
+      (void)hipEventRecord(s.a, stream);
+      stages.push_back(s);
+    }
+  }
+  void stage_end() {
+    if (!(cfg.flags & BVH_FLAG_TIMING) || stages.empty()) return;
+    (void)hipEventRecord(stages.back().b, stream);
+  }
+};
+
+struct bvh_object_s {
+  bvh_world w = nullptr;
+  size_t id = 0;
+  // input
+  ElemInput in{};
+  bool have_data = false;
+  bool built = false;
+  bool has_gids = false;
+  DevBuf input;  // uploaded host elements (or snapshot kdops)
+  DevBuf input_c;  // uploaded snapshot centroids
+  DevBuf gids;
+  // build buffers
+  DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
+      lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
+      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status;
+  BuildBufs bb{};
+  int bits = 63;
+  int fp32 = 0;
+  // results
+  DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count;
+  uint64_t npairs = 0;
+  uint64_t npatch_pairs = 0;
+  const uint64_t* result_ptr = nullptr;  // device pointer to the (sorted) pairs
+  bvh_object_s* result_other = nullptr;
+  bool self_result = false;
+  bool have_results = false;
+};
+
+// ---- helpers ---------------------------------------------------------------------------
+static TreeView view_of(const bvh_object_s* o) {
+  TreeView v{};
+  v.nodes = o->nodes.as<Node64>();
+  v.nodes32 = o->nodes32.as<Node32>();
+  v.root = o->root.as<uint32_t>();
+  v.patch_root = o->patch_root.as<uint32_t>();
+  v.patch_of_input = o->patch_of_input.as<uint32_t>();
+  v.perm = o->perm.as<uint32_t>();
+  v.n = o->in.n;
+  v.od = o->w->od;
+  v.status = o->status.as<uint32_t>();
+  return v;
+}
+
+static bool is_pow2(size_t v) { return v && !(v & (v - 1)); }
+
+static bvh_status copy_out(void* dst, const void* src, size_t bytes, uint32_t mem, hipStream_t s) {
+  if (bytes == 0) return BVH_OK;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, mem == BVH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+static bvh_status object_alloc(bvh_object_s* o) {
+  const uint64_t n = o->in.n;
+  const uint64_t od = o->w->od;
+  const uint64_t n1 = n > 0 ? n : 1;
+  const uint64_t tiles = sort_tiles(n1);
+  const size_t kw = o->bits == 63 ? 8 : 4;
+  BVH_TRY(o->centroid.ensure(o->in.kind == kPoly ? 3 * n1 * sizeof(double) : 256));
+  BVH_TRY(o->partials.ensure(prep_blocks(n1) * 6 * sizeof(double)));
+  BVH_TRY(o->bounds.ensure(sizeof(Bounds)));
+  BVH_TRY(o->codes0.ensure(n1 * kw));
+  BVH_TRY(o->codes_a.ensure(n1 * kw));
+  BVH_TRY(o->codes_b.ensure(n1 * kw));
+  BVH_TRY(o->perm.ensure(n1 * 4));
+  BVH_TRY(o->perm_b.ensure(n1 * 4));
+  BVH_TRY(o->hist.ensure(9 * 256 * 4));
+  BVH_TRY(o->ctl.ensure(2 * sizeof(SortCtl)));
+  BVH_TRY(o->lookback.ensure(9 * tiles * 256 * 4));
+  BVH_TRY(o->depth0.ensure(n1 * 4));
+  BVH_TRY(o->depth_a.ensure(n1 * 4));
+  BVH_TRY(o->depth_b.ensure(n1 * 4));
+  BVH_TRY(o->depth_idx.ensure(n1 * 4));
+  BVH_TRY(o->depth_idx_b.ensure(n1 * 4));
+  BVH_TRY(o->splits.ensure((od > 1 ? od - 1 : 1) * 4));
+  BVH_TRY(o->delta.ensure(n1));
+  BVH_TRY(o->patch_of_pos.ensure(n1 * 4));
+  BVH_TRY(o->patch_of_input.ensure(n1 * 4));
+  BVH_TRY(o->counters.ensure(n1 * 4));
+  BVH_TRY(o->patch_root.ensure(od * 4));
+  BVH_TRY(o->root.ensure(4));
+  BVH_TRY(o->status.ensure(2 * sizeof(uint32_t)));  // [error bits, last-block counter]
+  if (o->fp32) {
+    BVH_TRY(o->nodes32.ensure((2 * n1 - 1) * sizeof(Node32)));
+    BVH_TRY(o->origin.ensure(3 * sizeof(double)));
+  } else {
+    BVH_TRY(o->nodes.ensure((2 * n1 - 1) * sizeof(Node64)));
+  }
+  BuildBufs& b = o->bb;
+  b.centroid = o->centroid.as<double>();
+  b.partials = o->partials.as<double>();
+  b.done_counter = o->status.as<uint32_t>() + 1;
+  b.bounds = o->bounds.as<Bounds>();
+  b.codes0 = o->codes0.p;
+  b.codes_a = o->codes_a.p;
+  b.codes_b = o->codes_b.p;
+  b.perm = o->perm.as<uint32_t>();
+  b.perm_b = o->perm_b.as<uint32_t>();
+  b.hist = o->hist.as<uint32_t>();
+  b.ctl = o->ctl.as<SortCtl>();
+  b.lookback = o->lookback.as<uint32_t>();
+  b.depth0 = o->depth0.as<uint32_t>();
+  b.depth_a = o->depth_a.as<uint32_t>();
+  b.depth_b = o->depth_b.as<uint32_t>();
+  b.depth_idx = o->depth_idx.as<uint32_t>();
+  b.depth_idx_b = o->depth_idx_b.as<uint32_t>();
+  b.splits = o->splits.as<uint32_t>();
+  b.delta = o->delta.as<uint8_t>();
+  b.patch_of_pos = o->patch_of_pos.as<uint32_t>();
+  b.patch_of_input = o->patch_of_input.as<uint32_t>();
+  b.counters = o->counters.as<uint32_t>();
+  b.patch_root = o->patch_root.as<uint32_t>();
+  b.root = o->root.as<uint32_t>();
+  b.nodes = o->fp32 ? nullptr : o->nodes.as<Node64>();
+  b.nodes32 = o->fp32 ? o->nodes32.as<Node32>() : nullptr;
+  b.origin = o->fp32 ? o->origin.as<double>() : nullptr;
+  b.status = o->status.as<uint32_t>();
+  return BVH_OK;
+}
+
+// The whole object build, stream ordered, no host synchronization.
+static bvh_status object_build(bvh_object_s* o) {
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  const uint64_t n = o->in.n;
+  BVH_TRY(object_alloc(o));
+  HIP_TRY(launch_fill_u32(o->bb.status, 2, 0u, s));  // error bits and the bounds counter
+  if (n == 0) {
+    // splits are all 0 (split/cluster.hpp:101-105), every patch empty
+    HIP_TRY(launch_fill_u32(o->bb.splits, w->od > 1 ? w->od - 1 : 0, 0u, s));
+    HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
+    HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
+    o->built = true;
+    return BVH_OK;
+  }
+  w->stage_begin("bounds");
+  HIP_TRY(launch_prep(o->in, o->bb, s));
+  w->stage_end();
+  w->stage_begin("morton");
+  HIP_TRY(launch_morton(o->in, o->bb, o->bits, s));
+  w->stage_end();
+  w->stage_begin("sort");
+  const int npass = o->bits == 63 ? 8 : 4;
+  if (o->bits == 63) {
+    SortArgs<uint64_t> a{};
+    a.k0 = (const uint64_t*)o->bb.codes0; a.ka = (uint64_t*)o->bb.codes_a; a.kb = (uint64_t*)o->bb.codes_b;
+    a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
+    a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
+    HIP_TRY(launch_sort_passes<uint64_t>(a, npass, o->bb.hist, s));
+  } else {
+    SortArgs<uint32_t> a{};
+    a.k0 = (const uint32_t*)o->bb.codes0; a.ka = (uint32_t*)o->bb.codes_a; a.kb = (uint32_t*)o->bb.codes_b;
+    a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
+    a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
+    HIP_TRY(launch_sort_passes<uint32_t>(a, npass, o->bb.hist, s));
+  }
+  w->stage_end();
+  w->stage_begin("cluster");
+  HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
+  w->stage_end();
+  if (o->fp32) {
+    // object origin for the fp32 slabs: the Morton bounds minimum (fp64)
+    HIP_TRY(hipMemcpyAsync(o->bb.origin, o->bb.bounds, 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  w->stage_begin("tree");
+  HIP_TRY(launch_tree(o->in, o->bb, o->bits, w->od, o->fp32, s));
+  w->stage_end();
+  o->built = true;
+  return BVH_OK;
+}
+
+static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double eps, const double* data,
+                                    const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
+                                    uint32_t split) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  if (split != BVH_SPLIT_CLUSTERING && split != BVH_SPLIT_GEOM_AXIS && split != BVH_SPLIT_ML_GEOM_AXIS)
+    return fail(BVH_E_INVALID, "unknown split algorithm");
+  if (split != BVH_SPLIT_CLUSTERING)
+    return fail(BVH_E_UNSUPPORTED, "only split_algorithm::clustering is built on the GPU path");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
+  if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  w->reset_stages();
+  o->have_results = false;
+  o->in.kind = kind;
+  o->in.nv = nv;
+  o->in.eps = eps;
+  o->in.n = n;
+  o->bits = (int)w->cfg.morton_bits;
+  o->fp32 = w->cfg.precision == BVH_PREC_FP32_OUTWARD ? 1 : 0;
+  size_t planes = kind == kPoly ? (size_t)nv * 3 : (kind == kSphere ? 4 : kSlabs);
+  size_t bytes = planes * n * sizeof(double);
+  if (mem == BVH_MEM_HOST && n > 0) {
+    w->stage_begin("upload");
+    BVH_TRY(o->input.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(o->input.p, data, bytes, hipMemcpyHostToDevice, s));
+    o->in.data = o->input.as<double>();
+    if (kind == kSnapshot) {
+      BVH_TRY(o->input_c.ensure(3 * n * sizeof(double)));
+      HIP_TRY(hipMemcpyAsync(o->input_c.p, cent, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+      o->in.centroids = o->input_c.as<double>();
+    }
+    w->stage_end();
+  } else {
+    // device data is read in place; like the reference's element pointer
+    // (collision_object.cpp:128-129) it must stay valid until the object's
+    // next set_entity_data
+    o->in.data = data;
+    o->in.centroids = cent;
+  }
+  o->has_gids = gids != nullptr;
+  if (gids && n > 0) {
+    BVH_TRY(o->gids.ensure(n * sizeof(uint64_t)));
+    HIP_TRY(hipMemcpyAsync(o->gids.p, gids, n * sizeof(uint64_t),
+                           mem == BVH_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+  }
+  o->have_data = true;
+  o->built = false;
+  return object_build(o);
+}
+
+// Reads the count a kernel left in `dcount` and the device error words of
+// both objects (one synchronization).
+static bvh_status read_count(bvh_object_s* o, bvh_object_s* other, unsigned long long* dcount, uint64_t* out) {
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[1], o->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[2], other->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint32_t st = (uint32_t)w->h_count[1] | (uint32_t)w->h_count[2];
+  if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
+  *out = w->h_count[0];
+  return BVH_OK;
+}
+
+static bvh_status ensure_pairs(bvh_object_s* o, uint64_t cap) {
+  BVH_TRY(o->pairs.ensure(cap * sizeof(uint64_t)));
+  return BVH_OK;
+}
+
+// Stable sort of the result pairs by (element_p, element_q).
+static bvh_status sort_results(bvh_object_s* o) {
+  hipStream_t s = o->w->stream;
+  const uint64_t n = o->npairs;
+  if (n < 2) return BVH_OK;
+  BVH_TRY(o->pairs_a.ensure(n * 8));
+  BVH_TRY(o->pairs_b.ensure(n * 8));
+  BVH_TRY(o->sort_hist.ensure(8 * 256 * 4));
+  BVH_TRY(o->sort_ctl.ensure(sizeof(SortCtl)));
+  const uint64_t tiles = sort_tiles(n);
+  BVH_TRY(o->sort_lb.ensure(8 * tiles * 256 * 4));
+  HIP_TRY(launch_fill_u32(o->sort_hist.as<uint32_t>(), 8 * 256, 0u, s));
+  HIP_TRY(launch_fill_u32(o->sort_lb.as<uint32_t>(), 8 * tiles * 256, 0u, s));
+  HIP_TRY(launch_sort_histogram<uint64_t>(o->pairs.as<uint64_t>(), n, 8, o->sort_hist.as<uint32_t>(), s));
+  SortArgs<uint64_t> a{};
+  a.k0 = o->pairs.as<uint64_t>(); a.ka = o->pairs_a.as<uint64_t>(); a.kb = o->pairs_b.as<uint64_t>();
+  a.v0 = nullptr; a.va = nullptr; a.vb = nullptr;
+  a.n = n; a.ntiles = tiles; a.ctl = o->sort_ctl.as<SortCtl>(); a.lookback = o->sort_lb.as<uint32_t>();
+  HIP_TRY(launch_sort_passes<uint64_t>(a, 8, o->sort_hist.as<uint32_t>(), s));
+  o->result_ptr = o->pairs_a.as<uint64_t>();
+  return BVH_OK;
+}
+
+static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self) {
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
+  if (o->fp32 || other->fp32)
+    return fail(BVH_E_UNSUPPORTED, "fp32-outward queries are not built in this library version");
+  BVH_TRY(o->count.ensure(2 * sizeof(unsigned long long)));
+  unsigned long long* dcount = o->count.as<unsigned long long>();
+  const TreeView qv = view_of(o), tv = view_of(other);
+  // patch broadphase
+  if (!self) {
+    w->stage_begin("broadphase");
+    const uint64_t pcap = std::max<uint64_t>(1024, 4 * w->od);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      uint64_t cap = o->ppairs.bytes / 8;
+      if (cap < pcap) {
+        BVH_TRY(o->ppairs.ensure(pcap * 8));
+        cap = pcap;
+      }
+      HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2, 0u, s));
+      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount};
+      HIP_TRY(launch_patch_broadphase(qv, tv, po, s));
+      BVH_TRY(read_count(o, other, dcount, &o->npatch_pairs));
+      if (o->npatch_pairs <= cap) break;
+      BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024));
+    }
+    w->stage_end();
+  } else {
+    o->npatch_pairs = 0;
+  }
+  // element narrowphase
+  w->stage_begin("narrowphase");
+  uint64_t want = std::max<uint64_t>(1u << 20, 8 * o->in.n);
+  if (o->pairs.bytes / 8 < want) BVH_TRY(ensure_pairs(o, want));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const uint64_t cap = o->pairs.bytes / 8;
+    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2, 0u, s));
+    PairOut po{o->pairs.as<uint64_t>(), cap, dcount};
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, 0, po, s));
+    BVH_TRY(read_count(o, other, dcount, &o->npairs));
+    if (o->npairs <= cap) break;
+    // overflow: grow to the exact count and run again
+    BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+  }
+  w->stage_end();
+  o->result_ptr = o->pairs.as<uint64_t>();
+  if (w->cfg.flags & BVH_FLAG_SORT_RESULTS) {
+    w->stage_begin("sort_results");
+    BVH_TRY(sort_results(o));
+    w->stage_end();
+  }
+  o->result_other = other;
+  o->self_result = self;
+  o->have_results = true;
+  return BVH_OK;
+}
+
+// ---- stand-alone primitives ------------------------------------------------------------------
+template <typename K>
+static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  if (n == 0) return BVH_OK;
+  if (!keys) return fail(BVH_E_INVALID, "null keys");
+  if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 keys");
+  hipStream_t s = w->stream;
+  const int npass = sizeof(K);
+  const uint64_t tiles = sort_tiles(n);
+  BVH_TRY(w->s_ka.ensure(n * sizeof(K)));
+  BVH_TRY(w->s_kb.ensure(n * sizeof(K)));
+  BVH_TRY(w->s_va.ensure(n * 4));
+  BVH_TRY(w->s_vb.ensure(n * 4));
+  BVH_TRY(w->s_hist.ensure(8 * 256 * 4));
+  BVH_TRY(w->s_ctl.ensure(sizeof(SortCtl)));
+  BVH_TRY(w->s_lb.ensure(npass * tiles * 256 * 4));
+  HIP_TRY(launch_fill_u32(w->s_hist.as<uint32_t>(), 8 * 256, 0u, s));
+  HIP_TRY(launch_fill_u32(w->s_lb.as<uint32_t>(), npass * tiles * 256, 0u, s));
+  HIP_TRY(launch_sort_histogram<K>(keys, n, npass, w->s_hist.as<uint32_t>(), s));
+  SortArgs<K> a{};
+  a.k0 = keys; a.ka = w->s_ka.as<K>(); a.kb = w->s_kb.as<K>();
+  a.v0 = vals; a.va = vals ? w->s_va.as<uint32_t>() : nullptr; a.vb = vals ? w->s_vb.as<uint32_t>() : nullptr;
+  a.n = n; a.ntiles = tiles; a.ctl = w->s_ctl.as<SortCtl>(); a.lookback = w->s_lb.as<uint32_t>();
+  HIP_TRY(launch_sort_passes<K>(a, npass, w->s_hist.as<uint32_t>(), s));
+  HIP_TRY(hipMemcpyAsync(keys, a.ka, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+  if (vals) HIP_TRY(hipMemcpyAsync(vals, a.va, n * 4, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+// ============================================================================================
+extern "C" {
+
+const char* bvh_status_string(bvh_status s) {
+  switch (s) {
+    case BVH_OK: return "ok";
+    case BVH_E_INVALID: return "invalid argument";
+    case BVH_E_OOM: return "out of device memory";
+    case BVH_E_OVERFLOW: return "buffer too small";
+    case BVH_E_HIP: return "HIP error";
+    case BVH_E_STATE: return "call out of order";
+    case BVH_E_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown";
+}
+
+const char* bvh_last_error(void) { return g_err.c_str(); }
+uint32_t bvh_abi_version(void) { return BVH_ABI_VERSION; }
+
+void bvh_world_config_init(bvh_world_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->morton_bits = 63;
+  cfg->leaf_size = 1;
+  cfg->precision = BVH_PREC_FP64;
+  cfg->flags = 0;
+  cfg->stream = nullptr;
+  cfg->device = 0;
+}
+
+bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* out) {
+  if (!out) return fail(BVH_E_INVALID, "null output");
+  *out = nullptr;
+  if (!is_pow2(od)) return fail(BVH_E_INVALID, "overdecomposition factor must be a power of 2");
+  if (od > 16384) return fail(BVH_E_INVALID, "overdecomposition factor above 16384");
+  bvh_world_config c;
+  bvh_world_config_init(&c);
+  if (cfg) c = *cfg;
+  if (c.morton_bits != 63 && c.morton_bits != 30) return fail(BVH_E_INVALID, "morton_bits must be 63 or 30");
+  if (c.leaf_size < 1 || c.leaf_size > 8) return fail(BVH_E_INVALID, "leaf_size must be in 1..8");
+  if (c.precision != BVH_PREC_FP64 && c.precision != BVH_PREC_FP32_OUTWARD)
+    return fail(BVH_E_INVALID, "unknown precision");
+  HIP_TRY(hipSetDevice(c.device));
+  auto w = std::make_unique<bvh_world_s>();
+  w->od = od;
+  w->cfg = c;
+  if (c.stream) {
+    w->stream = (hipStream_t)c.stream;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    w->own_stream = true;
+  }
+  void* hc = nullptr;
+  HIP_TRY(hipHostMalloc(&hc, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+  w->h_count = (unsigned long long*)hc;
+  *out = w.release();
+  return BVH_OK;
+}
+
+bvh_status bvh_world_destroy(bvh_world w) {
+  if (!w) return BVH_OK;
+  (void)hipStreamSynchronize(w->stream);
+  w->objects.clear();
+  for (hipEvent_t e : w->event_pool) (void)hipEventDestroy(e);
+  if (w->h_count) (void)hipHostFree(w->h_count);
+  if (w->own_stream) (void)hipStreamDestroy(w->stream);
+  delete w;
+  return BVH_OK;
+}
+
+bvh_status bvh_world_overdecomposition(bvh_world w, size_t* od) {
+  if (!w || !od) return fail(BVH_E_INVALID, "null argument");
+  *od = w->od;
+  return BVH_OK;
+}
+
+bvh_status bvh_create_collision_object(bvh_world w, bvh_object* out) {
+  if (!w || !out) return fail(BVH_E_INVALID, "null argument");
+  auto o = std::make_unique<bvh_object_s>();
+  o->w = w;
+  o->id = w->objects.size();
+  *out = o.get();
+  w->objects.push_back(std::move(o));
+  return BVH_OK;
+}
+
+bvh_status bvh_num_collision_objects(bvh_world w, size_t* n) {
+  if (!w || !n) return fail(BVH_E_INVALID, "null argument");
+  *n = w->objects.size();
+  return BVH_OK;
+}
+
+bvh_status bvh_start_iteration(bvh_world w) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  w->iteration_open = true;
+  return BVH_OK;
+}
+
+bvh_status bvh_finish_iteration(bvh_world w) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  for (auto& o : w->objects) BVH_TRY(bvh_end_phase(o.get()));
+  w->iteration_open = false;
+  HIP_TRY(hipStreamSynchronize(w->stream));
+  return BVH_OK;
+}
+
+bvh_status bvh_synchronize(bvh_world w) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  HIP_TRY(hipStreamSynchronize(w->stream));
+  return BVH_OK;
+}
+
+bvh_status bvh_stage_times(bvh_world w, const char** names, double* ms, size_t cap, size_t* count) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  HIP_TRY(hipStreamSynchronize(w->stream));
+  size_t k = 0;
+  for (const Stage& st : w->stages) {
+    if (k < cap) {
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, st.a, st.b));
+      if (names) names[k] = st.name;
+      if (ms) ms[k] = t;
+    }
+    ++k;
+  }
+  if (count) *count = k;
+  return BVH_OK;
+}
+
+bvh_status bvh_object_id(bvh_object o, size_t* id) {
+  if (!o || !id) return fail(BVH_E_INVALID, "null argument");
+  *id = o->id;
+  return BVH_OK;
+}
+
+bvh_status bvh_object_overdecomposition(bvh_object o, size_t* od) {
+  if (!o || !od) return fail(BVH_E_INVALID, "null argument");
+  *od = o->w->od;
+  return BVH_OK;
+}
+
+bvh_status bvh_set_entity_data_poly(bvh_object o, const double* verts, uint32_t nv, uint64_t n,
+                                    const uint64_t* gids, double eps, uint32_t mem, uint32_t split) {
+  if (nv < 1 || nv > 8) return fail(BVH_E_INVALID, "verts_per_element must be in 1..8");
+  return set_entity_common(o, kPoly, (int)nv, eps, verts, nullptr, n, gids, mem, split);
+}
+
+bvh_status bvh_set_entity_data_sphere(bvh_object o, const double* xyzr, uint64_t n, const uint64_t* gids,
+                                      uint32_t mem, uint32_t split) {
+  return set_entity_common(o, kSphere, 1, 0.0, xyzr, nullptr, n, gids, mem, split);
+}
+
+bvh_status bvh_set_entity_data_snapshot(bvh_object o, const double* kdops, const double* centroids, uint64_t n,
+                                        const uint64_t* gids, uint32_t mem, uint32_t split) {
+  if (n > 0 && !centroids) return fail(BVH_E_INVALID, "null centroids");
+  return set_entity_common(o, kSnapshot, 1, 0.0, kdops, centroids, n, gids, mem, split);
+}
+
+bvh_status bvh_init_broadphase(bvh_object o) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  if (!o->built) return fail(BVH_E_STATE, "init_broadphase before set_entity_data");
+  o->have_results = false;
+  return BVH_OK;
+}
+
+bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
+  if (!o || !other) return fail(BVH_E_INVALID, "null object");
+  if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
+  o->w->reset_stages();
+  if (o == other) {
+    // the reference skips self collisions in the narrowphase
+    // (collision_object/impl.cpp:229-235): no results
+    o->npairs = 0;
+    o->npatch_pairs = 0;
+    o->have_results = true;
+    o->result_ptr = o->pairs.as<uint64_t>();
+    o->result_other = o;
+    o->self_result = true;
+    return BVH_OK;
+  }
+  return run_query(o, other, false);
+}
+
+bvh_status bvh_self_collision(bvh_object o) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  o->w->reset_stages();
+  return run_query(o, o, true);
+}
+
+bvh_status bvh_end_phase(bvh_object o) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  return BVH_OK;
+}
+
+bvh_status bvh_result_count(bvh_object o, uint64_t* count) {
+  if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  *count = o->have_results ? o->npairs : 0;
+  return BVH_OK;
+}
+
+bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_t* count) {
+  if (!o || !pairs || !count) return fail(BVH_E_INVALID, "null argument");
+  *pairs = o->have_results ? o->result_ptr : nullptr;
+  *count = o->have_results ? o->npairs : 0;
+  return BVH_OK;
+}
+
+bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t mem) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  const uint64_t n = o->have_results ? o->npairs : 0;
+  if (cap < n) return fail(BVH_E_OVERFLOW, "result buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
+  if (n && !out) return fail(BVH_E_INVALID, "null output");
+  return copy_out(out, o->result_ptr, n * 8, mem, o->w->stream);
+}
+
+bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_t cap, uint32_t mem) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  const uint64_t n = o->have_results ? o->npairs : 0;
+  if (cap < n) return fail(BVH_E_OVERFLOW, "record buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
+  if (n == 0) return BVH_OK;
+  if (!out) return fail(BVH_E_INVALID, "null output");
+  hipStream_t s = o->w->stream;
+  bvh_object_s* q = o->result_other;
+  BVH_TRY(o->records.ensure(n * sizeof(bvh_narrowphase_record)));
+  HIP_TRY(launch_records(o->result_ptr, n, o->patch_of_input.as<uint32_t>(),
+                         o->has_gids ? o->gids.as<uint64_t>() : nullptr, q->patch_of_input.as<uint32_t>(),
+                         q->has_gids ? q->gids.as<uint64_t>() : nullptr, 0, 0, o->records.as<uint64_t>(), s));
+  return copy_out(out, o->records.p, n * sizeof(bvh_narrowphase_record), mem, s);
+}
+
+bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {
+  if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  *count = o->have_results ? o->npatch_pairs : 0;
+  return BVH_OK;
+}
+
+bvh_status bvh_patch_pairs(bvh_object o, uint64_t* out, uint64_t cap) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  const uint64_t n = o->have_results ? o->npatch_pairs : 0;
+  if (cap < n) return fail(BVH_E_OVERFLOW, "patch pair buffer too small");
+  if (n == 0) return BVH_OK;
+  std::vector<uint64_t> tmp(n);
+  BVH_TRY(copy_out(tmp.data(), o->ppairs.p, n * 8, BVH_MEM_HOST, o->w->stream));
+  std::sort(tmp.begin(), tmp.end());
+  for (uint64_t i = 0; i < n; ++i) {
+    out[2 * i] = tmp[i] >> 32;
+    out[2 * i + 1] = tmp[i] & 0xffffffffu;
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_object_bounds(bvh_object o, double* min3, double* inv3) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  Bounds b{};
+  if (o->in.n > 0) BVH_TRY(copy_out(&b, o->bounds.p, sizeof(Bounds), BVH_MEM_HOST, o->w->stream));
+  for (int c = 0; c < 3; ++c) {
+    if (min3) min3[c] = b.min[c];
+    if (inv3) inv3[c] = b.inv[c];
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_object_sorted(bvh_object o, uint64_t* codes, uint32_t* perm, uint32_t mem) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  const uint64_t n = o->in.n;
+  if (n == 0) return BVH_OK;
+  hipStream_t s = o->w->stream;
+  if (codes) {
+    if (o->bits == 63) {
+      BVH_TRY(copy_out(codes, o->codes_a.p, n * 8, mem, s));
+    } else {
+      std::vector<uint32_t> tmp(n);
+      BVH_TRY(copy_out(tmp.data(), o->codes_a.p, n * 4, BVH_MEM_HOST, s));
+      if (mem == BVH_MEM_DEVICE) return fail(BVH_E_UNSUPPORTED, "30-bit codes are returned to host memory only");
+      for (uint64_t i = 0; i < n; ++i) codes[i] = tmp[i];
+    }
+  }
+  if (perm) BVH_TRY(copy_out(perm, o->perm.p, n * 4, mem, s));
+  return BVH_OK;
+}
+
+bvh_status bvh_object_splits(bvh_object o, uint64_t* splits, size_t cap) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  const size_t ns = o->w->od - 1;
+  if (cap < ns) return fail(BVH_E_OVERFLOW, "splits buffer too small");
+  if (ns == 0) return BVH_OK;
+  std::vector<uint32_t> tmp(ns);
+  BVH_TRY(copy_out(tmp.data(), o->splits.p, ns * 4, BVH_MEM_HOST, o->w->stream));
+  for (size_t i = 0; i < ns; ++i) splits[i] = tmp[i];
+  return BVH_OK;
+}
+
+bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "fp64 slabs are not stored in the fp32 form");
+  const uint64_t n = o->in.n;
+  if (n == 0) return BVH_OK;
+  hipStream_t s = o->w->stream;
+  // leaves n-1 .. 2n-2: copy the 26 slabs out of each 256-byte node
+  HIP_TRY(hipMemcpy2DAsync(kdops, kSlabs * sizeof(double), o->nodes.as<Node64>() + (n - 1), sizeof(Node64),
+                           kSlabs * sizeof(double), n,
+                           mem == BVH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  const size_t od = o->w->od;
+  if (cap < od) return fail(BVH_E_OVERFLOW, "patch buffer too small");
+  hipStream_t s = o->w->stream;
+  const uint64_t n = o->in.n;
+  std::vector<uint32_t> roots(od, kEnd), splits(od > 1 ? od - 1 : 1, 0);
+  BVH_TRY(copy_out(roots.data(), o->patch_root.p, od * 4, BVH_MEM_HOST, s));
+  if (od > 1) BVH_TRY(copy_out(splits.data(), o->splits.p, (od - 1) * 4, BVH_MEM_HOST, s));
+  std::vector<double> cent(3 * od, 0.0);
+  if (n > 0) {
+    DevBuf dc;
+    BVH_TRY(dc.ensure(3 * od * sizeof(double)));
+    const double* cpl = o->in.kind == kPoly ? o->centroid.as<double>() : nullptr;
+    const double* spl = o->in.kind == kSphere ? o->in.data : nullptr;
+    const double* cao = o->in.kind == kSnapshot ? o->in.centroids : nullptr;
+    HIP_TRY(launch_patch_centroids(cpl, spl, cao, o->perm.as<uint32_t>(), o->splits.as<uint32_t>(), n, od,
+                                   dc.as<double>(), s));
+    BVH_TRY(copy_out(cent.data(), dc.p, 3 * od * sizeof(double), BVH_MEM_HOST, s));
+  }
+  for (size_t k = 0; k < od; ++k) {
+    bvh_patch_info& pi = out[k];
+    const uint64_t b = (k == 0) ? 0 : splits[k - 1];
+    const uint64_t e = (k == od - 1) ? n : splits[k];
+    pi.global_id = k;
+    pi.size = e - b;
+    for (int a = 0; a < 13; ++a) {
+      pi.kdop[2 * a] = 1.7976931348623157e308;
+      pi.kdop[2 * a + 1] = -1.7976931348623157e308;
+    }
+    if (roots[k] != kEnd && !o->fp32) {
+      Node64 nd;
+      BVH_TRY(copy_out(&nd, o->nodes.as<Node64>() + roots[k], sizeof(Node64), BVH_MEM_HOST, s));
+      for (int i = 0; i < 26; ++i) pi.kdop[i] = nd.s[i];
+    }
+    for (int c = 0; c < 3; ++c) pi.centroid[c] = cent[3 * k + c];
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t* indices, uint64_t cap, uint64_t* count) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  const size_t od = o->w->od;
+  if (patch >= od) return fail(BVH_E_INVALID, "patch index out of range");
+  const uint64_t n = o->in.n;
+  std::vector<uint32_t> splits(od > 1 ? od - 1 : 1, 0);
+  if (od > 1) BVH_TRY(copy_out(splits.data(), o->splits.p, (od - 1) * 4, BVH_MEM_HOST, o->w->stream));
+  const uint64_t b = (patch == 0) ? 0 : splits[patch - 1];
+  const uint64_t e = (patch == od - 1) ? n : splits[patch];
+  if (count) *count = e - b;
+  if (cap < e - b) return fail(BVH_E_OVERFLOW, "index buffer too small");
+  if (e == b) return BVH_OK;
+  std::vector<uint32_t> tmp(e - b);
+  BVH_TRY(copy_out(tmp.data(), o->perm.as<uint32_t>() + b, (e - b) * 4, BVH_MEM_HOST, o->w->stream));
+  for (uint64_t i = 0; i < e - b; ++i) indices[i] = tmp[i];
+  return BVH_OK;
+}
+
+bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint64_t* ranges, size_t cap,
+                           size_t* nnodes, size_t* root) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "tree export is fp64 only");
+  const uint64_t n = o->in.n;
+  const size_t nn = n ? 2 * n - 1 : 0;
+  if (nnodes) *nnodes = nn;
+  uint32_t r = kEnd;
+  if (n) BVH_TRY(copy_out(&r, o->root.p, 4, BVH_MEM_HOST, o->w->stream));
+  if (root) *root = (r == kEnd) ? (size_t)-1 : r;
+  if (!kdops && !children && !ranges) return BVH_OK;
+  if (cap < nn) return fail(BVH_E_OVERFLOW, "tree buffer too small");
+  std::vector<Node64> nodes(nn);
+  if (nn) BVH_TRY(copy_out(nodes.data(), o->nodes.p, nn * sizeof(Node64), BVH_MEM_HOST, o->w->stream));
+  for (size_t i = 0; i < nn; ++i) {
+    const bool leaf = i >= n - 1;
+    if (kdops) std::memcpy(kdops + 26 * i, nodes[i].s, 26 * sizeof(double));
+    if (children) {
+      children[2 * i] = leaf ? -1 : (int64_t)nodes[i].left;
+      children[2 * i + 1] = leaf ? -1 : (int64_t)nodes[i].right;
+    }
+    if (ranges) {
+      ranges[2 * i] = nodes[i].first;
+      ranges[2 * i + 1] = nodes[i].last;
+    }
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  if (n == 0) return BVH_OK;
+  if (!vals) return fail(BVH_E_INVALID, "null values");
+  hipStream_t s = w->stream;
+  const uint64_t tiles = sort_tiles(n);
+  BVH_TRY(w->s_lb.ensure(tiles * 8 + 256));
+  BVH_TRY(w->s_ctl.ensure(sizeof(SortCtl)));
+  HIP_TRY(launch_fill_u32(w->s_lb.as<uint32_t>(), 2 * tiles, 0u, s));
+  HIP_TRY(launch_fill_u32(w->s_ctl.as<uint32_t>(), 1, 0u, s));
+  HIP_TRY(launch_scan_u32(vals, n, w->s_lb.as<uint64_t>(), w->s_ctl.as<uint32_t>(), s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_sort_u64(bvh_world w, uint64_t* keys, uint32_t* vals, uint64_t n) { return prim_sort<uint64_t>(w, keys, vals, n); }
+bvh_status bvh_sort_u32(bvh_world w, uint32_t* keys, uint32_t* vals, uint64_t n) { return prim_sort<uint32_t>(w, keys, vals, n); }
+
+}  // extern "C"

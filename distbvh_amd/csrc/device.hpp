@@ -1,0 +1,192 @@
+// device.hpp -- shared gfx950 device code: 26-DOP arithmetic, Morton codes,
+// node layout, wave helpers.
+//
+// All fp64 code is compiled with -ffp-contract=off so that every expression
+// is evaluated as written (bit-exact with the CPU restatement).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dbvh {
+
+constexpr int kAxes = 13;
+constexpr int kSlabs = 26;
+constexpr uint32_t kEnd = 0xffffffffu;     // "no node" / end of traversal
+constexpr uint32_t kMixed = 0xffffffffu;   // node spans several patches
+constexpr int kWave = 64;
+
+// The 13 positive normals of dop_26 (kdop.hpp:657-677).
+#define DBVH_R2 0.7071067811865475
+#define DBVH_R3 0.5773502691896258
+
+// project(v, axis) = v0*a0 + v1*a1 + v2*a2 left to right (kdop.hpp:223-227).
+// The normal components are compile-time constants; multiplication by 1.0
+// folds to the identity (exact), the 0.0 and +-R terms are kept.
+template <int A>
+__device__ __forceinline__ double project(double x, double y, double z) {
+  constexpr double N[13][3] = {
+      {1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.},
+      {DBVH_R3, DBVH_R3, DBVH_R3}, {DBVH_R3, -DBVH_R3, DBVH_R3},
+      {DBVH_R3, DBVH_R3, -DBVH_R3}, {DBVH_R3, -DBVH_R3, -DBVH_R3},
+      {DBVH_R2, DBVH_R2, 0.0}, {DBVH_R2, 0.0, DBVH_R2}, {0.0, DBVH_R2, DBVH_R2},
+      {DBVH_R2, -DBVH_R2, 0.0}, {DBVH_R2, 0.0, -DBVH_R2}, {0.0, DBVH_R2, -DBVH_R2}};
+  return x * N[A][0] + y * N[A][1] + z * N[A][2];
+}
+
+// std::min / std::max (kdop.hpp:258-259) and m::min / m::max
+// (math/common.hpp:86-96) as selects; never v_min_f64 (different +-0 / NaN
+// rules).
+__device__ __forceinline__ double sel_min(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double sel_max(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float sel_minf(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float sel_maxf(float a, float b) { return (a < b) ? b : a; }
+
+struct Kdop {
+  double s[kSlabs];  // min0, max0, ..., min12, max12
+};
+
+template <int A = 0>
+__device__ __forceinline__ void proj_all(double x, double y, double z, double* p) {
+  p[A] = project<A>(x, y, z);
+  if constexpr (A + 1 < kAxes) proj_all<A + 1>(x, y, z, p);
+}
+
+// from_vertices, kdop.hpp:238-264.
+template <int NV>
+__device__ __forceinline__ void kdop_from_vertices(const double (&v)[NV][3], double eps, double* s) {
+  double p[kAxes];
+  proj_all(v[0][0], v[0][1], v[0][2], p);
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    s[2 * a] = p[a] - eps;
+    s[2 * a + 1] = p[a] + eps;
+  }
+#pragma unroll
+  for (int j = 1; j < NV; ++j) {
+    proj_all(v[j][0], v[j][1], v[j][2], p);
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      s[2 * a] = sel_min(s[2 * a], p[a] - eps);
+      s[2 * a + 1] = sel_max(s[2 * a + 1], p[a] + eps);
+    }
+  }
+}
+
+// from_sphere, kdop.hpp:273-288.
+__device__ __forceinline__ void kdop_from_sphere(double x, double y, double z, double r, double* s) {
+  double p[kAxes];
+  proj_all(x, y, z, p);
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    s[2 * a] = p[a] - r;
+    s[2 * a + 1] = p[a] + r;
+  }
+}
+
+// overlap(kdop, kdop), kdop.hpp:120-125 / 538-549 (strict).
+__device__ __forceinline__ bool overlap(const double* a, const double* b) {
+  bool o = true;
+#pragma unroll
+  for (int i = 0; i < kAxes; ++i) o = o & (a[2 * i] < b[2 * i + 1]) & (b[2 * i] < a[2 * i + 1]);
+  return o;
+}
+__device__ __forceinline__ bool overlapf(const float* a, const float* b) {
+  bool o = true;
+#pragma unroll
+  for (int i = 0; i < kAxes; ++i) o = o & (a[2 * i] < b[2 * i + 1]) & (b[2 * i] < a[2 * i + 1]);
+  return o;
+}
+
+// union_with, kdop.hpp:317-325 (m::min / m::max).
+__device__ __forceinline__ void kdop_union(double* d, const double* s) {
+#pragma unroll
+  for (int i = 0; i < kAxes; ++i) {
+    d[2 * i] = (d[2 * i] < s[2 * i]) ? d[2 * i] : s[2 * i];
+    d[2 * i + 1] = (d[2 * i + 1] > s[2 * i + 1]) ? d[2 * i + 1] : s[2 * i + 1];
+  }
+}
+
+// ---- Morton (hash.hpp) ------------------------------------------------------
+__device__ __forceinline__ uint32_t expand32(uint32_t v) {
+  v = (v * 0x00010001u) & 0xff0000ffu;
+  v = (v * 0x00000101u) & 0x0f00f00fu;
+  v = (v * 0x00000011u) & 0xc30c30c3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+__device__ __forceinline__ uint64_t expand64(uint64_t v) {
+  v = (v * 0x1000001ull) & 0xfff000000fffull;
+  v = (v * 0x1001ull) & 0xfc003f000fc003full;
+  v = (v * 0x41ull) & 0x01c0e070381c0e07ull;
+  v = (v * 0x11ull) & 0x10c86432190c8643ull;
+  v = (v * 0x5ull) & 0x1249249249249249ull;
+  return v;
+}
+// The reference's static_cast<uintN_t>(double) as its x86-64 build evaluates
+// it: truncation to int64 and reinterpretation; NaN, +-inf and |s| >= 2^63
+// give 0 (what remains after the mask).  A degenerate axis (0 * inf = NaN,
+// hash.hpp:151-155) therefore quantizes to 0.
+__device__ __forceinline__ uint64_t float_to_uint(double s) {
+  if (!(s > -9223372036854775808.0 && s < 9223372036854775808.0)) return 0;
+  return (uint64_t)(int64_t)s;
+}
+
+struct Bounds {
+  double min[3];
+  double inv[3];
+  double max[3];
+  double pad;
+};
+
+// quantize64 + morton (hash.hpp:177-198, 121-130; contact_entity.hpp:112-117)
+__device__ __forceinline__ uint64_t morton63(double x, double y, double z, const Bounds& b) {
+  uint64_t q[3];
+  const double p[3] = {x, y, z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double norm = (p[i] - b.min[i]) * b.inv[i];
+    q[i] = float_to_uint(norm * 2097152.0) & 0x1fffffull;
+  }
+  return (expand64(q[2]) << 2) | (expand64(q[1]) << 1) | expand64(q[0]);
+}
+// quantize32 + morton (hash.hpp:146-167, 109-113)
+__device__ __forceinline__ uint32_t morton30(double x, double y, double z, const Bounds& b) {
+  uint32_t q[3];
+  const double p[3] = {x, y, z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double norm = (p[i] - b.min[i]) * b.inv[i];
+    q[i] = (uint32_t)float_to_uint(norm * 1024.0) & 0x3ffu;
+  }
+  return (expand32(q[2]) << 2) + (expand32(q[1]) << 1) + expand32(q[0]);
+}
+
+// ---- tree nodes -----------------------------------------------------------------
+// One 256-byte node per tree node, AoS: traversal reads nodes at random, so a
+// node's 26 slabs and its links share two 128-byte cache lines.
+// Internal node k (k in [0, n-2]) is the split between sorted positions k and
+// k+1; leaf p is node n-1+p.
+struct alignas(256) Node64 {
+  double s[kSlabs];  // 208 B
+  uint32_t left;     // internal: left child id; leaf: input index of the element
+  uint32_t right;    // internal: right child id; leaf: unused
+  uint32_t first;    // covered sorted positions [first, last]
+  uint32_t last;
+  uint32_t rope;     // next node in depth-first order after this subtree (kEnd: none)
+  uint32_t patch;    // patch id if the subtree lies in one patch, else kMixed
+  uint32_t pad[6];
+};
+static_assert(sizeof(Node64) == 256, "node layout");
+
+// fp32 node for the outward-rounded precision form: 104 B slabs + links in
+// one 128-byte line.
+struct alignas(128) Node32 {
+  float s[kSlabs];  // 104 B
+  uint32_t left, right, first, last, rope, patch;
+};
+static_assert(sizeof(Node32) == 128, "node32 layout");
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+}  // namespace dbvh

@@ -1,0 +1,142 @@
+// kernels.hpp -- host-side launchers of the device kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace dbvh {
+
+struct SortPassInfo {
+  uint32_t active, in_sel, out_sel, first;
+};
+struct SortCtl {
+  SortPassInfo pass[8];
+  uint32_t tile_counter[8];
+  uint32_t offsets[8 * 256];
+};
+// Buffers of one sort: keys in k0 (not modified), result in ka / va; kb / vb
+// are ping-pong scratch.  v0 == nullptr means payload = input index; va ==
+// nullptr means keys only.
+template <typename K>
+struct SortArgs {
+  const K* k0;
+  K* ka;
+  K* kb;
+  const uint32_t* v0;
+  uint32_t* va;
+  uint32_t* vb;
+  uint64_t n;
+  uint64_t ntiles;
+  SortCtl* ctl;
+  uint32_t* lookback;  // npass * ntiles * 256 words, zero before the sort
+};
+
+uint64_t sort_tiles(uint64_t n);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_iota_u32(uint32_t* p, uint64_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_copy(T* dst, const T* src, uint64_t n, hipStream_t s);
+template <typename K>
+hipError_t launch_sort_histogram(const K* keys, uint64_t n, int npass, uint32_t* hist, hipStream_t s);
+template <typename K>
+hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s);
+// exclusive scan in place; lb: sort_tiles(n) zeroed 64-bit words, counter: zeroed
+hipError_t launch_scan_u32(uint32_t* v, uint64_t n, uint64_t* lb, uint32_t* counter, hipStream_t s);
+
+// ---- object build ---------------------------------------------------------------
+enum ElemKind : int { kPoly = 0, kSphere = 1, kSnapshot = 2 };
+
+struct ElemInput {
+  int kind;
+  int nv;                   // vertices per element (kPoly)
+  double eps;               // from_vertices epsilon (kPoly)
+  const double* data;       // planes (kPoly / kSphere) or n x 26 kdops (kSnapshot)
+  const double* centroids;  // n x 3 (kSnapshot)
+  uint64_t n;
+};
+
+struct BuildBufs {
+  // original order
+  double* centroid;        // 3 planes of n (kPoly)
+  double* partials;        // per-block cardinal min/max (6 per block)
+  uint32_t* done_counter;  // last-block counter
+  Bounds* bounds;
+  // sort
+  void* codes0;            // Morton codes, input order (u64 or u32)
+  void* codes_a;           // sorted codes
+  void* codes_b;
+  uint32_t* perm;          // va: sorted position -> input index
+  uint32_t* perm_b;
+  uint32_t* hist;          // 8 * 256
+  SortCtl* ctl;
+  uint32_t* lookback;
+  // cluster
+  uint32_t* depth0;        // n-1 depth keys
+  uint32_t* depth_a;
+  uint32_t* depth_b;
+  uint32_t* depth_idx;
+  uint32_t* depth_idx_b;
+  uint32_t* splits;        // od - 1
+  // tree
+  uint8_t* delta;          // n-1
+  uint32_t* patch_of_pos;  // n
+  uint32_t* patch_of_input;// n
+  uint32_t* counters;      // n-1 arrival counters
+  uint32_t* patch_root;    // od
+  uint32_t* root;          // 1
+  Node64* nodes;           // 2n-1
+  Node32* nodes32;         // 2n-1 (fp32 form) or null
+  double* origin;          // 3 (fp32 form)
+  uint32_t* status;        // device error bits (kErr*)
+};
+
+// device error bits: set instead of faulting when an invariant breaks
+constexpr uint32_t kErrTree = 1u;      // inconsistent tree link during the build
+constexpr uint32_t kErrTraverse = 2u;  // traversal exceeded its step bound
+constexpr uint32_t kErrIndex = 4u;     // element index out of range
+
+hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s);
+hipError_t launch_morton(const ElemInput& in, const BuildBufs& b, int bits, hipStream_t s);
+hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsplits, hipStream_t s);
+hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, int fp32,
+                       hipStream_t s);
+uint64_t prep_blocks(uint64_t n);
+
+// ---- queries ----------------------------------------------------------------------
+struct TreeView {
+  const Node64* nodes;
+  const Node32* nodes32;
+  const uint32_t* root;
+  const uint32_t* patch_root;
+  const uint32_t* patch_of_input;
+  const uint32_t* perm;
+  uint64_t n;
+  uint64_t od;
+  uint32_t* status;
+};
+
+struct PairOut {
+  uint64_t* pairs;      // (input index a << 32) | input index b
+  uint64_t cap;
+  unsigned long long* count;
+};
+
+// element narrowphase: every leaf of `q` (queries, in its sorted order)
+// against tree `t`; self != 0 emits each unordered pair of one tree once.
+hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self,
+                              int fp32, const PairOut& out, hipStream_t s);
+// patch broadphase: patch roots of `q` against the patch level of `t`
+hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, const PairOut& out,
+                                   hipStream_t s);
+// records {patch_p, element_p, patch_q, element_q}
+hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t* patch_a,
+                          const uint64_t* gid_a, const uint32_t* patch_b, const uint64_t* gid_b,
+                          uint64_t patch_offset_a, uint64_t patch_offset_b, uint64_t* out,
+                          hipStream_t s);
+hipError_t launch_patch_centroids(const double* centroid_planes, const double* sphere_planes,
+                                  const double* snap_centroids, const uint32_t* perm,
+                                  const uint32_t* splits, uint64_t n, uint64_t od, double* out,
+                                  hipStream_t s);
+
+}  // namespace dbvh

@@ -1,0 +1,222 @@
+// query.hip -- broadphase and narrowphase traversal on gfx950.
+//
+//  * k_narrow: one lane per query element (the query object's leaves in its
+//    sorted order, so a wave holds 64 spatially adjacent elements).  Each lane
+//    walks the other object's tree without a stack: on overlap it descends to
+//    the left child, otherwise (or after a leaf) it follows the node's rope.
+//    Nodes whose range holds <= leaf_size elements are tested element by
+//    element.  Hits are appended with one atomic per wave and step.
+//    Replaces query_tree / query_node_impl (collision_query.hpp:166-213) as
+//    used by a narrowphase functor per patch pair, and self_collision_set
+//    (collision_query.hpp:331-368) in self mode (each unordered pair once).
+//  * k_patch_broadphase: one lane per patch root of the querying object; the
+//    walk stops at the first single-patch node, which is the other object's
+//    patch root (start_broadphase, collision_object/broadphase.cpp:68-119).
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace dbvh {
+
+__device__ __forceinline__ void emit(const PairOut& out, bool pred, uint64_t key) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(out.count, (unsigned long long)__popcll(m));
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)base, leader);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(base >> 32), leader);
+  base = ((unsigned long long)hi << 32) | lo;
+  if (pred) {
+    const unsigned long long idx = base + (unsigned long long)__popcll(m & lanemask_lt());
+    if (idx < out.cap) out.pairs[idx] = key;
+  }
+}
+
+__device__ __forceinline__ void load_node_slabs(const Node64* nd, double* k) {
+  const double2* d = (const double2*)nd->s;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    const double2 t = d[a];
+    k[2 * a] = t.x;
+    k[2 * a + 1] = t.y;
+  }
+}
+
+__device__ __forceinline__ uint64_t pair_key(uint32_t a, uint32_t b, bool self) {
+  if (self && b < a) {
+    const uint32_t t = a;
+    a = b;
+    b = t;
+  }
+  return ((uint64_t)a << 32) | b;
+}
+
+template <bool SELF>
+__global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf_size, PairOut out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= q.n || t.n == 0) return;
+  const Node64* qn = &q.nodes[q.n - 1 + p];
+  double qk[kSlabs];
+  load_node_slabs(qn, qk);
+  const uint32_t qa = qn->left;
+  const uint32_t tleaf0 = (uint32_t)(t.n - 1);
+  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
+  uint32_t node = *t.root;
+  uint32_t steps = 0;
+  while (node != kEnd) {
+    // every node is visited at most once in a depth-first walk
+    if (node >= nnodes || ++steps > nnodes) {
+      atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    const Node64* nd = &t.nodes[node];
+    const uint32_t first = nd->first, last = nd->last, rope = nd->rope;
+    if (SELF && last <= (uint32_t)p) {
+      node = rope;
+      continue;
+    }
+    double nk[kSlabs];
+    load_node_slabs(nd, nk);
+    const bool hit = overlap(qk, nk);
+    if (!hit) {
+      node = rope;
+      continue;
+    }
+    if (node >= tleaf0) {
+      emit(out, true, pair_key(qa, nd->left, SELF));
+      node = rope;
+    } else if ((int)(last - first) < leaf_size) {
+      for (uint32_t pos = first; pos <= last; ++pos) {
+        if (SELF && pos <= (uint32_t)p) continue;
+        const Node64* ln = &t.nodes[tleaf0 + pos];
+        double lk[kSlabs];
+        load_node_slabs(ln, lk);
+        const bool h = overlap(qk, lk);
+        emit(out, h, pair_key(qa, ln->left, SELF));
+      }
+      node = rope;
+    } else {
+      node = nd->left;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_patch_broadphase(TreeView q, TreeView t, PairOut out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= q.od || t.n == 0) return;
+  const uint32_t qr = q.patch_root[k];
+  if (qr == kEnd) return;  // empty patch (broadphase.cpp:79-80)
+  if (qr >= 2 * q.n - 1) {
+    atomicOr(q.status, kErrTree);
+    return;
+  }
+  double qk[kSlabs];
+  load_node_slabs(&q.nodes[qr], qk);
+  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
+  uint32_t node = *t.root;
+  uint32_t steps = 0;
+  while (node != kEnd) {
+    if (node >= nnodes || ++steps > nnodes) {
+      atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    const Node64* nd = &t.nodes[node];
+    double nk[kSlabs];
+    load_node_slabs(nd, nk);
+    const uint32_t rope = nd->rope;
+    if (overlap(qk, nk)) {
+      const uint32_t pt = nd->patch;
+      if (pt != kMixed) {
+        emit(out, true, ((uint64_t)k << 32) | pt);
+        node = rope;
+      } else {
+        node = nd->left;
+      }
+    } else {
+      node = rope;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_records(const uint64_t* __restrict__ pairs, uint64_t count,
+                                                 const uint32_t* __restrict__ patch_a, const uint64_t* __restrict__ gid_a,
+                                                 const uint32_t* __restrict__ patch_b, const uint64_t* __restrict__ gid_b,
+                                                 uint64_t off_a, uint64_t off_b, uint64_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = pairs[i];
+    const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
+    ulonglong4 r;
+    r.x = patch_a[a] + off_a;
+    r.y = gid_a ? gid_a[a] : a;
+    r.z = patch_b[b] + off_b;
+    r.w = gid_b ? gid_b[b] : b;
+    ((ulonglong4*)out)[i] = r;
+  }
+}
+
+// patch centroid = accumulate(element centroids in patch order, zeros) / size
+// (patch.hpp:113-116), summed sequentially per patch.
+__global__ __launch_bounds__(64) void k_patch_centroids(const double* __restrict__ cplanes,
+                                                        const double* __restrict__ splanes,
+                                                        const double* __restrict__ caos,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ splits, uint64_t n,
+                                                        uint64_t od, double* __restrict__ out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= od) return;
+  const uint64_t b = (k == 0) ? 0 : splits[k - 1];
+  const uint64_t e = (k == od - 1) ? n : splits[k];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (uint64_t pos = b; pos < e; ++pos) {
+    const uint32_t src = perm[pos];
+    for (int c = 0; c < 3; ++c) {
+      double v;
+      if (cplanes) v = cplanes[(uint64_t)c * n + src];
+      else if (splanes) v = splanes[(uint64_t)c * n + src];
+      else v = caos[3 * (uint64_t)src + c];
+      s[c] = s[c] + v;
+    }
+  }
+  for (int c = 0; c < 3; ++c) out[3 * k + c] = (e > b) ? s[c] / (double)(e - b) : 0.0;
+}
+
+// ---- launchers ---------------------------------------------------------------------
+hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
+                              const PairOut& out, hipStream_t s) {
+  (void)fp32;
+  if (q.n == 0 || t.n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((q.n + 255) / 256);
+  if (self) hipLaunchKernelGGL(k_narrow<true>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+  else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s) {
+  if (q.od == 0 || t.n == 0 || q.n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((q.od + 255) / 256);
+  hipLaunchKernelGGL(k_patch_broadphase, dim3(blocks), dim3(256), 0, s, q, t, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t* patch_a, const uint64_t* gid_a,
+                          const uint32_t* patch_b, const uint64_t* gid_b, uint64_t off_a, uint64_t off_b,
+                          uint64_t* out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, s, pairs, count, patch_a, gid_a, patch_b,
+                     gid_b, off_a, off_b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_centroids(const double* cplanes, const double* splanes, const double* caos,
+                                  const uint32_t* perm, const uint32_t* splits, uint64_t n, uint64_t od,
+                                  double* out, hipStream_t s) {
+  if (od == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_patch_centroids, dim3((unsigned)((od + 63) / 64)), dim3(64), 0, s, cplanes, splanes, caos,
+                     perm, splits, n, od, out);
+  return hipGetLastError();
+}
+
+}  // namespace dbvh

@@ -1,0 +1,201 @@
+/*
+ * distbvh.h -- C ABI of the MI355X (gfx950) collision pipeline.
+ *
+ * The entry points below are the drop-in boundary for distBVH's single-rank
+ * collision path.  Each one names the reference interface it replaces
+ * (paths relative to the distBVH source tree).  Plain pointers and sizes
+ * only; no exceptions cross this boundary; every call returns a bvh_status.
+ *
+ * Threading: one host thread drives a world.  All device work of a world is
+ * ordered on one HIP stream (the caller's, or one the world creates with the
+ * default priority).
+ *
+ * Element data: polytope vertices are SoA planes, plane (v*3 + c) of length
+ * n holds coordinate c (x, y, z) of vertex v of every element; spheres are
+ * the four planes x, y, z, r.  Pointers may be host or device memory
+ * (BVH_MEM_HOST / BVH_MEM_DEVICE); host data is uploaded during the call, so
+ * no pointer is retained after a call returns.
+ */
+#ifndef DISTBVH_H
+#define DISTBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BVH_ABI_VERSION 1
+
+typedef enum bvh_status {
+  BVH_OK = 0,
+  BVH_E_INVALID = 1,     /* bad argument; od not a power of two (collision_world.cpp:59-65) */
+  BVH_E_OOM = 2,         /* device allocation failed */
+  BVH_E_OVERFLOW = 3,    /* a caller buffer is too small; the needed size is reported */
+  BVH_E_HIP = 4,         /* HIP runtime error (text in bvh_last_error) */
+  BVH_E_STATE = 5,       /* call out of order (e.g. broadphase before set_entity_data) */
+  BVH_E_UNSUPPORTED = 6  /* option not built */
+} bvh_status;
+
+/* split_algorithm, types.hpp:52-57 */
+typedef enum bvh_split_algorithm {
+  BVH_SPLIT_GEOM_AXIS = 0,
+  BVH_SPLIT_ML_GEOM_AXIS = 1,
+  BVH_SPLIT_CLUSTERING = 2
+} bvh_split_algorithm;
+
+typedef enum bvh_memory { BVH_MEM_HOST = 0, BVH_MEM_DEVICE = 1 } bvh_memory;
+
+typedef enum bvh_precision {
+  BVH_PREC_FP64 = 0,          /* fp64 slabs, bit-exact with the CPU path */
+  BVH_PREC_FP32_OUTWARD = 1   /* fp32 slabs rounded outward + fp64 re-check */
+} bvh_precision;
+
+/* bvh_world_config.flags */
+#define BVH_FLAG_SORT_RESULTS 0x1u /* sort element pairs by (element_p, element_q) */
+#define BVH_FLAG_TIMING 0x2u       /* record per-stage HIP event timings */
+
+/* world_config, collision_world.hpp:48-52 (log levels replaced by device
+ * options) */
+typedef struct bvh_world_config {
+  uint32_t morton_bits; /* 63 (21 bits/axis, default) or 30 (10 bits/axis, the
+                           reference clustering path, split/cluster.hpp:139) */
+  uint32_t leaf_size;   /* elements per tree leaf, 1..8 (default 1) */
+  uint32_t precision;   /* bvh_precision */
+  uint32_t flags;
+  void *stream;         /* hipStream_t, or NULL for a world-owned stream */
+  int32_t device;       /* HIP device ordinal (default 0) */
+  uint32_t reserved[7];
+} bvh_world_config;
+
+typedef struct bvh_world_s *bvh_world;
+typedef struct bvh_object_s *bvh_object;
+
+/* detailed_narrowphase_result, tests/collision_object_test.cpp:318-324: the
+ * 32-byte record the built-in narrowphase emits, read back through
+ * for_each_result<R> (collision_object.hpp:177-187). */
+typedef struct bvh_narrowphase_record {
+  uint64_t patch_p;
+  uint64_t element_p;
+  uint64_t patch_q;
+  uint64_t element_q;
+} bvh_narrowphase_record;
+
+/* patch<>, patch.hpp:55-123 (global id, size, 26-DOP, centroid) */
+typedef struct bvh_patch_info {
+  uint64_t global_id;
+  uint64_t size;
+  double kdop[26]; /* min0, max0, ..., min12, max12 */
+  double centroid[3];
+} bvh_patch_info;
+
+/* ---- status ---------------------------------------------------------------- */
+const char *bvh_status_string(bvh_status s);
+/* text of the last error on this thread ("" when none) */
+const char *bvh_last_error(void);
+uint32_t bvh_abi_version(void);
+
+/* ---- world (collision_world.hpp:54-114) -------------------------------------- */
+void bvh_world_config_init(bvh_world_config *cfg);
+/* collision_world(std::size_t od, const world_config &), collision_world.hpp:61 */
+bvh_status bvh_world_create(size_t overdecomposition, const bvh_world_config *cfg, bvh_world *out);
+/* ~collision_world() */
+bvh_status bvh_world_destroy(bvh_world w);
+/* overdecomposition_factor(), collision_world.hpp:73 */
+bvh_status bvh_world_overdecomposition(bvh_world w, size_t *od);
+/* create_collision_object(), collision_world.hpp:70 (the world owns it) */
+bvh_status bvh_create_collision_object(bvh_world w, bvh_object *out);
+/* num_collision_objects(), collision_world.hpp:72 */
+bvh_status bvh_num_collision_objects(bvh_world w, size_t *n);
+/* start_iteration() / finish_iteration(), collision_world.hpp:96-97 */
+bvh_status bvh_start_iteration(bvh_world w);
+bvh_status bvh_finish_iteration(bvh_world w);
+/* wait for all device work of the world */
+bvh_status bvh_synchronize(bvh_world w);
+/* Per-stage device times (ms) of the last set_entity_data / broadphase calls,
+ * recorded when BVH_FLAG_TIMING is set.  names[i] gets a static string. */
+bvh_status bvh_stage_times(bvh_world w, const char **names, double *ms, size_t cap, size_t *count);
+
+/* ---- collision object (collision_object.hpp:55-283) --------------------------- */
+/* id(), collision_object.hpp:176 */
+bvh_status bvh_object_id(bvh_object o, size_t *id);
+/* set_entity_data(View<T*>, split_algorithm), collision_object.hpp:69-84 with
+ * T a polytope element whose kdop() is from_vertices (kdop.hpp:238-264) and
+ * whose centroid() is the vertex mean (tests/TestCommon.hpp:62-73).
+ * verts_per_element in 1..8; global_ids may be NULL (ids 0..n-1). */
+bvh_status bvh_set_entity_data_poly(bvh_object o, const double *verts, uint32_t verts_per_element,
+                                    uint64_t n, const uint64_t *global_ids, double epsilon,
+                                    uint32_t mem, uint32_t split_algorithm);
+/* set_entity_data for sphere elements: kdop = from_sphere (kdop.hpp:273-288),
+ * centroid = center.  xyzr: planes x, y, z, r. */
+bvh_status bvh_set_entity_data_sphere(bvh_object o, const double *xyzr, uint64_t n,
+                                      const uint64_t *global_ids, uint32_t mem,
+                                      uint32_t split_algorithm);
+/* set_entity_data for a generic ContactEntity (traits.hpp:114-135): the
+ * caller supplies each element's 26-DOP (n x 26, AoS) and centroid (n x 3),
+ * i.e. the reference's entity_snapshot (snapshot.hpp:68-149). */
+bvh_status bvh_set_entity_data_snapshot(bvh_object o, const double *kdops, const double *centroids,
+                                        uint64_t n, const uint64_t *global_ids, uint32_t mem,
+                                        uint32_t split_algorithm);
+/* init_broadphase(), collision_object.hpp:87 / collision_object.cpp:148-215 */
+bvh_status bvh_init_broadphase(bvh_object o);
+/* broadphase(collision_object &other), collision_object.hpp:170 /
+ * collision_object.cpp:248-299: patch broadphase + the built-in element
+ * narrowphase (strict 26-DOP overlap, kdop.hpp:538-549).  Results land in
+ * `o` (impl.cpp:328-348 sends both halves to the initiating object). */
+bvh_status bvh_broadphase(bvh_object o, bvh_object other);
+/* self_collision_set(tree), collision_query.hpp:360-368 at element level:
+ * unordered element pairs (i < j) of one object with overlapping 26-DOPs. */
+bvh_status bvh_self_collision(bvh_object o);
+/* end_phase(), collision_object.hpp:172 */
+bvh_status bvh_end_phase(bvh_object o);
+
+/* for_each_result<R>(f), collision_object.hpp:177-187: number of records and
+ * the 32-byte records themselves (host or device destination). */
+bvh_status bvh_result_count(bvh_object o, uint64_t *count);
+bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record *out, uint64_t cap, uint32_t mem);
+/* compact pairs (input index of element_p << 32 | input index of element_q) */
+bvh_status bvh_result_pairs(bvh_object o, uint64_t *out, uint64_t cap, uint32_t mem);
+/* device pointer to the compact pairs (valid until the next broadphase) */
+bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_t *count);
+/* broadphase patch pairs (query_tree emits, broadphase.cpp:93-118):
+ * out[2k] = patch of o, out[2k+1] = patch of other. */
+bvh_status bvh_patch_pair_count(bvh_object o, uint64_t *count);
+bvh_status bvh_patch_pairs(bvh_object o, uint64_t *out, uint64_t cap);
+
+/* local_patches(), collision_object.hpp:189; overdecomposition_factor() */
+bvh_status bvh_local_patches(bvh_object o, bvh_patch_info *out, size_t cap);
+bvh_status bvh_object_overdecomposition(bvh_object o, size_t *od);
+/* the input indices of the elements of one patch in patch order (the bytes
+ * a narrowphase functor receives, collision_object/impl.hpp:112-118) */
+bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t *indices, uint64_t cap,
+                              uint64_t *count);
+
+/* ---- parity hooks (stage outputs of the last set_entity_data) ------------------ */
+/* compute_bounds(min_inv_diag_bounds), contact_entity.hpp:136-148 */
+bvh_status bvh_object_bounds(bvh_object o, double *min3, double *inv_diag3);
+/* sorted Morton codes (always widened to 64 bits) and the sort permutation
+ * (sorted position -> input index), util/sort.hpp:100-115 */
+bvh_status bvh_object_sorted(bvh_object o, uint64_t *codes, uint32_t *perm, uint32_t mem);
+/* cluster splits (od - 1 values), split/cluster.hpp:89-168 */
+bvh_status bvh_object_splits(bvh_object o, uint64_t *splits, size_t cap);
+/* element 26-DOPs in sorted order (n x 26), the snapshots' kdops */
+bvh_status bvh_object_sorted_kdops(bvh_object o, double *kdops, uint32_t mem);
+/* tree export for for_each_tree (collision_object.hpp:164-168): node count,
+ * then per node kdop (26), children (-1 for none) and leaf range [first, last]. */
+bvh_status bvh_object_tree(bvh_object o, double *kdops, int64_t *children, uint64_t *ranges,
+                           size_t cap, size_t *nnodes, size_t *root);
+
+/* ---- stand-alone device primitives (device pointers, world's stream) ---------- */
+/* radix_sorter<T, I>, util/sort.hpp:75-144: stable sort of keys with a
+ * uint32 payload (payload may be NULL), in place. */
+bvh_status bvh_sort_u64(bvh_world w, uint64_t *keys, uint32_t *vals, uint64_t n);
+bvh_status bvh_sort_u32(bvh_world w, uint32_t *keys, uint32_t *vals, uint64_t n);
+/* prefix_sum, util/prefix_sum.hpp:41-54 (exclusive), in place */
+bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+"""GPU parity: every stage of the HIP pipeline against the CPU oracle on the
+same seeded inputs.  Integer / index outputs and fp64 slabs are compared bit
+for bit (slabs by value, so +0 == -0); pair sets exactly."""
+import numpy as np
+import pytest
+
+import oracle as O
+from distbvh_amd import CollisionWorld
+from distbvh_amd import workloads as W
+from helpers import gpu_object, oracle_elements, sort_pairs, sort_records, to_cuda, from_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a GPU"
+    return torch
+
+
+# ---- primitives -------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 5, 4095, 4096, 4097, 100003, 1 << 20])
+@pytest.mark.parametrize("bits", [64, 32])
+def test_sort_matches_stable_argsort(torch_cuda, n, bits):
+    rng = np.random.default_rng(n * 7 + bits)
+    if bits == 64:
+        keys = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+        keys[::3] = keys[0]  # duplicates exercise stability
+        keys = keys & np.uint64(0x7fff00ff00ff00ff)
+        dt = np.uint64
+    else:
+        keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        keys[1::4] = 7
+        dt = np.uint32
+    vals = np.arange(n, dtype=np.uint32)
+    with CollisionWorld(1) as w:
+        tk = to_cuda(keys)
+        tv = to_cuda(vals)
+        w.sort(tk, tv)
+        torch_cuda.cuda.synchronize()
+        gk = from_cuda(tk, dt)
+        gv = from_cuda(tv, np.uint32)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(gk, keys[order])
+    assert np.array_equal(gv, order.astype(np.uint32))
+
+
+def test_sort_known_answer(torch_cuda):
+    # tests/sort_test.cpp:9-30
+    keys = np.array([3, 7, 2, 1, 4, 9, 1, 3], np.uint32)
+    vals = np.arange(8, dtype=np.uint32)
+    with CollisionWorld(1) as w:
+        tk, tv = to_cuda(keys), to_cuda(vals)
+        w.sort(tk, tv)
+        assert list(from_cuda(tk, np.uint32)) == [1, 1, 2, 3, 3, 4, 7, 9]
+        assert list(from_cuda(tv, np.uint32)) == [3, 6, 2, 0, 7, 4, 1, 5]
+
+
+@pytest.mark.parametrize("n", [1, 8, 4097, 1000001])
+def test_exclusive_scan(torch_cuda, n):
+    rng = np.random.default_rng(n)
+    v = rng.integers(0, 1000, n, dtype=np.uint32)
+    if n == 8:
+        v = np.array([3, 7, 2, 1, 4, 9, 1, 3], np.uint32)  # tests/UtilTest.cpp:42-55
+    with CollisionWorld(1) as w:
+        t = to_cuda(v)
+        w.exclusive_scan(t)
+        got = from_cuda(t, np.uint32)
+    exp = (np.concatenate([np.zeros(1, np.uint64), np.cumsum(v.astype(np.uint64))[:-1]])
+           & np.uint64(0xffffffff)).astype(np.uint32)
+    assert np.array_equal(got, exp)
+
+
+# ---- object build -----------------------------------------------------------------
+def _oracle_object(kind, data, eps, od, bits, leaf=1):
+    kd, ce = oracle_elements(kind, data, eps)
+    return O.Object(kd, ce, od=od, morton_bits=bits, leaf_size=leaf), kd, ce
+
+
+@pytest.mark.parametrize("bits", [63, 30])
+@pytest.mark.parametrize("od", [1, 2, 4, 32, 64])
+@pytest.mark.parametrize("device", [False, True])
+def test_build_stages_flat_sheets(torch_cuda, bits, od, device):
+    a, b = W.flat_sheets(64)
+    for data in (a, b):
+        ref, kd, ce = _oracle_object("poly", data, 0.0, od, bits)
+        with CollisionWorld(od, morton_bits=bits) as w:
+            obj = gpu_object(w, "poly", data, 0.0, device=device)
+            mn, inv = obj.bounds()
+            rmn, rinv = O.compute_bounds(kd)
+            assert np.array_equal(mn, rmn) and np.array_equal(inv, rinv, equal_nan=True)
+            codes, perm = obj.sorted()
+            assert np.array_equal(codes, ref.codes)
+            assert np.array_equal(perm, ref.perm)
+            assert np.array_equal(obj.splits(), ref.splits)
+            assert np.array_equal(obj.sorted_kdops(), kd[perm])
+            pats = obj.local_patches()
+            for k, p in enumerate(pats):
+                assert p["size"] == ref.pend[k] - ref.pbegin[k]
+                if p["size"]:
+                    assert np.array_equal(p["kdop"], ref.patch_kdops[k])
+                    assert np.array_equal(p["centroid"], ref.patch_centroids[k])
+
+
+@pytest.mark.parametrize("nq,od,bits", [(64, 16, 63), (64, 16, 30), (128, 64, 63), (96, 1, 63)])
+def test_build_stages_wavy(torch_cuda, nq, od, bits):
+    a, b = W.wavy_sheets(nq)
+    ref, kd, ce = _oracle_object("poly", b, 0.0, od, bits)
+    with CollisionWorld(od, morton_bits=bits) as w:
+        obj = gpu_object(w, "poly", b, 0.0, device=True)
+        codes, perm = obj.sorted()
+        assert np.array_equal(codes, ref.codes)
+        assert np.array_equal(perm, ref.perm)
+        assert np.array_equal(obj.splits(), ref.splits)
+        assert np.array_equal(obj.sorted_kdops(), kd[perm])
+
+
+# ---- pair sets ----------------------------------------------------------------------
+def _collide_both(kind, da, db, eps, od, bits, leaf, device=True):
+    ra, ka, _ = _oracle_object(kind, da, eps, od, bits, leaf)
+    rb, kb, _ = _oracle_object(kind, db, eps, od, bits, leaf)
+    ref = ra.collide(rb)
+    brute = O.brute_pairs(ka, kb)
+    with CollisionWorld(od, morton_bits=bits, leaf_size=leaf) as w:
+        oa = gpu_object(w, kind, da, eps, device=device)
+        ob = gpu_object(w, kind, db, eps, device=device)
+        oa.broadphase(ob)
+        recs = sort_records(oa.results())
+        pp = oa.patch_pairs()
+    return recs, ref, brute, pp, ra, rb
+
+
+@pytest.mark.parametrize("leaf", [1, 4])
+def test_pairs_flat_sheets(torch_cuda, leaf):
+    a, b = W.flat_sheets(64)
+    # strict overlap with eps = 0: B is 0.02 above A, so the set is empty
+    recs, ref, brute, _, _, _ = _collide_both("poly", a, b, 0.0, 16, 63, leaf)
+    assert len(brute) == 0 and len(ref) == 0 and len(recs) == 0
+    # from_vertices epsilon 0.01 (kdop.hpp:238-264): non-empty set
+    recs, ref, brute, pp, ra, rb = _collide_both("poly", a, b, 0.01, 16, 63, leaf)
+    assert len(brute) > 0
+    assert np.array_equal(recs, sort_records(ref))
+    assert np.array_equal(sort_pairs(recs[:, [1, 3]]), sort_pairs(brute))
+
+
+@pytest.mark.parametrize("nq,od,bits,leaf", [(64, 16, 63, 1), (64, 16, 30, 4), (128, 32, 63, 4),
+                                             (128, 1, 63, 2)])
+def test_pairs_wavy_sheets(torch_cuda, nq, od, bits, leaf):
+    a, b = W.wavy_sheets(nq)
+    recs, ref, brute, pp, ra, rb = _collide_both("poly", a, b, 0.0, od, bits, leaf)
+    assert len(brute) > 0
+    assert np.array_equal(recs, sort_records(ref))
+    assert np.array_equal(sort_pairs(recs[:, [1, 3]]), sort_pairs(brute))
+    # broadphase patch pairs: every pair of non-empty patches with overlapping k-DOPs
+    exp = []
+    for p in range(od):
+        if ra.pend[p] == ra.pbegin[p]:
+            continue
+        for q in range(od):
+            if rb.pend[q] > rb.pbegin[q] and O.overlap(ra.patch_kdops[p], rb.patch_kdops[q]):
+                exp.append((p, q))
+    assert np.array_equal(pp, np.array(exp, np.uint64).reshape(-1, 2))
+
+
+def test_pairs_spheres(torch_cuda):
+    a = W.sphere_cloud(20000, radius=5e-3, ncomp=8, sigma=0.05, shift=(0, 0, 0), seed=11)
+    b = W.sphere_cloud(20000, radius=5e-3, ncomp=8, sigma=0.05, shift=(0, 0, 0), seed=12)
+    recs, ref, brute, _, _, _ = _collide_both("sphere", a, b, 0.0, 8, 63, 1)
+    assert len(brute) > 0
+    assert np.array_equal(recs, sort_records(ref))
+
+
+def test_self_collision_spheres(torch_cuda):
+    s = W.sphere_cloud(30000, radius=5e-3, ncomp=8, sigma=0.05, shift=(0, 0, 0), seed=13)
+    kd, _ = O.elements_sphere(s)
+    brute = O.brute_self(kd)
+    with CollisionWorld(8, leaf_size=2) as w:
+        o = gpu_object(w, "sphere", s, device=True)
+        o.self_collision()
+        got = sort_pairs(o.pairs())
+    assert len(brute) > 0
+    assert np.array_equal(got, sort_pairs(brute))
+
+
+def test_collision_known_answers(torch_cuda):
+    # tests/CollisionTest.cpp:43-81
+    g = W.hex_grid(4, 4, 4)
+    cube = W.hex_grid(1, 1, 1)
+    with CollisionWorld(1) as w:
+        oa = gpu_object(w, "poly", g)
+        ob = gpu_object(w, "poly", cube)
+        oa.broadphase(ob)
+        assert oa.result_count() == 64
+    with CollisionWorld(1) as w:
+        o = gpu_object(w, "poly", W.hex_grid(2, 2, 1))
+        o.self_collision()
+        assert o.result_count() == 0
+    with CollisionWorld(1) as w:
+        both = np.concatenate([W.hex_grid(2, 2, 2), cube], axis=2)
+        o = gpu_object(w, "poly", both)
+        o.self_collision()
+        assert o.result_count() == 8
