@@ -194,12 +194,16 @@ __global__ __launch_bounds__(256) void k_morton(ElemInput in, BuildBufs b) {
       ((uint32_t*)b.codes0)[e] = c32;
       code = c32;
     }
+    // a wave whose lanes share the digit adds once (the active-lane mask is
+    // taken before the leader branch)
+    const uint64_t active = __ballot(1);
+    const uint32_t leader = (uint32_t)(__ffsll((long long)active) - 1);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       const uint32_t d = (uint32_t)((code >> (8 * p)) & 255);
       const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
       if (__all(d == d0)) {
-        if (lane_id() == (uint32_t)(__ffsll(__ballot(1)) - 1)) atomicAdd(&sh[p * 256 + d0], (uint32_t)__popcll(__ballot(1)));
+        if (lane_id() == leader) atomicAdd(&sh[p * 256 + d0], (uint32_t)__popcll(active));
       } else {
         atomicAdd(&sh[p * 256 + d], 1u);
       }
@@ -613,7 +617,7 @@ static void tree_dispatch(const ElemInput& in, const BuildBufs& b, uint64_t od, 
   else hipLaunchKernelGGL((k_tree<KIND, NV, false>), dim3(blocks), dim3(256), 0, s, in, b, od);
 }
 
-hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, int fp32, hipStream_t s) {
+hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s) {
   const uint64_t n = in.n;
   if (n == 0) return hipSuccess;
   const uint64_t total = n > od ? n : od;
@@ -626,8 +630,12 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64
   else
     hipLaunchKernelGGL(k_delta<uint32_t>, dim3((unsigned)blocks), dim3(256), 0, s,
                        (const uint32_t*)b.codes_a, n, (const uint32_t*)b.splits, nsplits, od, b);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
+  const uint64_t n = in.n;
+  if (n == 0) return hipSuccess;
   if (in.kind == kPoly) {
     switch (in.nv) {
       case 1: tree_dispatch<kPoly, 1>(in, b, od, fp32, s); break;
@@ -645,8 +653,11 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64
   } else {
     tree_dispatch<kSnapshot, 1>(in, b, od, fp32, s);
   }
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s) {
+  if (n == 0) return hipSuccess;
   uint64_t rb = (2 * n - 1 + 255) / 256;
   if (rb > 4096) rb = 4096;
   if (fp32) hipLaunchKernelGGL(k_ropes<true>, dim3((unsigned)rb), dim3(256), 0, s, b, n);

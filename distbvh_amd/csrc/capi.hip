@@ -117,7 +117,7 @@ struct bvh_world_s {
     events_used = 0;
   }
   void stage_begin(const char* name) {
-    if (!(cfg.flags & BVH_FLAG_TIMING)) return;
+    if (!(cfg.flags & BVH_FLAG_TIMING) || stages.size() >= 512) return;
     Stage s{name, take_event(), take_event()};
     if (s.a && s.b) {
       (void)hipEventRecord(s.a, stream);
@@ -294,8 +294,14 @@ static bvh_status object_build(bvh_object_s* o) {
     // object origin for the fp32 slabs: the Morton bounds minimum (fp64)
     HIP_TRY(hipMemcpyAsync(o->bb.origin, o->bb.bounds, 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  w->stage_begin("delta");
+  HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
+  w->stage_end();
   w->stage_begin("tree");
-  HIP_TRY(launch_tree(o->in, o->bb, o->bits, w->od, o->fp32, s));
+  HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
+  w->stage_end();
+  w->stage_begin("ropes");
+  HIP_TRY(launch_ropes(o->bb, n, o->fp32, s));
   w->stage_end();
   o->built = true;
   return BVH_OK;
@@ -314,7 +320,6 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
   bvh_world w = o->w;
   hipStream_t s = w->stream;
-  w->reset_stages();
   o->have_results = false;
   o->in.kind = kind;
   o->in.nv = nv;
@@ -610,6 +615,7 @@ bvh_status bvh_stage_times(bvh_world w, const char** names, double* ms, size_t c
     ++k;
   }
   if (count) *count = k;
+  w->reset_stages();
   return BVH_OK;
 }
 
@@ -652,7 +658,6 @@ bvh_status bvh_init_broadphase(bvh_object o) {
 bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
   if (!o || !other) return fail(BVH_E_INVALID, "null object");
   if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
-  o->w->reset_stages();
   if (o == other) {
     // the reference skips self collisions in the narrowphase
     // (collision_object/impl.cpp:229-235): no results
@@ -669,7 +674,6 @@ bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
 
 bvh_status bvh_self_collision(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
-  o->w->reset_stages();
   return run_query(o, o, true);
 }
 

@@ -99,8 +99,12 @@ constexpr uint32_t kErrIndex = 4u;     // element index out of range
 hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s);
 hipError_t launch_morton(const ElemInput& in, const BuildBufs& b, int bits, hipStream_t s);
 hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsplits, hipStream_t s);
-hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, int fp32,
-                       hipStream_t s);
+// k_delta (patch ids, split heights, cleared counters)
+hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s);
+// k_tree (leaf slabs in sorted order + bottom-up build with refit)
+hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s);
+// k_ropes (escape links)
+hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
 
 // ---- queries ----------------------------------------------------------------------

@@ -13,8 +13,10 @@
  * Element data: polytope vertices are SoA planes, plane (v*3 + c) of length
  * n holds coordinate c (x, y, z) of vertex v of every element; spheres are
  * the four planes x, y, z, r.  Pointers may be host or device memory
- * (BVH_MEM_HOST / BVH_MEM_DEVICE); host data is uploaded during the call, so
- * no pointer is retained after a call returns.
+ * (BVH_MEM_HOST / BVH_MEM_DEVICE).  Host data is uploaded during the call and
+ * not retained.  Device data is read in place and, like the element pointer
+ * the reference keeps until the narrowphase (collision_object.cpp:128-129),
+ * must stay valid until the object's next set_entity_data.
  */
 #ifndef DISTBVH_H
 #define DISTBVH_H
@@ -113,8 +115,9 @@ bvh_status bvh_start_iteration(bvh_world w);
 bvh_status bvh_finish_iteration(bvh_world w);
 /* wait for all device work of the world */
 bvh_status bvh_synchronize(bvh_world w);
-/* Per-stage device times (ms) of the last set_entity_data / broadphase calls,
- * recorded when BVH_FLAG_TIMING is set.  names[i] gets a static string. */
+/* Per-stage device times (ms) recorded (BVH_FLAG_TIMING) since the previous
+ * call of this function, in call order; reading clears the record (at most
+ * 512 stages are kept).  names[i] gets a static string. */
 bvh_status bvh_stage_times(bvh_world w, const char **names, double *ms, size_t cap, size_t *count);
 
 /* ---- collision object (collision_object.hpp:55-283) --------------------------- */

@@ -201,3 +201,34 @@ def test_collision_known_answers(torch_cuda):
         o = gpu_object(w, "poly", both)
         o.self_collision()
         assert o.result_count() == 8
+
+
+def test_full_size_wavy_sheets(torch_cuda):
+    """The benchmark workload at full size (2 x 2,097,152 triangles, 256
+    patches, leaf size 4): sorted codes and permutation bit-exact, the
+    broadphase patch pairs and the narrowphase records identical to the
+    oracle's, plus invariants (codes sorted, permutation complete)."""
+    import os
+    a, b = W.wavy_sheets(1024)
+    nt = min(len(os.sched_getaffinity(0)), 16)
+    ka, ca = O.elements_poly(a)
+    kb, cb = O.elements_poly(b)
+    ra = O.Object(ka, ca, od=256, morton_bits=63, leaf_size=4, nthreads=nt)
+    rb = O.Object(kb, cb, od=256, morton_bits=63, leaf_size=4, nthreads=nt)
+    ref = sort_records(ra.collide(rb))
+    with CollisionWorld(256, morton_bits=63, leaf_size=4) as w:
+        oa = gpu_object(w, "poly", a, device=True)
+        ob = gpu_object(w, "poly", b, device=True)
+        codes, perm = oa.sorted()
+        assert np.all(codes[1:] >= codes[:-1])
+        assert np.array_equal(np.sort(perm), np.arange(len(perm), dtype=np.uint32))
+        assert np.array_equal(codes, ra.codes) and np.array_equal(perm, ra.perm)
+        assert np.array_equal(oa.splits(), ra.splits)
+        oa.broadphase(ob)
+        recs = sort_records(oa.results())
+        pp = oa.patch_pairs()
+    assert len(ref) > 0
+    assert np.array_equal(recs, ref)
+    # every narrowphase record lies in a broadphase patch pair
+    pset = set(map(tuple, pp.tolist()))
+    assert all((p, q) in pset for p, q in set(map(tuple, recs[:, [0, 2]].tolist())))
