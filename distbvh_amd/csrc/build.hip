@@ -291,6 +291,7 @@ __global__ __launch_bounds__(256) void k_delta(const K* __restrict__ codes, uint
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (i < od) b.patch_root[i] = kEnd;
+    if (i == 0) *b.root = kEnd;  // set again by the build; stays empty if it fails
     if (i >= n) continue;
     const uint32_t pi = patch_of(splits, nsplits, (uint32_t)i);
     b.patch_of_pos[i] = pi;
@@ -398,48 +399,92 @@ __device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t o
   return (uint64_t)k == od - 1 ? (uint32_t)n : splits[k];
 }
 
-// Leaf slabs (gathered in sorted order) + bottom-up build with refit.  Each
-// thread starts at its leaf and climbs: a node [l, r] is the left child of
-// internal node r when delta(r) < delta(l-1) (or l == 0), else the right
-// child of internal node l-1.  The agent-scope acq_rel arrival counter makes
-// the first arriver's node visible to the second, which merges both slab sets
-// and continues; the first arriver stops.
-template <int KIND, int NV, bool FP32>
-__global__ __launch_bounds__(256) void k_tree(ElemInput in, BuildBufs b, uint64_t od) {
-  const uint64_t n = in.n;
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t e = b.perm[p];
-  if (e >= n) {
-    atomicOr(b.status, kErrIndex);
-    return;
-  }
-  double kd[kSlabs];
-  element_kdop<KIND, NV>(in, e, kd);
-  const uint32_t pk = b.patch_of_pos[p];
-  b.patch_of_input[e] = pk;
-  const uint32_t leaf = (uint32_t)(n - 1 + p);
-  double po[kAxes];
-  float kf[kSlabs];
+// ---- bottom-up build with refit -------------------------------------------------
+// A node [l, r] is the left child of internal node r when delta(r) <
+// delta(l-1) (or l == 0), else the right child of internal node l-1.  Each
+// leaf climbs; at a parent, an arrival counter decides: the first child to
+// arrive stops, the second merges both children's slabs, writes the parent and
+// continues.  Every synchronization is workgroup-local:
+//  * phase 1 (k_tree_leaves): a workgroup owns 256 consecutive leaves and
+//    counts arrivals in LDS.  A child whose parent splits exactly at a chunk
+//    boundary (at most the left child of internal node c1-1 and the right
+//    child of internal node c0-1) is handed to the next phase; a first arriver
+//    whose sibling extends past the chunk leaves its count in global memory.
+//  * phase k >= 2 (k_tree_up): chunks 256x larger; a workgroup continues the
+//    handed-over nodes of its 256 sub-chunks with workgroup-scope atomics on
+//    the global counters of the splits inside its chunk (no other workgroup
+//    touches them in that launch; earlier launches are ordered by the kernel
+//    boundary).  The last phase is one chunk and finds the root.
+template <bool FP32>
+struct Slab {
+  using T = double;
+};
+template <>
+struct Slab<true> {
+  using T = float;
+};
+
+template <bool FP32>
+__device__ __forceinline__ void node_set_child(const BuildBufs& b, uint32_t parent, bool is_left, uint32_t child) {
   if constexpr (FP32) {
-    const double ox = b.origin[0], oy = b.origin[1], oz = b.origin[2];
-    proj_all(ox, oy, oz, po);
-    to_fp32_outward(kd, po, kf);
-    Node32* nd = &b.nodes32[leaf];
-    store_slabs32(nd, kf);
-    nd->left = e; nd->right = kEnd; nd->first = (uint32_t)p; nd->last = (uint32_t)p; nd->patch = pk;
+    if (is_left) b.nodes32[parent].left = child;
+    else b.nodes32[parent].right = child;
   } else {
-    Node64* nd = &b.nodes[leaf];
-    store_slabs(nd, kd);
-    nd->left = e; nd->right = kEnd; nd->first = (uint32_t)p; nd->last = (uint32_t)p; nd->patch = pk;
+    if (is_left) b.nodes[parent].left = child;
+    else b.nodes[parent].right = child;
   }
-  const uint32_t* splits = b.splits;
-  if (patch_begin(splits, od, pk) == p && patch_end(splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
-  if (n == 1) {
-    *b.root = leaf;
-    return;
+  if (!is_left) b.rightc[parent] = child;
+}
+
+template <bool FP32>
+__device__ __forceinline__ uint32_t node_child(const BuildBufs& b, uint32_t parent, bool left) {
+  if constexpr (FP32) return left ? b.nodes32[parent].left : b.nodes32[parent].right;
+  else return left ? b.nodes[parent].left : b.nodes[parent].right;
+}
+
+template <bool FP32>
+__device__ __forceinline__ void node_load(const BuildBufs& b, uint32_t id, typename Slab<FP32>::T* k,
+                                          uint32_t& first, uint32_t& last, uint32_t& patch) {
+  if constexpr (FP32) {
+    const Node32* nd = &b.nodes32[id];
+    load_slabs32(nd, k);
+    first = nd->first; last = nd->last; patch = nd->patch;
+  } else {
+    const Node64* nd = &b.nodes[id];
+    load_slabs(nd, k);
+    first = nd->first; last = nd->last; patch = nd->patch;
   }
-  uint32_t node = leaf, l = (uint32_t)p, r = (uint32_t)p, mypatch = pk;
+}
+
+template <bool FP32>
+__device__ __forceinline__ void node_store(const BuildBufs& b, uint32_t id, const typename Slab<FP32>::T* k,
+                                           uint32_t first, uint32_t last, uint32_t patch) {
+  if constexpr (FP32) {
+    Node32* nd = &b.nodes32[id];
+    store_slabs32(nd, k);
+    nd->first = first; nd->last = last; nd->patch = patch;
+  } else {
+    Node64* nd = &b.nodes[id];
+    store_slabs(nd, k);
+    nd->first = first; nd->last = last; nd->patch = patch;
+  }
+}
+
+template <bool FP32>
+__device__ __forceinline__ void slab_union(typename Slab<FP32>::T* d, const typename Slab<FP32>::T* s) {
+#pragma unroll
+  for (int i = 0; i < kAxes; ++i) {
+    d[2 * i] = (d[2 * i] < s[2 * i]) ? d[2 * i] : s[2 * i];
+    d[2 * i + 1] = (d[2 * i + 1] > s[2 * i + 1]) ? d[2 * i + 1] : s[2 * i + 1];
+  }
+}
+
+// Climb from a finished node until it is the first arrival at its parent, the
+// root, or a child whose parent splits on the chunk boundary [c0, c1).
+template <bool FP32, bool LDS>
+__device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
+                                      uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t c0,
+                                      uint32_t c1, uint32_t* s_cnt, uint32_t* s_front, uint32_t* s_nfront) {
   for (;;) {
     if (l == 0 && r == n - 1) {
       *b.root = node;
@@ -447,83 +492,139 @@ __global__ __launch_bounds__(256) void k_tree(ElemInput in, BuildBufs b, uint64_
     }
     const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
     const uint32_t parent = is_left ? r : l - 1;
-    if (FP32) {
-      if (is_left) b.nodes32[parent].left = node;
-      else b.nodes32[parent].right = node;
-    } else {
-      if (is_left) b.nodes[parent].left = node;
-      else b.nodes[parent].right = node;
+    if (parent < c0 || parent + 1 >= c1) {
+      const uint32_t k = atomicAdd(s_nfront, 1u);
+      if (k < 2) s_front[k] = node;
+      else atomicOr(b.status, kErrTree);
+      return;
     }
-    const uint32_t old = __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    node_set_child<FP32>(b, parent, is_left, node);
+    uint32_t old;
+    if constexpr (LDS)
+      old = __hip_atomic_fetch_add(&s_cnt[parent - c0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      old = __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) return;  // first arriver: the sibling finishes the parent
-    uint32_t sib, sfirst, slast, spatch;
-    if constexpr (FP32) {
-      const Node32* pn = &b.nodes32[parent];
-      sib = is_left ? pn->right : pn->left;
-      if (sib >= 2 * n - 1) {
-        atomicOr(b.status, kErrTree);
-        return;
-      }
-      const Node32* sn = &b.nodes32[sib];
-      float ks[kSlabs];
-      load_slabs32(sn, ks);
-#pragma unroll
-      for (int i = 0; i < kAxes; ++i) {
-        kf[2 * i] = (kf[2 * i] < ks[2 * i]) ? kf[2 * i] : ks[2 * i];
-        kf[2 * i + 1] = (kf[2 * i + 1] > ks[2 * i + 1]) ? kf[2 * i + 1] : ks[2 * i + 1];
-      }
-      sfirst = sn->first; slast = sn->last; spatch = sn->patch;
-    } else {
-      const Node64* pn = &b.nodes[parent];
-      sib = is_left ? pn->right : pn->left;
-      if (sib >= 2 * n - 1) {
-        atomicOr(b.status, kErrTree);
-        return;
-      }
-      const Node64* sn = &b.nodes[sib];
-      double ks[kSlabs];
-      load_slabs(sn, ks);
-      kdop_union(kd, ks);
-      sfirst = sn->first; slast = sn->last; spatch = sn->patch;
+    const uint32_t sib = node_child<FP32>(b, parent, !is_left);
+    if (old != 1 || sib >= 2 * n - 1) {
+      atomicOr(b.status, kErrTree);
+      return;
     }
+    typename Slab<FP32>::T ks[kSlabs];
+    uint32_t sfirst, slast, spatch;
+    node_load<FP32>(b, sib, ks, sfirst, slast, spatch);
+    slab_union<FP32>(kd, ks);
     if (is_left) r = slast;
     else l = sfirst;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    if constexpr (FP32) {
-      Node32* nd = &b.nodes32[parent];
-      store_slabs32(nd, kf);
-      nd->first = l; nd->last = r; nd->patch = mypatch;
-    } else {
-      Node64* nd = &b.nodes[parent];
-      store_slabs(nd, kd);
-      nd->first = l; nd->last = r; nd->patch = mypatch;
-    }
-    if (mypatch != kMixed && patch_begin(splits, od, mypatch) == l && patch_end(splits, od, n, mypatch) == r + 1)
+    node_store<FP32>(b, parent, kd, l, r, mypatch);
+    b.ilast[parent] = r;
+    if (mypatch != kMixed && patch_begin(b.splits, od, mypatch) == l && patch_end(b.splits, od, n, mypatch) == r + 1)
       b.patch_root[mypatch] = parent;
     node = parent;
   }
 }
 
-// rope(node [l, r]) = right child of internal node r (or end when r = n-1):
-// the next subtree in depth-first order once this one is done.
+constexpr uint32_t kTreeChunk = 256;
+
+template <int KIND, int NV, bool FP32>
+__global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildBufs b, uint64_t od) {
+  __shared__ uint32_t s_cnt[kTreeChunk];
+  __shared__ uint32_t s_front[2];
+  __shared__ uint32_t s_nfront;
+  const uint64_t n = in.n;
+  const uint32_t t = threadIdx.x;
+  const uint32_t c0 = blockIdx.x * kTreeChunk;
+  const uint32_t c1 = (uint32_t)((uint64_t)c0 + kTreeChunk < n ? c0 + kTreeChunk : n);
+  s_cnt[t] = 0;
+  if (t == 0) s_nfront = 0;
+  __syncthreads();
+  const uint32_t p = c0 + t;
+  if (p < c1) {
+    const uint32_t e = b.perm[p];
+    if (e < n) {
+      double kd[kSlabs];
+      element_kdop<KIND, NV>(in, e, kd);
+      const uint32_t pk = b.patch_of_pos[p];
+      b.patch_of_input[e] = pk;
+      const uint32_t leaf = (uint32_t)(n - 1 + p);
+      if (patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
+      if constexpr (FP32) {
+        double po[kAxes];
+        float kf[kSlabs];
+        proj_all(b.origin[0], b.origin[1], b.origin[2], po);
+        to_fp32_outward(kd, po, kf);
+        node_store<true>(b, leaf, kf, p, p, pk);
+        b.nodes32[leaf].left = e;
+        b.nodes32[leaf].right = kEnd;
+        if (n == 1) *b.root = leaf;
+        else climb<true, true>(b, n, od, leaf, p, p, pk, kf, c0, c1, s_cnt, s_front, &s_nfront);
+      } else {
+        node_store<false>(b, leaf, kd, p, p, pk);
+        b.nodes[leaf].left = e;
+        b.nodes[leaf].right = kEnd;
+        if (n == 1) *b.root = leaf;
+        else climb<false, true>(b, n, od, leaf, p, p, pk, kd, c0, c1, s_cnt, s_front, &s_nfront);
+      }
+    } else {
+      atomicOr(b.status, kErrIndex);
+    }
+  }
+  __syncthreads();
+  // a first arrival whose sibling lies outside the chunk keeps its count
+  if (c0 + t + 1 < c1 && s_cnt[t] == 1) b.counters[c0 + t] = 1;
+  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t od, uint64_t chunk,
+                                                        uint64_t nsub, const uint32_t* __restrict__ front_in,
+                                                        uint32_t* __restrict__ front_out) {
+  __shared__ uint32_t s_front[2];
+  __shared__ uint32_t s_nfront;
+  const uint32_t t = threadIdx.x;
+  const uint64_t c0l = (uint64_t)blockIdx.x * chunk;
+  const uint32_t c0 = (uint32_t)c0l;
+  const uint32_t c1 = (uint32_t)(c0l + chunk < n ? c0l + chunk : n);
+  if (t == 0) s_nfront = 0;
+  __syncthreads();
+  const uint64_t sub = (uint64_t)blockIdx.x * kTreeChunk + t;
+  if (sub < nsub) {
+    for (int i = 0; i < 2; ++i) {
+      const uint32_t node = front_in[2 * sub + i];
+      if (node == kEnd) continue;
+      if (node >= 2 * n - 1) {
+        atomicOr(b.status, kErrTree);
+        continue;
+      }
+      typename Slab<FP32>::T kd[kSlabs];
+      uint32_t l, r, pk;
+      node_load<FP32>(b, node, kd, l, r, pk);
+      climb<FP32, false>(b, n, od, node, l, r, pk, kd, c0, c1, nullptr, s_front, &s_nfront);
+    }
+  }
+  __syncthreads();
+  if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
+}
+
+// rope(node [l, r]) = right child of internal node r (none when r = n-1): the
+// next subtree in depth-first order once this one is done.
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_ropes(BuildBufs b, uint64_t n) {
   const uint64_t total = 2 * n - 1;
   for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (uint64_t)gridDim.x * blockDim.x) {
-    if constexpr (FP32) {
-      const uint32_t last = b.nodes32[id].last;
-      uint32_t rope = kEnd;
-      if (last < n - 1) rope = b.nodes32[last].right;
-      else if (last != n - 1) atomicOr(b.status, kErrTree);
-      b.nodes32[id].rope = rope < 2 * n - 1 ? rope : kEnd;
-    } else {
-      const uint32_t last = b.nodes[id].last;
-      uint32_t rope = kEnd;
-      if (last < n - 1) rope = b.nodes[last].right;
-      else if (last != n - 1) atomicOr(b.status, kErrTree);
-      b.nodes[id].rope = rope < 2 * n - 1 ? rope : kEnd;
+    const uint32_t last = id < n - 1 ? b.ilast[id] : (uint32_t)(id - (n - 1));
+    uint32_t rope = kEnd;
+    if (last < n - 1) {
+      rope = b.rightc[last];
+      if (rope >= total) {
+        atomicOr(b.status, kErrTree);
+        rope = kEnd;
+      }
     }
+    if constexpr (FP32) b.nodes32[id].rope = rope;
+    else b.nodes[id].rope = rope;
   }
 }
 
@@ -612,9 +713,9 @@ hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsp
 
 template <int KIND, int NV>
 static void tree_dispatch(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
-  const unsigned blocks = (unsigned)((in.n + 255) / 256);
-  if (fp32) hipLaunchKernelGGL((k_tree<KIND, NV, true>), dim3(blocks), dim3(256), 0, s, in, b, od);
-  else hipLaunchKernelGGL((k_tree<KIND, NV, false>), dim3(blocks), dim3(256), 0, s, in, b, od);
+  const unsigned blocks = (unsigned)((in.n + kTreeChunk - 1) / kTreeChunk);
+  if (fp32) hipLaunchKernelGGL((k_tree_leaves<KIND, NV, true>), dim3(blocks), dim3(kTreeChunk), 0, s, in, b, od);
+  else hipLaunchKernelGGL((k_tree_leaves<KIND, NV, false>), dim3(blocks), dim3(kTreeChunk), 0, s, in, b, od);
 }
 
 hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s) {
@@ -653,7 +754,26 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
   } else {
     tree_dispatch<kSnapshot, 1>(in, b, od, fp32, s);
   }
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // upper phases: chunks 256x larger until one chunk holds every leaf
+  uint64_t chunk = kTreeChunk;
+  uint32_t* fin = b.front_a;
+  uint32_t* fout = b.front_b;
+  while (chunk < n) {
+    const uint64_t nsub = (n + chunk - 1) / chunk;
+    const uint64_t next = chunk * kTreeChunk;
+    const unsigned blocks = (unsigned)((n + next - 1) / next);
+    if (fp32) hipLaunchKernelGGL(k_tree_up<true>, dim3(blocks), dim3(kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
+    else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uint32_t* tmp = fin;
+    fin = fout;
+    fout = tmp;
+    chunk = next;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s) {

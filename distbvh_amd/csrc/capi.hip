@@ -144,7 +144,7 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status;
+      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status, rightc, ilast, front_a, front_b;
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;
@@ -212,6 +212,10 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
   BVH_TRY(o->status.ensure(2 * sizeof(uint32_t)));  // [error bits, last-block counter]
+  BVH_TRY(o->rightc.ensure(n1 * 4));
+  BVH_TRY(o->ilast.ensure(n1 * 4));
+  BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
+  BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   if (o->fp32) {
     BVH_TRY(o->nodes32.ensure((2 * n1 - 1) * sizeof(Node32)));
     BVH_TRY(o->origin.ensure(3 * sizeof(double)));
@@ -247,6 +251,10 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.nodes32 = o->fp32 ? o->nodes32.as<Node32>() : nullptr;
   b.origin = o->fp32 ? o->origin.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
+  b.rightc = o->rightc.as<uint32_t>();
+  b.ilast = o->ilast.as<uint32_t>();
+  b.front_a = o->front_a.as<uint32_t>();
+  b.front_b = o->front_b.as<uint32_t>();
   return BVH_OK;
 }
 

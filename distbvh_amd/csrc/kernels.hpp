@@ -89,6 +89,10 @@ struct BuildBufs {
   Node32* nodes32;         // 2n-1 (fp32 form) or null
   double* origin;          // 3 (fp32 form)
   uint32_t* status;        // device error bits (kErr*)
+  uint32_t* rightc;        // n-1: right child of each internal node
+  uint32_t* ilast;         // n-1: last covered position of each internal node
+  uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
+  uint32_t* front_b;
 };
 
 // device error bits: set instead of faulting when an invariant breaks

@@ -51,47 +51,71 @@ __device__ __forceinline__ uint64_t pair_key(uint32_t a, uint32_t b, bool self) 
   return ((uint64_t)a << 32) | b;
 }
 
+// Node slabs and links read through the scalar cache: the traversal below
+// keeps the node index wave-uniform, so a constant-address-space pointer makes
+// every node load an s_load into SGPRs, and each lane compares its own query
+// slabs (VGPRs) against them.
+typedef __attribute__((address_space(4))) const Node64 CNode64;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ bool overlap_cnode(const double* q, CNode64* nd) {
+  bool o = true;
+#pragma unroll
+  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < nd->s[2 * i + 1]) & (nd->s[2 * i] < q[2 * i + 1]);
+  return o;
+}
+
+__device__ __forceinline__ void empty_slabs(double* k) {
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    k[2 * a] = __builtin_inf();
+    k[2 * a + 1] = -__builtin_inf();
+  }
+}
+
+// Element narrowphase: lane = query element at sorted position p of q; the
+// wave walks t depth first (left child on overlap, rope otherwise) visiting a
+// node when any lane's query overlaps it.
 template <bool SELF>
 __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf_size, PairOut out) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= q.n || t.n == 0) return;
-  const Node64* qn = &q.nodes[q.n - 1 + p];
+  const bool valid = p < q.n;
   double qk[kSlabs];
-  load_node_slabs(qn, qk);
-  const uint32_t qa = qn->left;
+  uint32_t qa = 0;
+  if (valid) {
+    const Node64* qn = &q.nodes[q.n - 1 + p];
+    load_node_slabs(qn, qk);
+    qa = qn->left;
+  } else {
+    empty_slabs(qk);
+  }
   const uint32_t tleaf0 = (uint32_t)(t.n - 1);
   const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
-  uint32_t node = *t.root;
+  CNode64* nodes = (CNode64*)t.nodes;
+  uint32_t node = uniform(*t.root);
   uint32_t steps = 0;
   while (node != kEnd) {
-    // every node is visited at most once in a depth-first walk
     if (node >= nnodes || ++steps > nnodes) {
-      atomicOr(t.status, kErrTraverse);
+      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
-    const Node64* nd = &t.nodes[node];
+    CNode64* nd = nodes + node;
     const uint32_t first = nd->first, last = nd->last, rope = nd->rope;
-    if (SELF && last <= (uint32_t)p) {
-      node = rope;
-      continue;
-    }
-    double nk[kSlabs];
-    load_node_slabs(nd, nk);
-    const bool hit = overlap(qk, nk);
-    if (!hit) {
+    bool hit = overlap_cnode(qk, nd);
+    if (SELF) hit = hit && last > (uint32_t)p;
+    if (__ballot(hit) == 0) {
       node = rope;
       continue;
     }
     if (node >= tleaf0) {
-      emit(out, true, pair_key(qa, nd->left, SELF));
+      emit(out, hit, pair_key(qa, nd->left, SELF));
       node = rope;
-    } else if ((int)(last - first) < leaf_size) {
+    } else if (last - first < (uint32_t)leaf_size) {
       for (uint32_t pos = first; pos <= last; ++pos) {
-        if (SELF && pos <= (uint32_t)p) continue;
-        const Node64* ln = &t.nodes[tleaf0 + pos];
-        double lk[kSlabs];
-        load_node_slabs(ln, lk);
-        const bool h = overlap(qk, lk);
+        CNode64* ln = nodes + (tleaf0 + pos);
+        bool h = hit && overlap_cnode(qk, ln);
+        if (SELF) h = h && pos > (uint32_t)p;
         emit(out, h, pair_key(qa, ln->left, SELF));
       }
       node = rope;
@@ -101,39 +125,42 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
   }
 }
 
+// Patch broadphase: lane = patch k of q (its subtree root's slabs); the walk
+// stops at the first node of t that lies in a single patch, t's patch root.
 __global__ __launch_bounds__(256) void k_patch_broadphase(TreeView q, TreeView t, PairOut out) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= q.od || t.n == 0) return;
-  const uint32_t qr = q.patch_root[k];
-  if (qr == kEnd) return;  // empty patch (broadphase.cpp:79-80)
-  if (qr >= 2 * q.n - 1) {
-    atomicOr(q.status, kErrTree);
-    return;
-  }
   double qk[kSlabs];
-  load_node_slabs(&q.nodes[qr], qk);
+  bool valid = false;
+  if (k < q.od && q.n > 0) {
+    const uint32_t qr = q.patch_root[k];
+    if (qr != kEnd && qr < 2 * q.n - 1) {
+      load_node_slabs(&q.nodes[qr], qk);
+      valid = true;
+    } else if (qr != kEnd) {
+      atomicOr(q.status, kErrTree);
+    }
+  }
+  if (!valid) empty_slabs(qk);  // empty patch (broadphase.cpp:79-80): no overlap
+  if (t.n == 0) return;
   const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
-  uint32_t node = *t.root;
+  CNode64* nodes = (CNode64*)t.nodes;
+  uint32_t node = uniform(*t.root);
   uint32_t steps = 0;
   while (node != kEnd) {
     if (node >= nnodes || ++steps > nnodes) {
-      atomicOr(t.status, kErrTraverse);
+      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
-    const Node64* nd = &t.nodes[node];
-    double nk[kSlabs];
-    load_node_slabs(nd, nk);
-    const uint32_t rope = nd->rope;
-    if (overlap(qk, nk)) {
-      const uint32_t pt = nd->patch;
-      if (pt != kMixed) {
-        emit(out, true, ((uint64_t)k << 32) | pt);
-        node = rope;
-      } else {
-        node = nd->left;
-      }
-    } else {
+    CNode64* nd = nodes + node;
+    const uint32_t rope = nd->rope, pt = nd->patch;
+    const bool hit = overlap_cnode(qk, nd);
+    if (__ballot(hit) == 0) {
       node = rope;
+    } else if (pt != kMixed) {
+      emit(out, hit, ((uint64_t)k << 32) | pt);
+      node = rope;
+    } else {
+      node = nd->left;
     }
   }
 }
