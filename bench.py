@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--od", type=int, default=256)
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-sized device buffer once (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -107,6 +110,12 @@ def main():
     from distbvh_amd import workloads as W
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy:
+        src = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     a, b = W.wavy_sheets(args.nq)
     n = a.shape[2]
     ta = torch.from_numpy(a).cuda()
@@ -147,6 +156,8 @@ def main():
             stage_sum[name] = stage_sum.get(name, 0.0) + ms
             stage_cnt[name] = stage_cnt.get(name, 0) + 1
     ms_per_step = total_ms / args.steps
+    qstats = oa.result_stats()
+    waves = (n + 63) // 64
     stages = {k: stage_sum[k] / args.steps for k in stage_sum}           # ms per step
     per_launch = {k: stage_sum[k] / stage_cnt[k] for k in stage_sum}      # ms per launch
 
@@ -176,6 +187,8 @@ def main():
         "config": {"workload": "wavy contact sheets, 2 objects x %d triangles (%dx%d quads), fp64, "
                                "26-DOP, 63-bit Morton, %d patches, leaf size %d" % (n, args.nq, args.nq, args.od, args.leaf),
                    "triangles_per_object": n, "od": args.od, "leaf_size": args.leaf, "pairs": int(npairs),
+                   "patch_pairs": int(qstats["patch_pairs"]),
+                   "node_visits_per_wave": round(qstats["node_visits"] / waves, 1),
                    "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "step_hbm_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -190,6 +190,12 @@ class CollisionObject:
         out = out[:n]
         return np.stack([out >> np.uint64(32), out & np.uint64(0xffffffff)], axis=1)
 
+    def result_stats(self):
+        v = C.c_uint64()
+        pp = C.c_uint64()
+        call("bvh_result_stats", self.handle, C.byref(v), C.byref(pp))
+        return {"node_visits": v.value, "patch_pairs": pp.value}
+
     def pairs_device(self):
         p = C.c_void_p()
         n = C.c_uint64()

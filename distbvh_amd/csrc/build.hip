@@ -142,16 +142,35 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, 
     s_last = (prev == gridDim.x - 1);
   }
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    double r[6];
-    for (int c = 0; c < 6; ++c) r[c] = __hip_atomic_load(&b.partials[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (unsigned blk = 1; blk < gridDim.x; ++blk)
+  if (!s_last) return;
+  // last block: every thread folds a strided share of the block partials
+  __threadfence();
+  double r[6];
+  for (int c = 0; c < 3; ++c) {
+    r[c] = 1.7976931348623157e308;
+    r[3 + c] = -1.7976931348623157e308;
+  }
+  for (unsigned blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      const double a = __hip_atomic_load(&b.partials[blk * 6 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double z = __hip_atomic_load(&b.partials[blk * 6 + 3 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      r[c] = sel_min(r[c], a);
+      r[3 + c] = sel_max(r[3 + c], z);
+    }
+  for (int c = 0; c < 3; ++c) {
+    r[c] = wave_min(r[c]);
+    r[3 + c] = wave_max(r[3 + c]);
+  }
+  __syncthreads();
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) s_part[w][c] = r[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 6; ++c) r[c] = s_part[0][c];
+    for (int ww = 1; ww < kPrepBlock / 64; ++ww)
       for (int c = 0; c < 3; ++c) {
-        double a = __hip_atomic_load(&b.partials[blk * 6 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double z = __hip_atomic_load(&b.partials[blk * 6 + 3 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        r[c] = sel_min(r[c], a);
-        r[3 + c] = sel_max(r[3 + c], z);
+        r[c] = sel_min(r[c], s_part[ww][c]);
+        r[3 + c] = sel_max(r[3 + c], s_part[ww][3 + c]);
       }
     Bounds bd;
     for (int c = 0; c < 3; ++c) {
@@ -453,6 +472,8 @@ __device__ __forceinline__ void node_load(const BuildBufs& b, uint32_t id, typen
     const Node64* nd = &b.nodes[id];
     load_slabs(nd, k);
     first = nd->first; last = nd->last; patch = nd->patch;
+    // keep all 14 loads in flight: one memory round trip per tree level
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -623,8 +644,7 @@ __global__ __launch_bounds__(256) void k_ropes(BuildBufs b, uint64_t n) {
         rope = kEnd;
       }
     }
-    if constexpr (FP32) b.nodes32[id].rope = rope;
-    else b.nodes[id].rope = rope;
+    b.ropes[id] = rope;
   }
 }
 

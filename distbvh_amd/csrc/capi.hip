@@ -144,7 +144,7 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status, rightc, ilast, front_a, front_b;
+      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status, rightc, ilast, front_a, front_b, ropes;
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;
@@ -152,6 +152,7 @@ struct bvh_object_s {
   DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count;
   uint64_t npairs = 0;
   uint64_t npatch_pairs = 0;
+  uint64_t nvisits = 0;
   const uint64_t* result_ptr = nullptr;  // device pointer to the (sorted) pairs
   bvh_object_s* result_other = nullptr;
   bool self_result = false;
@@ -170,6 +171,7 @@ static TreeView view_of(const bvh_object_s* o) {
   v.n = o->in.n;
   v.od = o->w->od;
   v.status = o->status.as<uint32_t>();
+  v.ropes = o->ropes.as<uint32_t>();
   return v;
 }
 
@@ -213,6 +215,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->root.ensure(4));
   BVH_TRY(o->status.ensure(2 * sizeof(uint32_t)));  // [error bits, last-block counter]
   BVH_TRY(o->rightc.ensure(n1 * 4));
+  BVH_TRY(o->ropes.ensure((2 * n1 - 1) * 4));
   BVH_TRY(o->ilast.ensure(n1 * 4));
   BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
@@ -255,6 +258,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.ilast = o->ilast.as<uint32_t>();
   b.front_a = o->front_a.as<uint32_t>();
   b.front_b = o->front_b.as<uint32_t>();
+  b.ropes = o->ropes.as<uint32_t>();
   return BVH_OK;
 }
 
@@ -371,11 +375,12 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
 static bvh_status read_count(bvh_object_s* o, bvh_object_s* other, unsigned long long* dcount, uint64_t* out) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
-  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[1], o->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[2], other->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[2], o->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[3], other->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  const uint32_t st = (uint32_t)w->h_count[1] | (uint32_t)w->h_count[2];
+  o->nvisits = w->h_count[1];
+  const uint32_t st = (uint32_t)w->h_count[2] | (uint32_t)w->h_count[3];
   if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
   *out = w->h_count[0];
   return BVH_OK;
@@ -409,7 +414,7 @@ static bvh_status sort_results(bvh_object_s* o) {
   return BVH_OK;
 }
 
-static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self) {
+static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
@@ -429,7 +434,7 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self) {
         cap = pcap;
       }
       HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2, 0u, s));
-      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount};
+      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount, nullptr};
       HIP_TRY(launch_patch_broadphase(qv, tv, po, s));
       BVH_TRY(read_count(o, other, dcount, &o->npatch_pairs));
       if (o->npatch_pairs <= cap) break;
@@ -439,14 +444,23 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self) {
   } else {
     o->npatch_pairs = 0;
   }
+  if (!narrow) {
+    o->npairs = 0;
+    o->nvisits = 0;
+    o->result_ptr = o->pairs.as<uint64_t>();
+    o->result_other = other;
+    o->self_result = false;
+    o->have_results = true;
+    return BVH_OK;
+  }
   // element narrowphase
   w->stage_begin("narrowphase");
   uint64_t want = std::max<uint64_t>(1u << 20, 8 * o->in.n);
   if (o->pairs.bytes / 8 < want) BVH_TRY(ensure_pairs(o, want));
   for (int attempt = 0; attempt < 2; ++attempt) {
     const uint64_t cap = o->pairs.bytes / 8;
-    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2, 0u, s));
-    PairOut po{o->pairs.as<uint64_t>(), cap, dcount};
+    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 4, 0u, s));
+    PairOut po{o->pairs.as<uint64_t>(), cap, dcount, dcount + 1};
     HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, 0, po, s));
     BVH_TRY(read_count(o, other, dcount, &o->npairs));
     if (o->npairs <= cap) break;
@@ -680,6 +694,21 @@ bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
   return run_query(o, other, false);
 }
 
+bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other) {
+  if (!o || !other) return fail(BVH_E_INVALID, "null object");
+  if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
+  if (o == other) {
+    o->npairs = 0;
+    o->npatch_pairs = 0;
+    o->have_results = true;
+    o->result_ptr = o->pairs.as<uint64_t>();
+    o->result_other = o;
+    o->self_result = true;
+    return BVH_OK;
+  }
+  return run_query(o, other, false, false);
+}
+
 bvh_status bvh_self_collision(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   return run_query(o, o, true);
@@ -724,6 +753,13 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_
                          o->has_gids ? o->gids.as<uint64_t>() : nullptr, q->patch_of_input.as<uint32_t>(),
                          q->has_gids ? q->gids.as<uint64_t>() : nullptr, 0, 0, o->records.as<uint64_t>(), s));
   return copy_out(out, o->records.p, n * sizeof(bvh_narrowphase_record), mem, s);
+}
+
+bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  if (node_visits) *node_visits = o->have_results ? o->nvisits : 0;
+  if (patch_pairs) *patch_pairs = o->have_results ? o->npatch_pairs : 0;
+  return BVH_OK;
 }
 
 bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {

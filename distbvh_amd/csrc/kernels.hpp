@@ -93,6 +93,7 @@ struct BuildBufs {
   uint32_t* ilast;         // n-1: last covered position of each internal node
   uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
   uint32_t* front_b;
+  uint32_t* ropes;         // 2n-1: escape link of every node
 };
 
 // device error bits: set instead of faulting when an invariant breaks
@@ -122,12 +123,14 @@ struct TreeView {
   uint64_t n;
   uint64_t od;
   uint32_t* status;
+  const uint32_t* ropes;
 };
 
 struct PairOut {
   uint64_t* pairs;      // (input index a << 32) | input index b
   uint64_t cap;
   unsigned long long* count;
+  unsigned long long* visits;  // optional: node visits summed over waves
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)

@@ -74,12 +74,53 @@ __device__ __forceinline__ void empty_slabs(double* k) {
   }
 }
 
+// Work-group b runs on XCD b % 8 (round-robin dispatch).  Tile t = xcd_tile(b)
+// gives each XCD a contiguous range of tiles, so queries that are adjacent in
+// the sorted order (and walk the same part of the tree) share one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, j = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return x * per + (x < rem ? x : rem) + j;
+}
+
+// Per-wave pair staging in LDS: hits are appended with a ballot (all lanes of
+// the wave are converged in the uniform traversal loop) and written to
+// global memory in coalesced runs with one atomic per flush.
+constexpr uint32_t kWaveBuf = 128;
+
+__device__ __forceinline__ void wave_flush(uint64_t* buf, uint32_t& fill, const PairOut& out) {
+  if (fill == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(out.count, (unsigned long long)fill);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+  base = ((unsigned long long)hi << 32) | lo;
+  for (uint32_t i = lane_id(); i < fill; i += kWave)
+    if (base + i < out.cap) out.pairs[base + i] = buf[i];
+  __builtin_amdgcn_wave_barrier();
+  fill = 0;
+}
+
+__device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const PairOut& out, bool pred,
+                                          uint64_t key) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t cnt = (uint32_t)__popcll(m);
+  if (fill + cnt > kWaveBuf) wave_flush(buf, fill, out);
+  if (pred) buf[fill + (uint32_t)__popcll(m & lanemask_lt())] = key;
+  fill += cnt;
+}
+
 // Element narrowphase: lane = query element at sorted position p of q; the
 // wave walks t depth first (left child on overlap, rope otherwise) visiting a
 // node when any lane's query overlaps it.
 template <bool SELF>
 __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf_size, PairOut out) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  uint64_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t fill = 0;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
   double qk[kSlabs];
   uint32_t qa = 0;
@@ -93,6 +134,7 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
   const uint32_t tleaf0 = (uint32_t)(t.n - 1);
   const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
   CNode64* nodes = (CNode64*)t.nodes;
+  const __attribute__((address_space(4))) uint32_t* ropes = (const __attribute__((address_space(4))) uint32_t*)t.ropes;
   uint32_t node = uniform(*t.root);
   uint32_t steps = 0;
   while (node != kEnd) {
@@ -101,7 +143,7 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
       break;
     }
     CNode64* nd = nodes + node;
-    const uint32_t first = nd->first, last = nd->last, rope = nd->rope;
+    const uint32_t first = nd->first, last = nd->last, rope = ropes[node];
     bool hit = overlap_cnode(qk, nd);
     if (SELF) hit = hit && last > (uint32_t)p;
     if (__ballot(hit) == 0) {
@@ -109,59 +151,63 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
       continue;
     }
     if (node >= tleaf0) {
-      emit(out, hit, pair_key(qa, nd->left, SELF));
+      wave_emit(buf, fill, out, hit, pair_key(qa, nd->left, SELF));
       node = rope;
     } else if (last - first < (uint32_t)leaf_size) {
       for (uint32_t pos = first; pos <= last; ++pos) {
         CNode64* ln = nodes + (tleaf0 + pos);
         bool h = hit && overlap_cnode(qk, ln);
         if (SELF) h = h && pos > (uint32_t)p;
-        emit(out, h, pair_key(qa, ln->left, SELF));
+        wave_emit(buf, fill, out, h, pair_key(qa, ln->left, SELF));
       }
       node = rope;
     } else {
       node = nd->left;
     }
   }
+  wave_flush(buf, fill, out);
+  if (out.visits && lane_id() == 0) atomicAdd(out.visits, (unsigned long long)steps);
 }
 
-// Patch broadphase: lane = patch k of q (its subtree root's slabs); the walk
-// stops at the first node of t that lies in a single patch, t's patch root.
-__global__ __launch_bounds__(256) void k_patch_broadphase(TreeView q, TreeView t, PairOut out) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double qk[kSlabs];
-  bool valid = false;
-  if (k < q.od && q.n > 0) {
-    const uint32_t qr = q.patch_root[k];
-    if (qr != kEnd && qr < 2 * q.n - 1) {
-      load_node_slabs(&q.nodes[qr], qk);
-      valid = true;
-    } else if (qr != kEnd) {
-      atomicOr(q.status, kErrTree);
+// Patch broadphase over all patch pairs: a 256 x 256 block of (patch of q,
+// patch of t); the t patch slabs are staged in LDS and read as broadcasts.
+// Empty patches never overlap (broadphase.cpp:79-80).
+__global__ __launch_bounds__(256) void k_patch_pairs(TreeView q, TreeView t, PairOut out) {
+  __shared__ double s_k[256][kSlabs + 1];
+  __shared__ uint32_t s_ok[256];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t qp = (uint64_t)blockIdx.x * 256 + tid;
+  const uint64_t tp = (uint64_t)blockIdx.y * 256 + tid;
+  // stage the t patches of this block
+  s_ok[tid] = 0;
+  if (tp < t.od && t.n > 0) {
+    const uint32_t r = t.patch_root[tp];
+    if (r != kEnd && r < 2 * t.n - 1) {
+      double k[kSlabs];
+      load_node_slabs(&t.nodes[r], k);
+      for (int i = 0; i < kSlabs; ++i) s_k[tid][i] = k[i];
+      s_ok[tid] = 1;
     }
   }
-  if (!valid) empty_slabs(qk);  // empty patch (broadphase.cpp:79-80): no overlap
-  if (t.n == 0) return;
-  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
-  CNode64* nodes = (CNode64*)t.nodes;
-  uint32_t node = uniform(*t.root);
-  uint32_t steps = 0;
-  while (node != kEnd) {
-    if (node >= nnodes || ++steps > nnodes) {
-      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
-      break;
+  double qk[kSlabs];
+  bool valid = false;
+  if (qp < q.od && q.n > 0) {
+    const uint32_t r = q.patch_root[qp];
+    if (r != kEnd && r < 2 * q.n - 1) {
+      load_node_slabs(&q.nodes[r], qk);
+      valid = true;
     }
-    CNode64* nd = nodes + node;
-    const uint32_t rope = nd->rope, pt = nd->patch;
-    const bool hit = overlap_cnode(qk, nd);
-    if (__ballot(hit) == 0) {
-      node = rope;
-    } else if (pt != kMixed) {
-      emit(out, hit, ((uint64_t)k << 32) | pt);
-      node = rope;
-    } else {
-      node = nd->left;
+  }
+  if (!valid) empty_slabs(qk);
+  __syncthreads();
+  const uint32_t nt = (uint32_t)((t.od - (uint64_t)blockIdx.y * 256) < 256 ? (t.od - (uint64_t)blockIdx.y * 256) : 256);
+  for (uint32_t j = 0; j < nt; ++j) {
+    bool h = valid && s_ok[j];
+    if (h) {
+#pragma unroll
+      for (int a = 0; a < kAxes; ++a) h = h & (qk[2 * a] < s_k[j][2 * a + 1]) & (s_k[j][2 * a] < qk[2 * a + 1]);
     }
+    emit(out, h, (qp << 32) | ((uint64_t)blockIdx.y * 256 + j));
   }
 }
 
@@ -220,9 +266,9 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
 }
 
 hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s) {
-  if (q.od == 0 || t.n == 0 || q.n == 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((q.od + 255) / 256);
-  hipLaunchKernelGGL(k_patch_broadphase, dim3(blocks), dim3(256), 0, s, q, t, out);
+  if (q.od == 0 || t.od == 0 || t.n == 0 || q.n == 0) return hipSuccess;
+  const dim3 grid((unsigned)((q.od + 255) / 256), (unsigned)((t.od + 255) / 256));
+  hipLaunchKernelGGL(k_patch_pairs, grid, dim3(256), 0, s, q, t, out);
   return hipGetLastError();
 }
 

@@ -148,6 +148,10 @@ bvh_status bvh_init_broadphase(bvh_object o);
  * narrowphase (strict 26-DOP overlap, kdop.hpp:538-549).  Results land in
  * `o` (impl.cpp:328-348 sends both halves to the initiating object). */
 bvh_status bvh_broadphase(bvh_object o, bvh_object other);
+/* the broadphase half of broadphase(): patch pairs only, for callers that
+ * run their own narrowphase per patch pair (the reference's narrowphase
+ * functor, collision_world.hpp:75-94); the result count is then 0. */
+bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other);
 /* self_collision_set(tree), collision_query.hpp:360-368 at element level:
  * unordered element pairs (i < j) of one object with overlapping 26-DOPs. */
 bvh_status bvh_self_collision(bvh_object o);
@@ -162,6 +166,9 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record *out, uint64_
 bvh_status bvh_result_pairs(bvh_object o, uint64_t *out, uint64_t cap, uint32_t mem);
 /* device pointer to the compact pairs (valid until the next broadphase) */
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_t *count);
+/* counters of the last query: tree nodes visited (summed over waves of 64
+ * queries) and broadphase patch pairs */
+bvh_status bvh_result_stats(bvh_object o, uint64_t *node_visits, uint64_t *patch_pairs);
 /* broadphase patch pairs (query_tree emits, broadphase.cpp:93-118):
  * out[2k] = patch of o, out[2k+1] = patch of other. */
 bvh_status bvh_patch_pair_count(bvh_object o, uint64_t *count);
