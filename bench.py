@@ -26,13 +26,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 256      # one fp64 tree node (208 B slabs + links, padded)
-SLAB_BYTES = 208      # 13 fp64 (min, max) pairs
 
 
-def kernel_bytes(stage, n, npairs, n_other):
+def kernel_bytes(stage, n, npairs, n_other, fp32=False):
     """Algorithmic (compulsory) HBM bytes of one launch of a stage for an
-    object of n triangles (3 fp64 vertices = 72 B)."""
+    object of n triangles (3 fp64 vertices = 72 B).  Node slabs are 13
+    (min, max) pairs: 208 B in fp64, 104 B in the fp32 form."""
+    SLAB_BYTES = 104 if fp32 else 208
     if stage == "bounds":      # read vertices, write centroid
         return n * (72 + 24)
     if stage == "morton":      # read centroid, write code
@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
     ap.add_argument("--od", type=int, default=256)
     ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="fp64 tree, or the fp32-outward tree with an exact fp64 re-check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized device buffer once (the reference kernel for "
@@ -121,8 +123,9 @@ def main():
     ta = torch.from_numpy(a).cuda()
     tb = torch.from_numpy(b).cuda()
     stream = torch.cuda.current_stream()
+    fp32 = args.precision == "fp32"
     world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
-                           stream=stream.cuda_stream)
+                           stream=stream.cuda_stream, precision=args.precision)
     oa = world.create_collision_object()
     ob = world.create_collision_object()
 
@@ -163,13 +166,13 @@ def main():
 
     # dominant kernel: largest share of the step
     dom = max(stages, key=lambda k: stages[k])
-    nbytes = kernel_bytes(dom, n, npairs, n)
+    nbytes = kernel_bytes(dom, n, npairs, n, fp32)
     t_launch = per_launch[dom] * 1e-3
     achieved = nbytes / t_launch / 1e9 if t_launch > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes": int(nbytes)}
-    step_bytes = sum(kernel_bytes(k, n, npairs, n) * stage_cnt[k] / args.steps for k in stages)
+    step_bytes = sum(kernel_bytes(k, n, npairs, n, fp32) * stage_cnt[k] / args.steps for k in stages)
 
     out = {
         "metric": "ms/step build+broad+narrow @2M tris/object, %HBM peak; batched p99 latency",
@@ -182,10 +185,13 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
         "config": {"workload": "wavy contact sheets, 2 objects x %d triangles (%dx%d quads), fp64, "
                                "26-DOP, 63-bit Morton, %d patches, leaf size %d" % (n, args.nq, args.nq, args.od, args.leaf),
+                   "precision": "fp32 tree, exact fp64 re-check" if fp32 else "fp64",
+                   "candidate_pairs": int(qstats["candidate_pairs"]),
+                   "rechecked_pairs": int(qstats["rechecked_pairs"]),
                    "triangles_per_object": n, "od": args.od, "leaf_size": args.leaf, "pairs": int(npairs),
                    "patch_pairs": int(qstats["patch_pairs"]),
                    "node_visits_per_wave": round(qstats["node_visits"] / waves, 1),

@@ -67,6 +67,7 @@ SIGNATURES = {
     "bvh_set_entity_data_poly": [_vp, _vp, _u32, _u64, _vp, _d, _u32, _u32],
     "bvh_set_entity_data_sphere": [_vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_set_entity_data_snapshot": [_vp, _vp, _vp, _u64, _vp, _u32, _u32],
+    "bvh_refit": [_vp, _vp, _u32],
     "bvh_init_broadphase": [_vp],
     "bvh_broadphase": [_vp, _vp],
     "bvh_broadphase_patches": [_vp, _vp],
@@ -76,7 +77,7 @@ SIGNATURES = {
     "bvh_result_records": [_vp, _vp, _u64, _u32],
     "bvh_result_pairs": [_vp, _vp, _u64, _u32],
     "bvh_result_pairs_device": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
-    "bvh_result_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64)],
+    "bvh_result_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_d)],
     "bvh_patch_pair_count": [_vp, C.POINTER(_u64)],
     "bvh_patch_pairs": [_vp, _vp, _u64],
     "bvh_local_patches": [_vp, _vp, _sz],

@@ -152,6 +152,14 @@ class CollisionObject:
         self.n = n
         call("bvh_set_entity_data_snapshot", self.handle, p, c, n, g, mem, split_algorithm)
 
+    def refit(self, data):
+        """New element data of the last set_entity_data's kind and count
+        (same array shape); the tree keeps its topology and every k-DOP is
+        recomputed (bvh_refit)."""
+        p, mem, keep = _ptr_mem(data, np.float64)
+        self._keep = (keep,) + tuple(self._keep[1:])
+        call("bvh_refit", self.handle, p, mem)
+
     def init_broadphase(self):
         call("bvh_init_broadphase", self.handle)
 
@@ -193,8 +201,12 @@ class CollisionObject:
     def result_stats(self):
         v = C.c_uint64()
         pp = C.c_uint64()
-        call("bvh_result_stats", self.handle, C.byref(v), C.byref(pp))
-        return {"node_visits": v.value, "patch_pairs": pp.value}
+        cand = C.c_uint64()
+        rck = C.c_uint64()
+        err = C.c_double()
+        call("bvh_result_stats", self.handle, C.byref(v), C.byref(pp), C.byref(cand), C.byref(rck), C.byref(err))
+        return {"node_visits": v.value, "patch_pairs": pp.value, "candidate_pairs": cand.value,
+                "rechecked_pairs": rck.value, "max_slab_error_ulps": err.value}
 
     def pairs_device(self):
         p = C.c_void_p()

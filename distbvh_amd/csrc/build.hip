@@ -329,88 +329,6 @@ __global__ __launch_bounds__(256) void k_delta(const K* __restrict__ codes, uint
   }
 }
 
-// next representable float below / above a finite f (IEEE-754 bit stepping)
-__device__ __forceinline__ float next_down(float f) {
-  if (!(f == f) || f == -__builtin_inff()) return f;
-  if (f == 0.0f) return __uint_as_float(0x80000001u);
-  const uint32_t u = __float_as_uint(f);
-  return __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
-}
-__device__ __forceinline__ float next_up(float f) {
-  if (!(f == f) || f == __builtin_inff()) return f;
-  if (f == 0.0f) return __uint_as_float(0x00000001u);
-  const uint32_t u = __float_as_uint(f);
-  return __uint_as_float(f > 0.0f ? u + 1u : u - 1u);
-}
-__device__ __forceinline__ float round_down_1ulp(double x) { return next_down((float)x); }
-__device__ __forceinline__ float round_up_1ulp(double x) { return next_up((float)x); }
-
-template <int KIND, int NV>
-__device__ __forceinline__ void element_kdop(const ElemInput& in, uint32_t e, double* kd) {
-  const uint64_t n = in.n;
-  if constexpr (KIND == kPoly) {
-    double v[NV][3];
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) v[j][c] = in.data[(uint64_t)(3 * j + c) * n + e];
-    kdop_from_vertices<NV>(v, in.eps, kd);
-  } else if constexpr (KIND == kSphere) {
-    kdop_from_sphere(in.data[e], in.data[n + e], in.data[2 * n + e], in.data[3 * n + e], kd);
-  } else {
-    const double2* k = (const double2*)(in.data + (uint64_t)e * kSlabs);
-#pragma unroll
-    for (int a = 0; a < kAxes; ++a) {
-      double2 t = k[a];
-      kd[2 * a] = t.x;
-      kd[2 * a + 1] = t.y;
-    }
-  }
-}
-
-__device__ __forceinline__ void store_slabs(Node64* nd, const double* kd) {
-  double2* d = (double2*)nd->s;
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) d[a] = make_double2(kd[2 * a], kd[2 * a + 1]);
-}
-__device__ __forceinline__ void load_slabs(const Node64* nd, double* kd) {
-  const double2* d = (const double2*)nd->s;
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) {
-    double2 t = d[a];
-    kd[2 * a] = t.x;
-    kd[2 * a + 1] = t.y;
-  }
-}
-
-// fp32 outward slabs relative to the object origin: the fp64 slab minus the
-// origin's projection (fp64), rounded to nearest fp32 and moved one ulp out.
-__device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) {
-    kf[2 * a] = round_down_1ulp(kd[2 * a] - po[a]);
-    kf[2 * a + 1] = round_up_1ulp(kd[2 * a + 1] - po[a]);
-  }
-}
-
-__device__ __forceinline__ void store_slabs32(Node32* nd, const float* kf) {
-  float4* d = (float4*)nd->s;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) d[q] = make_float4(kf[4 * q], kf[4 * q + 1], kf[4 * q + 2], kf[4 * q + 3]);
-  nd->s[24] = kf[24];
-  nd->s[25] = kf[25];
-}
-__device__ __forceinline__ void load_slabs32(const Node32* nd, float* kf) {
-  const float4* d = (const float4*)nd->s;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    float4 t = d[q];
-    kf[4 * q] = t.x; kf[4 * q + 1] = t.y; kf[4 * q + 2] = t.z; kf[4 * q + 3] = t.w;
-  }
-  kf[24] = nd->s[24];
-  kf[25] = nd->s[25];
-}
-
 __device__ __forceinline__ uint32_t patch_begin(const uint32_t* splits, uint64_t od, uint32_t k) {
   return k == 0 ? 0u : splits[k - 1];
 }
@@ -548,53 +466,223 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t o
 
 constexpr uint32_t kTreeChunk = 256;
 
+__device__ __forceinline__ void empty_kdop(double* kd) {
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    kd[2 * a] = __builtin_inf();
+    kd[2 * a + 1] = -__builtin_inf();
+  }
+}
+
+// fp32 form: exact (fp64) patch slabs, the union of the patch's element
+// k-DOPs, kept beside the fp32 tree for the patch broadphase.  Called by every
+// lane of the wave; lanes without an element pass pk = kEnd and the empty
+// k-DOP.  A wave whose elements all lie in one patch reduces in registers and
+// lane 0 issues the 26 atomics; otherwise each lane issues its own.
+__device__ __forceinline__ void patch_union64(unsigned long long* pk64, uint32_t pk, const double* kd) {
+  const uint64_t m = __ballot(pk != kEnd);
+  if (m == 0) return;
+  const uint32_t pk0 = __builtin_amdgcn_readlane(pk, __ffsll((long long)m) - 1);
+  if (__ballot(pk != kEnd && pk != pk0) == 0) {
+    unsigned long long* d = pk64 + (uint64_t)pk0 * kSlabs;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      // one slab pair at a time: the reductions are not interleaved, which
+      // keeps the register count of the leaf kernel down
+      const double lo = wave_min(kd[2 * a]);
+      const double hi = wave_max(kd[2 * a + 1]);
+      if (lane_id() == 0) {
+        atomicMin(d + 2 * a, (unsigned long long)ordered_u64(lo));
+        atomicMax(d + 2 * a + 1, (unsigned long long)ordered_u64(hi));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else if (pk != kEnd) {
+    unsigned long long* d = pk64 + (uint64_t)pk * kSlabs;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      atomicMin(d + 2 * a, (unsigned long long)ordered_u64(kd[2 * a]));
+      atomicMax(d + 2 * a + 1, (unsigned long long)ordered_u64(kd[2 * a + 1]));
+    }
+  }
+}
+
+__global__ void k_set_frame(const Bounds* bounds, double* frame) {
+  uint32_t* flag = (uint32_t*)(frame + 3);
+  if (threadIdx.x != 0 || *flag != 0) return;
+  for (int c = 0; c < 3; ++c) {
+    const double v = bounds->min[c];
+    frame[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
+  }
+  *flag = 1;
+}
+
+__global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) pk64[i] = (i & 1) ? 0ull : ~0ull;
+}
+
+// Phase-1 state of one 256-leaf chunk, all in LDS.  Slot j holds the newest
+// finished node whose range starts at position c0 + j (its slabs, node id,
+// last position, patch); endslot[j] is the slot of the newest finished node
+// that ends at c0 + j.  A parent is written to its first position's slot, so
+// a slot is only overwritten by the thread that has just read it.
+template <bool FP32>
+struct ChunkLds {
+  typename Slab<FP32>::T slab[kSlabs][kTreeChunk];  // SoA: conflict-free stores
+  uint32_t cnt[kTreeChunk];      // arrivals at internal node c0 + j
+  uint32_t id[kTreeChunk];
+  uint32_t last[kTreeChunk];
+  uint32_t patch[kTreeChunk];
+  uint32_t endslot[kTreeChunk];
+  uint8_t delta[kTreeChunk + 1];  // delta[c0 - 1 + j]
+  uint32_t front[2];
+  uint32_t nfront;
+  uint32_t err;
+};
+
+template <bool FP32>
+__device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32_t endslot,
+                                        const typename Slab<FP32>::T* kd, uint32_t id, uint32_t last,
+                                        uint32_t patch) {
+#pragma unroll
+  for (int i = 0; i < kSlabs; ++i) L.slab[i][slot] = kd[i];
+  L.id[slot] = id;
+  L.last[slot] = last;
+  L.patch[slot] = patch;
+  L.endslot[endslot] = slot;
+}
+
+// Phase-1 climb inside the chunk [c0, c1): the sibling's slabs, id and range
+// come from LDS and arrivals are LDS atomics whose fences order LDS only, so a
+// level costs LDS latency; node links and slabs go to global memory with
+// stores that nothing in this kernel waits for.
+template <bool FP32>
+__device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
+                                            uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t c0,
+                                            uint32_t c1, ChunkLds<FP32>& L) {
+  for (;;) {
+    if (l == 0 && r == n - 1) {
+      *b.root = node;
+      return;
+    }
+    // delta[r] = L.delta[r - c0 + 1], delta[l - 1] = L.delta[l - c0]
+    const bool is_left = (l == 0) || (r != n - 1 && L.delta[r - c0 + 1] < L.delta[l - c0]);
+    const uint32_t parent = is_left ? r : l - 1;
+    if (parent < c0 || parent + 1 >= c1) {
+      const uint32_t k = atomicAdd(&L.nfront, 1u);
+      if (k < 2) L.front[k] = node;
+      else atomicOr(b.status, kErrTree);
+      return;
+    }
+    node_set_child<FP32>(b, parent, is_left, node);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const uint32_t old = __hip_atomic_fetch_add(&L.cnt[parent - c0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (old == 0) return;  // first arriver: the sibling finishes the parent
+    if (old != 1) {
+      atomicOr(b.status, kErrTree);
+      return;
+    }
+    // the sibling starts at r + 1 (left child) or ends at l - 1 (right child)
+    const uint32_t ss = is_left ? (r + 1 - c0) : L.endslot[l - 1 - c0];
+    if (ss >= kTreeChunk) {
+      atomicOr(b.status, kErrTree);
+      return;
+    }
+    typename Slab<FP32>::T ks[kSlabs];
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][ss];
+    const uint32_t slast = L.last[ss], spatch = L.patch[ss];
+    slab_union<FP32>(kd, ks);
+    if (is_left) r = slast;
+    else l = c0 + ss;
+    mypatch = (mypatch == spatch) ? mypatch : kMixed;
+    node_store<FP32>(b, parent, kd, l, r, mypatch);
+    b.ilast[parent] = r;
+    if (mypatch != kMixed && patch_begin(b.splits, od, mypatch) == l && patch_end(b.splits, od, n, mypatch) == r + 1)
+      b.patch_root[mypatch] = parent;
+    publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
+    node = parent;
+  }
+}
+
 template <int KIND, int NV, bool FP32>
 __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildBufs b, uint64_t od) {
-  __shared__ uint32_t s_cnt[kTreeChunk];
-  __shared__ uint32_t s_front[2];
-  __shared__ uint32_t s_nfront;
+  __shared__ ChunkLds<FP32> L;
   const uint64_t n = in.n;
   const uint32_t t = threadIdx.x;
   const uint32_t c0 = blockIdx.x * kTreeChunk;
   const uint32_t c1 = (uint32_t)((uint64_t)c0 + kTreeChunk < n ? c0 + kTreeChunk : n);
-  s_cnt[t] = 0;
-  if (t == 0) s_nfront = 0;
-  __syncthreads();
-  const uint32_t p = c0 + t;
-  if (p < c1) {
-    const uint32_t e = b.perm[p];
-    if (e < n) {
-      double kd[kSlabs];
-      element_kdop<KIND, NV>(in, e, kd);
-      const uint32_t pk = b.patch_of_pos[p];
-      b.patch_of_input[e] = pk;
-      const uint32_t leaf = (uint32_t)(n - 1 + p);
-      if (patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
-      if constexpr (FP32) {
-        double po[kAxes];
-        float kf[kSlabs];
-        proj_all(b.origin[0], b.origin[1], b.origin[2], po);
-        to_fp32_outward(kd, po, kf);
-        node_store<true>(b, leaf, kf, p, p, pk);
-        b.nodes32[leaf].left = e;
-        b.nodes32[leaf].right = kEnd;
-        if (n == 1) *b.root = leaf;
-        else climb<true, true>(b, n, od, leaf, p, p, pk, kf, c0, c1, s_cnt, s_front, &s_nfront);
-      } else {
-        node_store<false>(b, leaf, kd, p, p, pk);
-        b.nodes[leaf].left = e;
-        b.nodes[leaf].right = kEnd;
-        if (n == 1) *b.root = leaf;
-        else climb<false, true>(b, n, od, leaf, p, p, pk, kd, c0, c1, s_cnt, s_front, &s_nfront);
-      }
-    } else {
-      atomicOr(b.status, kErrIndex);
+  L.cnt[t] = 0;
+  {
+    // delta[c0 - 1 + t] for t in [0, 256], where the index is in [0, n - 2]
+    const uint64_t di = (uint64_t)c0 + t;  // = (c0 - 1 + t) + 1
+    L.delta[t] = (di >= 1 && di - 1 < n - 1) ? b.delta[di - 1] : 0;
+    if (t == 0) {
+      const uint64_t dj = (uint64_t)c0 + kTreeChunk;
+      L.delta[kTreeChunk] = (dj - 1 < n - 1) ? b.delta[dj - 1] : 0;
+      L.nfront = 0;
+      L.err = 0;
     }
   }
+  const uint32_t p = c0 + t;
+  const uint32_t e = p < c1 ? b.perm[p] : kEnd;
+  if (p < c1 && e >= n) atomicOr(b.status, kErrIndex);
+  const bool act = p < c1 && e < n;
+  double kd[kSlabs];
+  uint32_t pk = kEnd;
+  if (act) {
+    element_kdop<KIND, NV>(in, e, kd);
+    pk = b.patch_of_pos[p];
+  } else {
+    empty_kdop(kd);
+  }
+  if constexpr (FP32) patch_union64(b.patch_k64, pk, kd);
+  typename Slab<FP32>::T ks[kSlabs];
+  const uint32_t leaf = (uint32_t)(n - 1 + p);
+  if (act) {
+    b.patch_of_input[e] = pk;
+    if (patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
+    if constexpr (FP32) {
+      double po[kAxes];
+      proj_all(b.origin[0], b.origin[1], b.origin[2], po);
+      to_fp32_outward(kd, po, ks);
+      // largest distance of an fp32 slab from its fp64 value, in fp32 ulps
+      float err = 0.0f;
+#pragma unroll
+      for (int i = 0; i < kSlabs; ++i) {
+        const double exact = kd[i] - po[i / 2];
+        const float mag = __builtin_fabsf(ks[i]);
+        const double ulp = (double)(next_up(mag) - mag);
+        const float d = (float)(__builtin_fabs((double)ks[i] - exact) / ulp);
+        err = d > err ? d : err;
+      }
+      atomicMax(&L.err, __float_as_uint(err));
+    } else {
+#pragma unroll
+      for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
+    }
+    node_store<FP32>(b, leaf, ks, p, p, pk);
+    if constexpr (FP32) {
+      b.nodes32[leaf].left = e;
+      b.nodes32[leaf].right = kEnd;
+    } else {
+      b.nodes[leaf].left = e;
+      b.nodes[leaf].right = kEnd;
+    }
+    publish<FP32>(L, t, t, ks, leaf, p, pk);
+  }
+  __syncthreads();  // counters, deltas and leaf slots ready
+  if (act) {
+    if (n == 1) *b.root = leaf;
+    else climb_chunk<FP32>(b, n, od, leaf, p, p, pk, ks, c0, c1, L);
+  }
   __syncthreads();
+  if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);
   // a first arrival whose sibling lies outside the chunk keeps its count
-  if (c0 + t + 1 < c1 && s_cnt[t] == 1) b.counters[c0 + t] = 1;
-  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
+  if (c0 + t + 1 < c1 && L.cnt[t] == 1) b.counters[c0 + t] = 1;
+  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < L.nfront) ? L.front[t] : kEnd;
 }
 
 template <bool FP32>
@@ -794,6 +882,18 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     chunk = next;
   }
   return hipSuccess;
+}
+
+hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(64), 0, s, bounds, frame);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_k64_init(unsigned long long* pk64, uint64_t od, hipStream_t s) {
+  const uint64_t count = od * kSlabs;
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_patch_k64_init, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, pk64, count);
+  return hipGetLastError();
 }
 
 hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s) {

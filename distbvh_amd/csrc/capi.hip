@@ -101,6 +101,8 @@ struct bvh_world_s {
   size_t events_used = 0;
   // primitive-call scratch
   DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
+  // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
+  DevBuf frame;
   unsigned long long* h_count = nullptr;  // pinned
   bool iteration_open = false;
 
@@ -144,15 +146,19 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, nodes, nodes32, origin, status, rightc, ilast, front_a, front_b, ropes;
+      patch_of_input, counters, patch_root, root, nodes, nodes32, status, rightc, ilast, front_a, front_b, ropes,
+      patch_k64;
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;
   // results
-  DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count;
+  DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count, recheck;
   uint64_t npairs = 0;
   uint64_t npatch_pairs = 0;
   uint64_t nvisits = 0;
+  uint64_t ncandidates = 0;
+  uint64_t nrecheck = 0;  // fp32 candidates re-checked in fp64
+  uint32_t slab_err_bits = 0;  // max fp32 slab error (ulps, float bits) of the queried objects
   const uint64_t* result_ptr = nullptr;  // device pointer to the (sorted) pairs
   bvh_object_s* result_other = nullptr;
   bool self_result = false;
@@ -172,6 +178,9 @@ static TreeView view_of(const bvh_object_s* o) {
   v.od = o->w->od;
   v.status = o->status.as<uint32_t>();
   v.ropes = o->ropes.as<uint32_t>();
+  v.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
+  v.elem = o->in;
+  v.patch_k64 = o->fp32 ? o->patch_k64.as<unsigned long long>() : nullptr;
   return v;
 }
 
@@ -213,7 +222,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->counters.ensure(n1 * 4));
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
-  BVH_TRY(o->status.ensure(2 * sizeof(uint32_t)));  // [error bits, last-block counter]
+  BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, last-block counter, max slab error, -]
   BVH_TRY(o->rightc.ensure(n1 * 4));
   BVH_TRY(o->ropes.ensure((2 * n1 - 1) * 4));
   BVH_TRY(o->ilast.ensure(n1 * 4));
@@ -221,7 +230,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   if (o->fp32) {
     BVH_TRY(o->nodes32.ensure((2 * n1 - 1) * sizeof(Node32)));
-    BVH_TRY(o->origin.ensure(3 * sizeof(double)));
+    BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   } else {
     BVH_TRY(o->nodes.ensure((2 * n1 - 1) * sizeof(Node64)));
   }
@@ -252,13 +261,14 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.root = o->root.as<uint32_t>();
   b.nodes = o->fp32 ? nullptr : o->nodes.as<Node64>();
   b.nodes32 = o->fp32 ? o->nodes32.as<Node32>() : nullptr;
-  b.origin = o->fp32 ? o->origin.as<double>() : nullptr;
+  b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
   b.rightc = o->rightc.as<uint32_t>();
   b.ilast = o->ilast.as<uint32_t>();
   b.front_a = o->front_a.as<uint32_t>();
   b.front_b = o->front_b.as<uint32_t>();
   b.ropes = o->ropes.as<uint32_t>();
+  b.patch_k64 = o->fp32 ? o->patch_k64.as<unsigned long long>() : nullptr;
   return BVH_OK;
 }
 
@@ -268,7 +278,7 @@ static bvh_status object_build(bvh_object_s* o) {
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n;
   BVH_TRY(object_alloc(o));
-  HIP_TRY(launch_fill_u32(o->bb.status, 2, 0u, s));  // error bits and the bounds counter
+  HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, bounds counter, slab error
   if (n == 0) {
     // splits are all 0 (split/cluster.hpp:101-105), every patch empty
     HIP_TRY(launch_fill_u32(o->bb.splits, w->od > 1 ? w->od - 1 : 0, 0u, s));
@@ -303,8 +313,9 @@ static bvh_status object_build(bvh_object_s* o) {
   HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
   w->stage_end();
   if (o->fp32) {
-    // object origin for the fp32 slabs: the Morton bounds minimum (fp64)
-    HIP_TRY(hipMemcpyAsync(o->bb.origin, o->bb.bounds, 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // the world's fp32 frame: the Morton bounds minimum of its first non-empty build
+    HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
+    HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   }
   w->stage_begin("delta");
   HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
@@ -316,6 +327,25 @@ static bvh_status object_build(bvh_object_s* o) {
   HIP_TRY(launch_ropes(o->bb, n, o->fp32, s));
   w->stage_end();
   o->built = true;
+  return BVH_OK;
+}
+
+// Refit: new element k-DOPs on the topology of the last build (same
+// permutation, patches and split heights); counters and roots are cleared and
+// the bottom-up pass runs again.  Ropes depend on the topology only.
+static bvh_status object_refit(bvh_object_s* o) {
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  const uint64_t n = o->in.n;
+  HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));
+  if (n == 0) return BVH_OK;
+  w->stage_begin("refit");
+  if (n > 1) HIP_TRY(launch_fill_u32(o->bb.counters, n - 1, 0u, s));
+  HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
+  HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
+  if (o->fp32) HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
+  HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
+  w->stage_end();
   return BVH_OK;
 }
 
@@ -375,12 +405,17 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
 static bvh_status read_count(bvh_object_s* o, bvh_object_s* other, unsigned long long* dcount, uint64_t* out) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
-  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[2], o->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[3], other->status.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[4], o->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[6], other->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   o->nvisits = w->h_count[1];
-  const uint32_t st = (uint32_t)w->h_count[2] | (uint32_t)w->h_count[3];
+  o->ncandidates = w->h_count[2];
+  o->nrecheck = w->h_count[3];
+  uint32_t sw[8];
+  std::memcpy(sw, &w->h_count[4], sizeof(sw));
+  o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
+  const uint32_t st = sw[0] | sw[4];
   if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
   *out = w->h_count[0];
   return BVH_OK;
@@ -418,9 +453,7 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
-  if (o->fp32 || other->fp32)
-    return fail(BVH_E_UNSUPPORTED, "fp32-outward queries are not built in this library version");
-  BVH_TRY(o->count.ensure(2 * sizeof(unsigned long long)));
+  BVH_TRY(o->count.ensure(4 * sizeof(unsigned long long)));
   unsigned long long* dcount = o->count.as<unsigned long long>();
   const TreeView qv = view_of(o), tv = view_of(other);
   // patch broadphase
@@ -433,9 +466,9 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
         BVH_TRY(o->ppairs.ensure(pcap * 8));
         cap = pcap;
       }
-      HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2, 0u, s));
-      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount, nullptr};
-      HIP_TRY(launch_patch_broadphase(qv, tv, po, s));
+      HIP_TRY(launch_fill_u32((uint32_t*)dcount, 8, 0u, s));
+      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount, nullptr, nullptr, nullptr, 0, nullptr};
+      HIP_TRY(launch_patch_broadphase(qv, tv, o->fp32, po, s));
       BVH_TRY(read_count(o, other, dcount, &o->npatch_pairs));
       if (o->npatch_pairs <= cap) break;
       BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024));
@@ -457,15 +490,19 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   w->stage_begin("narrowphase");
   uint64_t want = std::max<uint64_t>(1u << 20, 8 * o->in.n);
   if (o->pairs.bytes / 8 < want) BVH_TRY(ensure_pairs(o, want));
+  if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n / 8) * 8));
   for (int attempt = 0; attempt < 2; ++attempt) {
     const uint64_t cap = o->pairs.bytes / 8;
-    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 4, 0u, s));
-    PairOut po{o->pairs.as<uint64_t>(), cap, dcount, dcount + 1};
-    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, 0, po, s));
+    const uint64_t rcap = o->recheck.bytes / 8;
+    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 8, 0u, s));
+    PairOut po{o->pairs.as<uint64_t>(), cap, dcount, dcount + 1, dcount + 2,
+               o->recheck.as<uint64_t>(), rcap, dcount + 3};
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, po, s));
     BVH_TRY(read_count(o, other, dcount, &o->npairs));
-    if (o->npairs <= cap) break;
-    // overflow: grow to the exact count and run again
-    BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+    if (o->npairs <= cap && (!o->fp32 || o->nrecheck <= rcap)) break;
+    // overflow: grow to the exact counts and run again
+    if (o->npairs > cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+    if (o->fp32 && o->nrecheck > rcap) BVH_TRY(o->recheck.ensure((o->nrecheck + (o->nrecheck >> 3) + 1024) * 8));
   }
   w->stage_end();
   o->result_ptr = o->pairs.as<uint64_t>();
@@ -563,8 +600,12 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
     HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     w->own_stream = true;
   }
+  if (c.precision == BVH_PREC_FP32_OUTWARD) {
+    BVH_TRY(w->frame.ensure(4 * sizeof(double)));
+    HIP_TRY(launch_fill_u32(w->frame.as<uint32_t>(), 8, 0u, w->stream));
+  }
   void* hc = nullptr;
-  HIP_TRY(hipHostMalloc(&hc, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&hc, 8 * sizeof(unsigned long long), hipHostMallocDefault));
   w->h_count = (unsigned long long*)hc;
   *out = w.release();
   return BVH_OK;
@@ -670,6 +711,27 @@ bvh_status bvh_set_entity_data_snapshot(bvh_object o, const double* kdops, const
   return set_entity_common(o, kSnapshot, 1, 0.0, kdops, centroids, n, gids, mem, split);
 }
 
+bvh_status bvh_refit(bvh_object o, const double* data, uint32_t mem) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  if (!o->built || !o->have_data) return fail(BVH_E_STATE, "refit before set_entity_data");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  const uint64_t n = o->in.n;
+  if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
+  o->have_results = false;
+  if (n > 0) {
+    const size_t planes = o->in.kind == kPoly ? (size_t)o->in.nv * 3 : (o->in.kind == kSphere ? 4 : kSlabs);
+    const size_t bytes = planes * n * sizeof(double);
+    if (mem == BVH_MEM_HOST) {
+      BVH_TRY(o->input.ensure(bytes));
+      HIP_TRY(hipMemcpyAsync(o->input.p, data, bytes, hipMemcpyHostToDevice, o->w->stream));
+      o->in.data = o->input.as<double>();
+    } else {
+      o->in.data = data;
+    }
+  }
+  return object_refit(o);
+}
+
 bvh_status bvh_init_broadphase(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   if (!o->built) return fail(BVH_E_STATE, "init_broadphase before set_entity_data");
@@ -755,10 +817,18 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_
   return copy_out(out, o->records.p, n * sizeof(bvh_narrowphase_record), mem, s);
 }
 
-bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs) {
+bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs, uint64_t* candidates,
+                            uint64_t* rechecked, double* max_slab_error_ulps) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   if (node_visits) *node_visits = o->have_results ? o->nvisits : 0;
   if (patch_pairs) *patch_pairs = o->have_results ? o->npatch_pairs : 0;
+  if (candidates) *candidates = o->have_results ? (o->fp32 ? o->ncandidates : o->npairs) : 0;
+  if (rechecked) *rechecked = o->have_results && o->fp32 ? o->nrecheck : 0;
+  if (max_slab_error_ulps) {
+    float f;
+    std::memcpy(&f, &o->slab_err_bits, sizeof(f));
+    *max_slab_error_ulps = o->have_results ? (double)f : 0.0;
+  }
   return BVH_OK;
 }
 
@@ -826,7 +896,7 @@ bvh_status bvh_object_splits(bvh_object o, uint64_t* splits, size_t cap) {
 
 bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
-  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "fp64 slabs are not stored in the fp32 form");
+  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "the fp32 form stores fp32 slabs only");
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
@@ -868,7 +938,11 @@ bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
       pi.kdop[2 * a] = 1.7976931348623157e308;
       pi.kdop[2 * a + 1] = -1.7976931348623157e308;
     }
-    if (roots[k] != kEnd && !o->fp32) {
+    if (roots[k] != kEnd && o->fp32) {
+      uint64_t enc[kSlabs];
+      BVH_TRY(copy_out(enc, o->patch_k64.as<unsigned long long>() + k * kSlabs, sizeof(enc), BVH_MEM_HOST, s));
+      for (int i = 0; i < 26; ++i) pi.kdop[i] = from_ordered_u64(enc[i]);
+    } else if (roots[k] != kEnd) {
       Node64 nd;
       BVH_TRY(copy_out(&nd, o->nodes.as<Node64>() + roots[k], sizeof(Node64), BVH_MEM_HOST, s));
       for (int i = 0; i < 26; ++i) pi.kdop[i] = nd.s[i];

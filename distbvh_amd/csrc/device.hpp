@@ -186,6 +186,137 @@ struct alignas(128) Node32 {
 };
 static_assert(sizeof(Node32) == 128, "node32 layout");
 
+
+// Order-preserving map of a double to an unsigned 64-bit integer (x < y ==>
+// ordered_u64(x) < ordered_u64(y) for non-NaN values), so integer atomic
+// min / max act as double min / max.
+__host__ __device__ __forceinline__ uint64_t ordered_u64(double x) {
+  uint64_t u;
+  __builtin_memcpy(&u, &x, 8);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double from_ordered_u64(uint64_t u) {
+  u = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+  double x;
+  __builtin_memcpy(&x, &u, 8);
+  return x;
+}
+
+// ---- element data ----------------------------------------------------------------
+enum ElemKind : int { kPoly = 0, kSphere = 1, kSnapshot = 2 };
+
+struct ElemInput {
+  int kind;
+  int nv;                   // vertices per element (kPoly)
+  double eps;               // from_vertices epsilon (kPoly)
+  const double* data;       // planes (kPoly / kSphere) or n x 26 kdops (kSnapshot)
+  const double* centroids;  // n x 3 (kSnapshot)
+  uint64_t n;
+};
+
+// next representable float below / above a finite f (IEEE-754 bit stepping)
+__device__ __forceinline__ float next_down(float f) {
+  if (!(f == f) || f == -__builtin_inff()) return f;
+  if (f == 0.0f) return __uint_as_float(0x80000001u);
+  const uint32_t u = __float_as_uint(f);
+  return __uint_as_float(f > 0.0f ? u - 1u : u + 1u);
+}
+__device__ __forceinline__ float next_up(float f) {
+  if (!(f == f) || f == __builtin_inff()) return f;
+  if (f == 0.0f) return __uint_as_float(0x00000001u);
+  const uint32_t u = __float_as_uint(f);
+  return __uint_as_float(f > 0.0f ? u + 1u : u - 1u);
+}
+__device__ __forceinline__ float round_down_1ulp(double x) { return next_down((float)x); }
+__device__ __forceinline__ float round_up_1ulp(double x) { return next_up((float)x); }
+
+template <int KIND, int NV>
+__device__ __forceinline__ void element_kdop(const ElemInput& in, uint32_t e, double* kd) {
+  const uint64_t n = in.n;
+  if constexpr (KIND == kPoly) {
+    double v[NV][3];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[j][c] = in.data[(uint64_t)(3 * j + c) * n + e];
+    kdop_from_vertices<NV>(v, in.eps, kd);
+  } else if constexpr (KIND == kSphere) {
+    kdop_from_sphere(in.data[e], in.data[n + e], in.data[2 * n + e], in.data[3 * n + e], kd);
+  } else {
+    const double2* k = (const double2*)(in.data + (uint64_t)e * kSlabs);
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      double2 t = k[a];
+      kd[2 * a] = t.x;
+      kd[2 * a + 1] = t.y;
+    }
+  }
+}
+
+// element_kdop with the kind and vertex count chosen at run time (a
+// wave-uniform switch)
+__device__ __forceinline__ void element_kdop_any(const ElemInput& in, uint32_t e, double* kd) {
+  if (in.kind == kSphere) {
+    element_kdop<kSphere, 1>(in, e, kd);
+  } else if (in.kind == kSnapshot) {
+    element_kdop<kSnapshot, 1>(in, e, kd);
+  } else {
+    switch (in.nv) {
+      case 1: element_kdop<kPoly, 1>(in, e, kd); break;
+      case 2: element_kdop<kPoly, 2>(in, e, kd); break;
+      case 3: element_kdop<kPoly, 3>(in, e, kd); break;
+      case 4: element_kdop<kPoly, 4>(in, e, kd); break;
+      case 5: element_kdop<kPoly, 5>(in, e, kd); break;
+      case 6: element_kdop<kPoly, 6>(in, e, kd); break;
+      case 7: element_kdop<kPoly, 7>(in, e, kd); break;
+      default: element_kdop<kPoly, 8>(in, e, kd); break;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_slabs(Node64* nd, const double* kd) {
+  double2* d = (double2*)nd->s;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) d[a] = make_double2(kd[2 * a], kd[2 * a + 1]);
+}
+__device__ __forceinline__ void load_slabs(const Node64* nd, double* kd) {
+  const double2* d = (const double2*)nd->s;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    double2 t = d[a];
+    kd[2 * a] = t.x;
+    kd[2 * a + 1] = t.y;
+  }
+}
+
+// fp32 outward slabs relative to the object origin: the fp64 slab minus the
+// origin's projection (fp64), rounded to nearest fp32 and moved one ulp out.
+__device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    kf[2 * a] = round_down_1ulp(kd[2 * a] - po[a]);
+    kf[2 * a + 1] = round_up_1ulp(kd[2 * a + 1] - po[a]);
+  }
+}
+
+__device__ __forceinline__ void store_slabs32(Node32* nd, const float* kf) {
+  float4* d = (float4*)nd->s;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) d[q] = make_float4(kf[4 * q], kf[4 * q + 1], kf[4 * q + 2], kf[4 * q + 3]);
+  nd->s[24] = kf[24];
+  nd->s[25] = kf[25];
+}
+__device__ __forceinline__ void load_slabs32(const Node32* nd, float* kf) {
+  const float4* d = (const float4*)nd->s;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    float4 t = d[q];
+    kf[4 * q] = t.x; kf[4 * q + 1] = t.y; kf[4 * q + 2] = t.z; kf[4 * q + 3] = t.w;
+  }
+  kf[24] = nd->s[24];
+  kf[25] = nd->s[25];
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 

@@ -45,17 +45,6 @@ hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* h
 hipError_t launch_scan_u32(uint32_t* v, uint64_t n, uint64_t* lb, uint32_t* counter, hipStream_t s);
 
 // ---- object build ---------------------------------------------------------------
-enum ElemKind : int { kPoly = 0, kSphere = 1, kSnapshot = 2 };
-
-struct ElemInput {
-  int kind;
-  int nv;                   // vertices per element (kPoly)
-  double eps;               // from_vertices epsilon (kPoly)
-  const double* data;       // planes (kPoly / kSphere) or n x 26 kdops (kSnapshot)
-  const double* centroids;  // n x 3 (kSnapshot)
-  uint64_t n;
-};
-
 struct BuildBufs {
   // original order
   double* centroid;        // 3 planes of n (kPoly)
@@ -87,13 +76,14 @@ struct BuildBufs {
   uint32_t* root;          // 1
   Node64* nodes;           // 2n-1
   Node32* nodes32;         // 2n-1 (fp32 form) or null
-  double* origin;          // 3 (fp32 form)
+  double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
   uint32_t* rightc;        // n-1: right child of each internal node
   uint32_t* ilast;         // n-1: last covered position of each internal node
   uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
   uint32_t* front_b;
   uint32_t* ropes;         // 2n-1: escape link of every node
+  unsigned long long* patch_k64;  // fp32 form: od x 26 exact patch slabs (ordered_u64 encoding)
 };
 
 // device error bits: set instead of faulting when an invariant breaks
@@ -108,6 +98,11 @@ hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsp
 hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s);
 // k_tree (leaf slabs in sorted order + bottom-up build with refit)
 hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s);
+// fp32 form: set the world frame (3 doubles, then a flag word) from the
+// bounds minimum unless already set; non-finite components become 0
+hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s);
+// fp32 form: patch_k64 set to the empty k-DOP before the leaf kernel
+hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hipStream_t s);
 // k_ropes (escape links)
 hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
@@ -124,13 +119,21 @@ struct TreeView {
   uint64_t od;
   uint32_t* status;
   const uint32_t* ropes;
+  const double* origin;  // fp32 form: frame origin (3 doubles)
+  ElemInput elem;        // the object's element data (fp32 form re-checks in fp64)
+  const unsigned long long* patch_k64;  // fp32 form: exact patch slabs (ordered_u64)
 };
 
 struct PairOut {
   uint64_t* pairs;      // (input index a << 32) | input index b
   uint64_t cap;
   unsigned long long* count;
-  unsigned long long* visits;  // optional: node visits summed over waves
+  unsigned long long* visits;      // optional: node visits summed over waves
+  unsigned long long* candidates;  // optional: fp32 candidate pairs before the fp64 check
+  // fp32 form: candidates too close to call in fp32, re-checked in fp64
+  uint64_t* recheck;
+  uint64_t rcap;
+  unsigned long long* rcount;
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)
@@ -138,7 +141,7 @@ struct PairOut {
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self,
                               int fp32, const PairOut& out, hipStream_t s);
 // patch broadphase: patch roots of `q` against the patch level of `t`
-hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, const PairOut& out,
+hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
                                    hipStream_t s);
 // records {patch_p, element_p, patch_q, element_q}
 hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t* patch_a,

@@ -9,9 +9,9 @@
 //    Replaces query_tree / query_node_impl (collision_query.hpp:166-213) as
 //    used by a narrowphase functor per patch pair, and self_collision_set
 //    (collision_query.hpp:331-368) in self mode (each unordered pair once).
-//  * k_patch_broadphase: one lane per patch root of the querying object; the
-//    walk stops at the first single-patch node, which is the other object's
-//    patch root (start_broadphase, collision_object/broadphase.cpp:68-119).
+//  * k_patch_pairs: one lane per pair of patches, testing the two patch
+//    k-DOPs (start_broadphase, collision_object/broadphase.cpp:68-119, which
+//    walks the other object's patch-level tree to the same set).
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -169,46 +169,189 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
   if (out.visits && lane_id() == 0) atomicAdd(out.visits, (unsigned long long)steps);
 }
 
-// Patch broadphase over all patch pairs: a 256 x 256 block of (patch of q,
-// patch of t); the t patch slabs are staged in LDS and read as broadcasts.
-// Empty patches never overlap (broadphase.cpp:79-80).
-__global__ __launch_bounds__(256) void k_patch_pairs(TreeView q, TreeView t, PairOut out) {
-  __shared__ double s_k[256][kSlabs + 1];
-  __shared__ uint32_t s_ok[256];
-  const uint32_t tid = threadIdx.x;
-  const uint64_t qp = (uint64_t)blockIdx.x * 256 + tid;
-  const uint64_t tp = (uint64_t)blockIdx.y * 256 + tid;
-  // stage the t patches of this block
-  s_ok[tid] = 0;
-  if (tp < t.od && t.n > 0) {
-    const uint32_t r = t.patch_root[tp];
-    if (r != kEnd && r < 2 * t.n - 1) {
-      double k[kSlabs];
-      load_node_slabs(&t.nodes[r], k);
-      for (int i = 0; i < kSlabs; ++i) s_k[tid][i] = k[i];
-      s_ok[tid] = 1;
-    }
-  }
-  double qk[kSlabs];
-  bool valid = false;
-  if (qp < q.od && q.n > 0) {
-    const uint32_t r = q.patch_root[qp];
-    if (r != kEnd && r < 2 * q.n - 1) {
-      load_node_slabs(&q.nodes[r], qk);
-      valid = true;
-    }
-  }
-  if (!valid) empty_slabs(qk);
-  __syncthreads();
-  const uint32_t nt = (uint32_t)((t.od - (uint64_t)blockIdx.y * 256) < 256 ? (t.od - (uint64_t)blockIdx.y * 256) : 256);
-  for (uint32_t j = 0; j < nt; ++j) {
-    bool h = valid && s_ok[j];
-    if (h) {
+// fp32-outward form.  Node slabs are fp32 in the frame of the world's origin P
+// (one frame for every object of a world), rounded outward, so an fp32 miss
+// proves an fp64 miss: with q, t the exact slabs and f(.) the rounded ones,
+// q_lo < t_hi implies f(q_lo) <= q_lo - P < t_hi - P <= f(t_hi).  The query's
+// slabs are its own fp32 leaf slabs.
+//
+// A leaf that passes the fp32 test is a candidate.  The rounding is also
+// bounded from the inside: a leaf slab f = next_down(rn(fl64(x - P))) has
+// x - P < next_up(next_up(f)) (rn moves by at most half an ulp and fl64 by far
+// less), and likewise for upper slabs.  In the ordered-integer image ord() of
+// the floats, where next_up adds one, ord(f(t_hi)) - ord(f(q_lo)) > 6 on a
+// slab pair therefore proves q_lo < t_hi in fp64.  Candidates proven on every
+// slab are emitted; the rest go to a re-check list whose pairs are tested on
+// fp64 k-DOPs recomputed from element data (k_recheck).  The pairs are
+// exactly those of the fp64 form.
+typedef __attribute__((address_space(4))) const Node32 CNode32;
+
+// ordered-integer image of a float: monotone, consecutive floats differ by 1,
+// ord(-0) = ord(+0) = 0
+__device__ __forceinline__ int32_t ord_f32(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u >> 31) ? -(int32_t)(u & 0x7fffffffu) : (int32_t)u;
+}
+constexpr int32_t kInnerUlps = 6;
+
+__device__ __forceinline__ bool overlap_cnode32(const float* q, CNode32* nd) {
+  bool o = true;
 #pragma unroll
-      for (int a = 0; a < kAxes; ++a) h = h & (qk[2 * a] < s_k[j][2 * a + 1]) & (s_k[j][2 * a] < qk[2 * a + 1]);
+  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < nd->s[2 * i + 1]) & (nd->s[2 * i] < q[2 * i + 1]);
+  return o;
+}
+
+template <bool SELF>
+__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int leaf_size, PairOut out) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  uint64_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t fill = 0;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
+  const bool valid = p < q.n;
+  float qf[kSlabs];
+  int32_t qin[kSlabs];  // ord(qf_lo) + kInnerUlps, ord(qf_hi) - kInnerUlps
+  uint32_t qa = 0;
+  if (valid) {
+    // q and t share the world's fp32 frame: the query slabs are q's leaf slabs
+    const Node32* qn = &q.nodes32[q.n - 1 + p];
+    load_slabs32(qn, qf);
+    qa = qn->left;
+  } else {
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      qf[2 * a] = __builtin_inff();
+      qf[2 * a + 1] = -__builtin_inff();
     }
-    emit(out, h, (qp << 32) | ((uint64_t)blockIdx.y * 256 + j));
   }
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    // (wrapping arithmetic: only NaN images come near the int32 range ends,
+    // and NaN slabs fail the fp32 test before this bound is used)
+    qin[2 * a] = (int32_t)((uint32_t)ord_f32(qf[2 * a]) + (uint32_t)kInnerUlps);
+    qin[2 * a + 1] = (int32_t)((uint32_t)ord_f32(qf[2 * a + 1]) - (uint32_t)kInnerUlps);
+  }
+  const uint32_t tleaf0 = (uint32_t)(t.n - 1);
+  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
+  CNode32* nodes = (CNode32*)t.nodes32;
+  const __attribute__((address_space(4))) uint32_t* ropes = (const __attribute__((address_space(4))) uint32_t*)t.ropes;
+  uint32_t node = uniform(*t.root);
+  uint32_t steps = 0;
+  uint32_t ncand = 0;
+  // fp32 candidates at one (wave-uniform) leaf: emit what the inner bound
+  // proves, queue the rest for the fp64 re-check
+  auto leaf_check = [&](CNode32* ln, bool h) {
+    const uint64_t m = __ballot(h);
+    if (m == 0) return;
+    ncand += (uint32_t)__popcll(m);
+    bool sure = true;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a)
+      sure = sure && (qin[2 * a] < ord_f32(ln->s[2 * a + 1])) && (ord_f32(ln->s[2 * a]) < qin[2 * a + 1]);
+    const uint64_t key = pair_key(qa, ln->left, SELF);
+    wave_emit(buf, fill, out, h && sure, key);
+    const bool unsure = h && !sure;
+    const uint64_t mu = __ballot(unsure);
+    if (mu != 0) {
+      unsigned long long base = 0;
+      if (lane_id() == 0) base = atomicAdd(out.rcount, (unsigned long long)__popcll(mu));
+      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+      base = ((unsigned long long)hi << 32) | lo;
+      if (unsure) {
+        const unsigned long long idx = base + (unsigned long long)__popcll(mu & lanemask_lt());
+        if (idx < out.rcap) out.recheck[idx] = key;
+      }
+    }
+  };
+  while (node != kEnd) {
+    if (node >= nnodes || ++steps > nnodes) {
+      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    CNode32* nd = nodes + node;
+    const uint32_t first = nd->first, last = nd->last, rope = ropes[node];
+    bool hit = overlap_cnode32(qf, nd);
+    if (SELF) hit = hit && last > (uint32_t)p;
+    if (__ballot(hit) == 0) {
+      node = rope;
+      continue;
+    }
+    if (node >= tleaf0) {
+      leaf_check(nd, hit);
+      node = rope;
+    } else if (last - first < (uint32_t)leaf_size) {
+      for (uint32_t pos = first; pos <= last; ++pos) {
+        CNode32* ln = nodes + (tleaf0 + pos);
+        bool h = hit && overlap_cnode32(qf, ln);
+        if (SELF) h = h && pos > (uint32_t)p;
+        leaf_check(ln, h);
+      }
+      node = rope;
+    } else {
+      node = nd->left;
+    }
+  }
+  wave_flush(buf, fill, out);
+  if (lane_id() == 0) {
+    if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
+    if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
+  }
+}
+
+// fp64 re-check of the fp32 candidates that the inner bound left open: both
+// elements' k-DOPs recomputed from element data, the pair emitted on overlap.
+// The list length is read on the device, so no host round trip separates the
+// two kernels.
+__global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut out) {
+  const unsigned long long cnt0 = *out.rcount;
+  const uint64_t cnt = cnt0 < out.rcap ? cnt0 : out.rcap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < cnt; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    bool hit = false;
+    uint64_t key = 0;
+    if (i < cnt) {
+      key = out.recheck[i];
+      double ka[kSlabs], kb[kSlabs];
+      element_kdop_any(q.elem, (uint32_t)(key >> 32), ka);
+      element_kdop_any(t.elem, (uint32_t)key, kb);
+      hit = overlap(ka, kb);
+    }
+    emit(out, hit, key);
+  }
+}
+
+// exact patch slabs: the patch root node (fp64 form) or the fp64 patch slabs
+// kept beside an fp32 tree
+__device__ __forceinline__ void patch_slabs(const TreeView& v, uint64_t k, uint32_t r, double* out) {
+  if (v.patch_k64) {
+    const unsigned long long* e = v.patch_k64 + k * kSlabs;
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) out[i] = from_ordered_u64(e[i]);
+  } else {
+    load_node_slabs(&v.nodes[r], out);
+  }
+}
+
+// Patch broadphase over all patch pairs: one lane per (patch of q, patch of
+// t), t fastest, so a wave shares its q patch (a broadcast load) and reads 64
+// consecutive t patches; both objects' patch slabs stay in L2.  Empty patches
+// never overlap (broadphase.cpp:79-80).
+__global__ __launch_bounds__(256) void k_patch_pairs(TreeView q, TreeView t, PairOut out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t qp = k / t.od, tp = k - qp * t.od;
+  bool hit = false;
+  if (qp < q.od) {
+    const uint32_t rq = q.patch_root[qp], rt = t.patch_root[tp];
+    if (rq != kEnd && rq < 2 * q.n - 1 && rt != kEnd && rt < 2 * t.n - 1) {
+      double qk[kSlabs], tk[kSlabs];
+      patch_slabs(q, qp, rq, qk);
+      patch_slabs(t, tp, rt, tk);
+      hit = overlap(qk, tk);
+    }
+  }
+  emit(out, hit, (qp << 32) | tp);
 }
 
 __global__ __launch_bounds__(256) void k_records(const uint64_t* __restrict__ pairs, uint64_t count,
@@ -257,18 +400,28 @@ __global__ __launch_bounds__(64) void k_patch_centroids(const double* __restrict
 // ---- launchers ---------------------------------------------------------------------
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
                               const PairOut& out, hipStream_t s) {
-  (void)fp32;
   if (q.n == 0 || t.n == 0) return hipSuccess;
   const unsigned blocks = (unsigned)((q.n + 255) / 256);
+  if (fp32) {
+    if (!q.nodes32 || !t.nodes32 || !t.origin || q.origin != t.origin) return hipErrorInvalidValue;
+    if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
+    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+    // in self mode both keys index the one object (q == t)
+    hipLaunchKernelGGL(k_recheck, dim3(512), dim3(256), 0, s, q, t, out);
+    return hipGetLastError();
+  }
   if (self) hipLaunchKernelGGL(k_narrow<true>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
   else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
   return hipGetLastError();
 }
 
-hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s) {
+hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
+                                   hipStream_t s) {
   if (q.od == 0 || t.od == 0 || t.n == 0 || q.n == 0) return hipSuccess;
-  const dim3 grid((unsigned)((q.od + 255) / 256), (unsigned)((t.od + 255) / 256));
-  hipLaunchKernelGGL(k_patch_pairs, grid, dim3(256), 0, s, q, t, out);
+  if (fp32 && (!q.patch_k64 || !t.patch_k64)) return hipErrorInvalidValue;
+  const uint64_t total = q.od * t.od;
+  hipLaunchKernelGGL(k_patch_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, q, t, out);
   return hipGetLastError();
 }
 

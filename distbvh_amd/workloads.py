@@ -131,3 +131,27 @@ def hex_grid(nx, ny, nz, origin_shift=0.0):
         out[v, 1] = cy
         out[v, 2] = cz
     return out
+
+
+def touching_spheres(n=4096, radius=1e-3, shift=(1e3, -1e3, 1e3), max_ulps=4, seed=0xC4):
+    """Two sphere sets whose i-th spheres are within a few fp64 ulps of
+    touching along x: B_i's centre is fl(A_i.x + 2r) stepped k ulps, k in
+    [-max_ulps, max_ulps].  The pairs with a negative gap overlap strictly,
+    the others do not; every pair is a case an fp32 test cannot decide.
+    Sphere i sits on a lattice 4r apart in y and z, far from the origin.
+    Returns (a, b) planes (4, n)."""
+    side = int(np.ceil(np.sqrt(n)))
+    i = np.arange(n)
+    a = np.empty((4, n), np.float64)
+    a[0] = shift[0] + 0.37 * radius * (uniform(seed, n) - 0.5)
+    a[1] = shift[1] + (i % side) * (8.0 * radius)
+    a[2] = shift[2] + (i // side) * (8.0 * radius)
+    a[3] = radius
+    b = a.copy()
+    k = (i % (2 * max_ulps + 1)) - max_ulps
+    x = a[0] + 2.0 * radius
+    for s in range(max_ulps):
+        x = np.where(k > s, np.nextafter(x, np.inf), x)
+        x = np.where(k < -s, np.nextafter(x, -np.inf), x)
+    b[0] = x
+    return a, b

@@ -141,6 +141,13 @@ bvh_status bvh_set_entity_data_sphere(bvh_object o, const double *xyzr, uint64_t
 bvh_status bvh_set_entity_data_snapshot(bvh_object o, const double *kdops, const double *centroids,
                                         uint64_t n, const uint64_t *global_ids, uint32_t mem,
                                         uint32_t split_algorithm);
+/* Refit (no reference counterpart: the reference rebuilds on every
+ * set_entity_data).  New element data of the same kind and count as the last
+ * set_entity_data (vertex planes, sphere planes, or n x 26 k-DOPs) on the
+ * topology of the last build: the permutation, patches and tree shape are
+ * kept and every node's k-DOP is recomputed bottom-up, so queries stay exact.
+ * Morton codes, bounds and patch centroids stay those of the last build. */
+bvh_status bvh_refit(bvh_object o, const double *data, uint32_t mem);
 /* init_broadphase(), collision_object.hpp:87 / collision_object.cpp:148-215 */
 bvh_status bvh_init_broadphase(bvh_object o);
 /* broadphase(collision_object &other), collision_object.hpp:170 /
@@ -167,8 +174,14 @@ bvh_status bvh_result_pairs(bvh_object o, uint64_t *out, uint64_t cap, uint32_t 
 /* device pointer to the compact pairs (valid until the next broadphase) */
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_t *count);
 /* counters of the last query: tree nodes visited (summed over waves of 64
- * queries) and broadphase patch pairs */
-bvh_status bvh_result_stats(bvh_object o, uint64_t *node_visits, uint64_t *patch_pairs);
+ * queries), broadphase patch pairs, candidate pairs (fp32 form: pairs whose
+ * outward-rounded fp32 slabs overlap; fp64 form: the pair count), rechecked
+ * pairs (fp32 form: candidates too close to decide in fp32, tested again on
+ * fp64 k-DOPs; 0 in the fp64 form) and the largest fp32 slab error in ulps of
+ * the objects' last builds (0 in the fp64 form).  Any pointer may be NULL. */
+bvh_status bvh_result_stats(bvh_object o, uint64_t *node_visits, uint64_t *patch_pairs,
+                            uint64_t *candidate_pairs, uint64_t *rechecked_pairs,
+                            double *max_slab_error_ulps);
 /* broadphase patch pairs (query_tree emits, broadphase.cpp:93-118):
  * out[2k] = patch of o, out[2k+1] = patch of other. */
 bvh_status bvh_patch_pair_count(bvh_object o, uint64_t *count);

@@ -232,3 +232,90 @@ def test_full_size_wavy_sheets(torch_cuda):
     # every narrowphase record lies in a broadphase patch pair
     pset = set(map(tuple, pp.tolist()))
     assert all((p, q) in pset for p, q in set(map(tuple, recs[:, [0, 2]].tolist())))
+
+
+def _check_tree(tree, skd, n, od, splits):
+    """Structural invariants of the radix tree: internal node k splits
+    positions k | k+1, a node's range is the union of its children's, its
+    slabs are the exact (min, max) union of theirs, leaf p holds the k-DOP of
+    sorted element p, every node is reachable from the root once, and every
+    non-empty patch is one subtree."""
+    kd, ch, rg, root = tree["kdop"], tree["children"], tree["range"], tree["root"]
+    assert len(kd) == 2 * n - 1
+    leaf0 = n - 1
+    assert np.array_equal(kd[leaf0:], skd)
+    assert np.array_equal(rg[leaf0:, 0], np.arange(n)) and np.array_equal(rg[leaf0:, 1], np.arange(n))
+    assert np.all(ch[leaf0:] == -1)
+    if n == 1:
+        assert root == 0
+        return
+    assert tuple(rg[root]) == (0, n - 1)
+    k = np.arange(n - 1)
+    left, right = ch[:n - 1, 0], ch[:n - 1, 1]
+    assert np.all((left >= 0) & (left < 2 * n - 1) & (right >= 0) & (right < 2 * n - 1))
+    assert np.array_equal(rg[left, 1], k.astype(np.uint64))
+    assert np.array_equal(rg[right, 0], (k + 1).astype(np.uint64))
+    assert np.array_equal(rg[:n - 1, 0], rg[left, 0]) and np.array_equal(rg[:n - 1, 1], rg[right, 1])
+    lo = np.minimum(kd[left][:, 0::2], kd[right][:, 0::2])
+    hi = np.maximum(kd[left][:, 1::2], kd[right][:, 1::2])
+    assert np.array_equal(kd[:n - 1, 0::2], lo) and np.array_equal(kd[:n - 1, 1::2], hi)
+    # reachability: every node has exactly one parent, except the root
+    parents = np.zeros(2 * n - 1, np.int64)
+    np.add.at(parents, left, 1)
+    np.add.at(parents, right, 1)
+    assert parents[root] == 0 and np.all(np.delete(parents, root) == 1)
+    # patches are subtrees: a node spans exactly [begin, end) of each non-empty patch
+    b = np.concatenate([[0], splits]).astype(np.uint64)
+    e = np.concatenate([splits, [n]]).astype(np.uint64)
+    spans = set(map(tuple, rg.tolist()))
+    for p in range(od):
+        if e[p] > b[p]:
+            assert (int(b[p]), int(e[p]) - 1) in spans, p
+
+
+@pytest.mark.parametrize("case", ["wavy64_od16", "wavy200_od64", "spheres_od8", "wavy1024_od256", "flat_od1"])
+def test_tree_structure(torch_cuda, case):
+    if case.startswith("wavy"):
+        nq = int(case[4:].split("_")[0])
+        od = int(case.split("od")[1])
+        data, _ = W.wavy_sheets(nq)
+        kind = "poly"
+    elif case == "spheres_od8":
+        data = W.sphere_cloud(70001, radius=5e-3, ncomp=8, sigma=0.05, shift=(0, 0, 0), seed=31)
+        kind, od = "sphere", 8
+    else:
+        data, _ = W.flat_sheets(64)
+        kind, od = "poly", 1
+    with CollisionWorld(od, morton_bits=63, leaf_size=4) as w:
+        obj = gpu_object(w, kind, data, device=True)
+        tree = obj.tree()
+        skd = obj.sorted_kdops()
+        splits = obj.splits()
+        n = skd.shape[0]
+    _check_tree(tree, skd, n, od, splits)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_refit_pairs(torch_cuda, precision):
+    """bvh_refit keeps the topology and recomputes every k-DOP: after moving
+    sheet B the pair set is brute force on the moved sheet, and refitting back
+    gives the original set again."""
+    a, b0 = W.wavy_sheets(64)
+    rng = np.random.default_rng(7)
+    b1 = b0 + np.array([0.003, -0.002, -0.001]).reshape(1, 3, 1) + rng.normal(0, 1e-4, b0.shape)
+    ka, _ = O.elements_poly(a)
+    want0 = sort_pairs(O.brute_pairs(ka, O.elements_poly(b0)[0]))
+    want1 = sort_pairs(O.brute_pairs(ka, O.elements_poly(b1)[0]))
+    assert len(want0) > 0 and len(want1) > 0 and not np.array_equal(want0, want1)
+    with CollisionWorld(16, leaf_size=4, precision=precision) as w:
+        oa = gpu_object(w, "poly", a, device=True)
+        ob = gpu_object(w, "poly", b0, device=True)
+        t1 = to_cuda(b1)
+        ob.refit(t1)
+        oa.broadphase(ob)
+        got1 = sort_pairs(oa.results()[:, [1, 3]])
+        ob.refit(to_cuda(b0))
+        oa.broadphase(ob)
+        got0 = sort_pairs(oa.results()[:, [1, 3]])
+    assert np.array_equal(got1, want1)
+    assert np.array_equal(got0, want0)
