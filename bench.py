@@ -9,10 +9,12 @@ patches, bottom-up tree with refit, leaf size 4) and A is collided with B
 (patch broadphase + element narrowphase); the step ends when the pair count is
 on the host.
 
-Prints one JSON line (see the contract in the repository's README of the
-benchmark driver): value = ms per step (lower is better), the roofline of the
-dominant kernel from HIP events on the library's stream, and the CPU oracle
-timed on this host's cores as a baseline.
+Prints one JSON line: value = ms per step (lower is better), the roofline of
+the dominant kernel from HIP events on the library's stream, and the CPU
+oracle timed on this host's cores as a baseline.  --precision both adds the
+fp32-outward form's time and whether its pairs equal the fp64 form's (count
+and an order-independent digest).  --workload spheres-self times self
+collision of a 16,777,216-sphere cloud instead.
 """
 import argparse
 import json
@@ -86,21 +88,145 @@ def cpu_baseline(a, b, od, leaf):
     }
 
 
+def run_steps(world, step, steps, warmup, stream, torch):
+    """Warmup, then `steps` timed steps bracketed by HIP events on the
+    library's stream; returns (ms per step, per-stage ms per step, per-stage
+    ms per launch, last step's return value)."""
+    for _ in range(warmup):
+        step()
+        world.stage_times()
+    torch.cuda.synchronize()
+    stage_sum, stage_cnt = {}, {}
+    total_ms = 0.0
+    ret = None
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ret = step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        total_ms += e0.elapsed_time(e1)
+        for name, ms in world.stage_times():
+            stage_sum[name] = stage_sum.get(name, 0.0) + ms
+            stage_cnt[name] = stage_cnt.get(name, 0) + 1
+    stages = {k: stage_sum[k] / steps for k in stage_sum}
+    per_launch = {k: stage_sum[k] / stage_cnt[k] for k in stage_sum}
+    counts = {k: stage_cnt[k] / steps for k in stage_cnt}
+    return total_ms / steps, stages, per_launch, counts, ret
+
+
+def roofline_of(stages, per_launch, counts, nbytes_fn, ms_per_step):
+    dom = max(stages, key=lambda k: stages[k])
+    nbytes = nbytes_fn(dom)
+    t_launch = per_launch[dom] * 1e-3
+    achieved = nbytes / t_launch / 1e9 if t_launch > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": int(nbytes)}
+    step_bytes = sum(nbytes_fn(k) * counts[k] for k in stages)
+    step_frac = step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+    return roofline, round(step_frac, 4)
+
+
+def bench_sheets(args, torch, stream):
+    """Two wavy sheets, both rebuilt per step, A collided with B."""
+    from distbvh_amd import CollisionWorld
+    from distbvh_amd import workloads as W
+
+    a, b = W.wavy_sheets(args.nq)
+    n = a.shape[2]
+    ta = torch.from_numpy(a).cuda()
+    tb = torch.from_numpy(b).cuda()
+    out = {}
+    for prec in args.precisions:
+        fp32 = prec == "fp32"
+        world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
+                               stream=stream.cuda_stream, precision=prec)
+        oa = world.create_collision_object()
+        ob = world.create_collision_object()
+
+        def step():
+            oa.set_entity_data_poly(ta)
+            ob.set_entity_data_poly(tb)
+            oa.init_broadphase()
+            ob.init_broadphase()
+            oa.broadphase(ob)
+            return oa.result_count()
+
+        ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
+        qstats = oa.result_stats()
+        roof, step_frac = roofline_of(stages, per_launch, counts,
+                                      lambda k: kernel_bytes(k, n, npairs, n, fp32), ms)
+        out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
+                     "roofline": roof, "step_hbm_frac": step_frac, "digest": oa.digest()}
+        world.close()
+    return a, b, n, out
+
+
+def bench_spheres_self(args, torch, stream):
+    """One sphere cloud (Gaussian mixture, translated far from the origin),
+    rebuilt per step, self collision (each unordered pair once)."""
+    from distbvh_amd import CollisionWorld
+    from distbvh_amd import workloads as W
+
+    s = W.sphere_cloud(args.spheres)
+    n = s.shape[1]
+    ts = torch.from_numpy(s).cuda()
+    del s
+    out = {}
+    for prec in args.precisions:
+        fp32 = prec == "fp32"
+        world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
+                               stream=stream.cuda_stream, precision=prec)
+        o = world.create_collision_object()
+
+        def step():
+            o.set_entity_data_sphere(ts)
+            o.init_broadphase()
+            o.self_collision()
+            return o.result_count()
+
+        ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
+        qstats = o.result_stats()
+
+        def nb(k):
+            if k == "bounds":  # read x, y, z, r; write centroid
+                return n * (32 + 24)
+            if k == "tree":
+                slab = 104 if fp32 else 208
+                return n * (4 + 4 + 32 + slab + 4) + (n - 1) * (3 * slab + 12)
+            return kernel_bytes(k, n, npairs, n, fp32)
+
+        roof, step_frac = roofline_of(stages, per_launch, counts, nb, ms)
+        out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
+                     "roofline": roof, "step_hbm_frac": step_frac, "digest": o.digest()}
+        world.close()
+    return n, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["sheets", "spheres-self"], default="sheets",
+                    help="sheets: the 2 x 2,097,152-triangle contact step (default); spheres-self: "
+                         "self collision of a translated Gaussian-mixture sphere cloud")
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
+    ap.add_argument("--spheres", type=int, default=16777216, help="spheres of the spheres-self workload")
     ap.add_argument("--od", type=int, default=256)
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
-                    help="fp64 tree, or the fp32-outward tree with an exact fp64 re-check")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="fp64",
+                    help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
+                         "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized device buffer once (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    args.precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -108,8 +234,6 @@ def main():
         sys.exit(2)
 
     import torch
-    from distbvh_amd import CollisionWorld
-    from distbvh_amd import workloads as W
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.calibrate_copy:
@@ -118,93 +242,63 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    a, b = W.wavy_sheets(args.nq)
-    n = a.shape[2]
-    ta = torch.from_numpy(a).cuda()
-    tb = torch.from_numpy(b).cuda()
     stream = torch.cuda.current_stream()
-    fp32 = args.precision == "fp32"
-    world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
-                           stream=stream.cuda_stream, precision=args.precision)
-    oa = world.create_collision_object()
-    ob = world.create_collision_object()
-
-    def step():
-        oa.set_entity_data_poly(ta)
-        ob.set_entity_data_poly(tb)
-        oa.init_broadphase()
-        ob.init_broadphase()
-        oa.broadphase(ob)
-        return oa.result_count()
-
-    for _ in range(args.warmup):
-        step()
-        world.stage_times()
-    torch.cuda.synchronize()
-
-    stage_sum = {}
-    stage_cnt = {}
-    total_ms = 0.0
-    npairs = 0
-    for _ in range(args.steps):
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        npairs = step()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        total_ms += e0.elapsed_time(e1)
-        for name, ms in world.stage_times():
-            stage_sum[name] = stage_sum.get(name, 0.0) + ms
-            stage_cnt[name] = stage_cnt.get(name, 0) + 1
-    ms_per_step = total_ms / args.steps
-    qstats = oa.result_stats()
+    if args.workload == "sheets":
+        a, b, n, res = bench_sheets(args, torch, stream)
+        workload = ("wavy contact sheets, 2 objects x %d triangles (%dx%d quads), fp64 vertices, 26-DOP, "
+                    "63-bit Morton, %d patches, leaf size %d" % (n, args.nq, args.nq, args.od, args.leaf))
+        metric = "ms/step build+broad+narrow @2M tris/object, %HBM peak; batched p99 latency"
+    else:
+        n, res = bench_spheres_self(args, torch, stream)
+        workload = ("sphere cloud self collision, %d spheres (r = 1e-3, 64-component Gaussian mixture, "
+                    "translated by (1e4, -1e4, 1e4)), 26-DOP, 63-bit Morton, %d patches, leaf size %d"
+                    % (n, args.od, args.leaf))
+        metric = "ms/step build+self-collision @%d spheres" % n
+    first = res[args.precisions[0]]
+    fp32 = args.precisions[0] == "fp32"
     waves = (n + 63) // 64
-    stages = {k: stage_sum[k] / args.steps for k in stage_sum}           # ms per step
-    per_launch = {k: stage_sum[k] / stage_cnt[k] for k in stage_sum}      # ms per launch
-
-    # dominant kernel: largest share of the step
-    dom = max(stages, key=lambda k: stages[k])
-    nbytes = kernel_bytes(dom, n, npairs, n, fp32)
-    t_launch = per_launch[dom] * 1e-3
-    achieved = nbytes / t_launch / 1e9 if t_launch > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": int(nbytes)}
-    step_bytes = sum(kernel_bytes(k, n, npairs, n, fp32) * stage_cnt[k] / args.steps for k in stages)
-
+    st = first["stats"]
     out = {
-        "metric": "ms/step build+broad+narrow @2M tris/object, %HBM peak; batched p99 latency",
-        "value": round(ms_per_step, 4),
+        "metric": metric,
+        "value": round(first["ms_per_step"], 4),
         "unit": "ms/step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(first["ms_per_step"], 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "wavy contact sheets, 2 objects x %d triangles (%dx%d quads), fp64, "
-                               "26-DOP, 63-bit Morton, %d patches, leaf size %d" % (n, args.nq, args.nq, args.od, args.leaf),
+        "config": {"workload": workload,
                    "precision": "fp32 tree, exact fp64 re-check" if fp32 else "fp64",
-                   "candidate_pairs": int(qstats["candidate_pairs"]),
-                   "rechecked_pairs": int(qstats["rechecked_pairs"]),
-                   "triangles_per_object": n, "od": args.od, "leaf_size": args.leaf, "pairs": int(npairs),
-                   "patch_pairs": int(qstats["patch_pairs"]),
-                   "node_visits_per_wave": round(qstats["node_visits"] / waves, 1),
+                   "elements_per_object": n, "od": args.od, "leaf_size": args.leaf,
+                   "pairs": first["npairs"],
+                   "patch_pairs": int(st["patch_pairs"]),
+                   "candidate_pairs": int(st["candidate_pairs"]),
+                   "rechecked_pairs": int(st["rechecked_pairs"]),
+                   "node_visits_per_wave": round(st["node_visits"] / waves, 1),
                    "parallelism": "single GPU"},
-        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "step_hbm_frac": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "roofline": roofline,
+        "stages_ms": {k: round(v, 4) for k, v in first["stages"].items()},
+        "step_hbm_frac": first["step_hbm_frac"],
+        "roofline": first["roofline"],
     }
-    world.close()
-    if not args.no_cpu:
+    if len(args.precisions) == 2:
+        r64, r32 = res["fp64"], res["fp32"]
+        out["precision_forms"] = {
+            "fp64_ms": round(r64["ms_per_step"], 4), "fp32_ms": round(r32["ms_per_step"], 4),
+            "fp32_speedup": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
+            "pairs_identical": bool(r64["npairs"] == r32["npairs"] and r64["digest"] == r32["digest"]),
+            "fp32_stages_ms": {k: round(v, 4) for k, v in r32["stages"].items()},
+            "fp32_candidate_pairs": int(r32["stats"]["candidate_pairs"]),
+            "fp32_rechecked_pairs": int(r32["stats"]["rechecked_pairs"]),
+            "fp32_max_slab_error_ulps": r32["stats"]["max_slab_error_ulps"],
+        }
+    if not args.no_cpu and args.workload == "sheets":
         out["cpu_baseline"] = cpu_baseline(a, b, args.od, args.leaf)
         cpu_pairs = out["cpu_baseline"].pop("pairs")
-        out["config"]["cpu_pairs_equal_count"] = bool(cpu_pairs == npairs)
+        out["config"]["cpu_pairs_equal_count"] = bool(cpu_pairs == first["npairs"])
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

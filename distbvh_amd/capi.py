@@ -76,6 +76,7 @@ SIGNATURES = {
     "bvh_result_count": [_vp, C.POINTER(_u64)],
     "bvh_result_records": [_vp, _vp, _u64, _u32],
     "bvh_result_pairs": [_vp, _vp, _u64, _u32],
+    "bvh_result_digest": [_vp, _vp],
     "bvh_result_pairs_device": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
     "bvh_result_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_d)],
     "bvh_patch_pair_count": [_vp, C.POINTER(_u64)],

@@ -208,6 +208,12 @@ class CollisionObject:
         return {"node_visits": v.value, "patch_pairs": pp.value, "candidate_pairs": cand.value,
                 "rechecked_pairs": rck.value, "max_slab_error_ulps": err.value}
 
+    def digest(self):
+        """Order-independent (sum, xor) digest of the pair keys (device)."""
+        d = (C.c_uint64 * 2)()
+        call("bvh_result_digest", self.handle, d)
+        return int(d[0]), int(d[1])
+
     def pairs_device(self):
         p = C.c_void_p()
         n = C.c_uint64()

@@ -431,6 +431,8 @@ static bvh_status sort_results(bvh_object_s* o) {
   hipStream_t s = o->w->stream;
   const uint64_t n = o->npairs;
   if (n < 2) return BVH_OK;
+  // the sort's look-back words carry 30-bit counts
+  if (n >= (1ull << 30)) return fail(BVH_E_UNSUPPORTED, "sort_results holds at most 2^30 - 1 pairs");
   BVH_TRY(o->pairs_a.ensure(n * 8));
   BVH_TRY(o->pairs_b.ensure(n * 8));
   BVH_TRY(o->sort_hist.ensure(8 * 256 * 4));
@@ -792,6 +794,17 @@ bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_
   *pairs = o->have_results ? o->result_ptr : nullptr;
   *count = o->have_results ? o->npairs : 0;
   return BVH_OK;
+}
+
+bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]) {
+  if (!o || !digest) return fail(BVH_E_INVALID, "null argument");
+  const uint64_t n = o->have_results ? o->npairs : 0;
+  hipStream_t s = o->w->stream;
+  BVH_TRY(o->count.ensure(4 * sizeof(unsigned long long)));
+  unsigned long long* d = o->count.as<unsigned long long>();
+  HIP_TRY(launch_fill_u32((uint32_t*)d, 4, 0u, s));
+  HIP_TRY(launch_digest(o->result_ptr, n, d, s));
+  return copy_out(digest, d, 2 * sizeof(uint64_t), BVH_MEM_HOST, s);
 }
 
 bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t mem) {

@@ -148,6 +148,8 @@ hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t*
                           const uint64_t* gid_a, const uint32_t* patch_b, const uint64_t* gid_b,
                           uint64_t patch_offset_a, uint64_t patch_offset_b, uint64_t* out,
                           hipStream_t s);
+// out[0] += sum of mix(key), out[1] ^= xor of mix'(key) (out zeroed by the caller)
+hipError_t launch_digest(const uint64_t* pairs, uint64_t count, unsigned long long* out, hipStream_t s);
 hipError_t launch_patch_centroids(const double* centroid_planes, const double* sphere_planes,
                                   const double* snap_centroids, const uint32_t* perm,
                                   const uint32_t* splits, uint64_t n, uint64_t od, double* out,

@@ -397,6 +397,33 @@ __global__ __launch_bounds__(64) void k_patch_centroids(const double* __restrict
   for (int c = 0; c < 3; ++c) out[3 * k + c] = (e > b) ? s[c] / (double)(e - b) : 0.0;
 }
 
+// Order-independent digest of a pair list: the sum and the xor of two
+// independent 64-bit mixes (splitmix64 finalizer) of every key.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_digest(const uint64_t* __restrict__ pairs, uint64_t count,
+                                                unsigned long long* out) {
+  uint64_t s = 0, x = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = pairs[i];
+    s += mix64(k);
+    x ^= mix64(k ^ 0x9e3779b97f4a7c15ull);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, 64);
+    x ^= __shfl_xor(x, o, 64);
+  }
+  if (lane_id() == 0 && (s | x) != 0) {
+    atomicAdd(out, (unsigned long long)s);
+    atomicXor(out + 1, (unsigned long long)x);
+  }
+}
+
 // ---- launchers ---------------------------------------------------------------------
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
                               const PairOut& out, hipStream_t s) {
@@ -433,6 +460,14 @@ hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t*
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_records, dim3((unsigned)blocks), dim3(256), 0, s, pairs, count, patch_a, gid_a, patch_b,
                      gid_b, off_a, off_b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_digest(const uint64_t* pairs, uint64_t count, unsigned long long* out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_digest, dim3((unsigned)blocks), dim3(256), 0, s, pairs, count, out);
   return hipGetLastError();
 }
 

@@ -171,6 +171,11 @@ bvh_status bvh_result_count(bvh_object o, uint64_t *count);
 bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record *out, uint64_t cap, uint32_t mem);
 /* compact pairs (input index of element_p << 32 | input index of element_q) */
 bvh_status bvh_result_pairs(bvh_object o, uint64_t *out, uint64_t cap, uint32_t mem);
+/* order-independent digest of the compact pairs, computed on the device:
+ * digest[0] = sum of mix(key), digest[1] = xor of mix'(key), with mix the
+ * splitmix64 finalizer.  Equal pair sets give equal digests in any order (no
+ * reference counterpart; for comparing result sets too large to copy). */
+bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]);
 /* device pointer to the compact pairs (valid until the next broadphase) */
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_t *count);
 /* counters of the last query: tree nodes visited (summed over waves of 64
