@@ -68,6 +68,11 @@ SIGNATURES = {
     "bvh_set_entity_data_sphere": [_vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_set_entity_data_snapshot": [_vp, _vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_refit": [_vp, _vp, _u32],
+    "bvh_autotune_leaf_size": [_vp, _vp, _vp, _vp],
+    "bvh_capture_begin": [_vp],
+    "bvh_capture_end": [_vp, _vp],
+    "bvh_graph_launch": [_vp],
+    "bvh_graph_destroy": [_vp],
     "bvh_init_broadphase": [_vp],
     "bvh_broadphase": [_vp, _vp],
     "bvh_broadphase_patches": [_vp, _vp],

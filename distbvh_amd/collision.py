@@ -98,6 +98,11 @@ class CollisionWorld:
         return [(names[i].decode(), ms[i]) for i in range(min(cnt.value, 64))]
 
     # stand-alone primitives on device tensors (torch)
+    def capture(self):
+        """Context manager capturing the world's stream into a StepGraph:
+        `with world.capture() as g: ...step...`, then `g.launch()`."""
+        return StepGraph(self)
+
     def sort(self, keys, vals=None):
         fn = "bvh_sort_u64" if keys.element_size() == 8 else "bvh_sort_u32"
         call(fn, self.handle, C.c_void_p(keys.data_ptr()),
@@ -105,6 +110,43 @@ class CollisionWorld:
 
     def exclusive_scan(self, vals):
         call("bvh_exclusive_scan_u32", self.handle, C.c_void_p(vals.data_ptr()), vals.numel())
+
+
+class StepGraph:
+    """A step captured into a HIP graph (bvh_capture_begin / _end)."""
+
+    def __init__(self, world):
+        self.world = world
+        self.handle = None
+
+    def __enter__(self):
+        call("bvh_capture_begin", self.world.handle)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        if exc_type is not None:
+            capi.load().bvh_capture_end(self.world.handle, C.byref(h))
+            if h.value:
+                capi.load().bvh_graph_destroy(h)
+            return False
+        call("bvh_capture_end", self.world.handle, C.byref(h))
+        self.handle = h
+        return False
+
+    def launch(self):
+        call("bvh_graph_launch", self.handle)
+
+    def close(self):
+        if self.handle is not None:
+            capi.load().bvh_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CollisionObject:
@@ -159,6 +201,13 @@ class CollisionObject:
         p, mem, keep = _ptr_mem(data, np.float64)
         self._keep = (keep,) + tuple(self._keep[1:])
         call("bvh_refit", self.handle, p, mem)
+
+    def autotune_leaf_size(self, other=None):
+        """bvh_autotune_leaf_size: (best leaf size, ms for leaf 1, 2, 4, 8)."""
+        best = C.c_uint32()
+        ms = (C.c_double * 4)()
+        call("bvh_autotune_leaf_size", self.handle, (other or self).handle, C.byref(best), ms)
+        return best.value, list(ms)
 
     def init_broadphase(self):
         call("bvh_init_broadphase", self.handle)

@@ -55,7 +55,7 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Centroid (poly: accumulate(vertices, zeros) / nv) and the cardinal slab
-// bounds; block partials; the last block finalizes min / inv_diag / max.
+// bounds as block partials (k_prep_final folds them).
 template <int KIND, int NV>
 __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, ZeroList zl) {
   const uint64_t n = in.n;
@@ -115,8 +115,7 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, 
       mx[c] = (mx[c] < hi[c]) ? hi[c] : mx[c];
     }
   }
-  __shared__ double s_part[4][6];
-  __shared__ bool s_last;
+  __shared__ double s_part[kPrepBlock / 64][6];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -137,31 +136,29 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, 
         r[3 + c] = sel_max(r[3 + c], s_part[ww][3 + c]);
       }
     for (int c = 0; c < 6; ++c) b.partials[blockIdx.x * 6 + c] = r[c];
-    __threadfence();
-    unsigned prev = atomicAdd(b.done_counter, 1u);
-    s_last = (prev == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!s_last) return;
-  // last block: every thread folds a strided share of the block partials
-  __threadfence();
+}
+
+// Fold of the block partials into min / inv_diag / max (one block).  A
+// separate launch instead of a last-block fold: on gfx950 the device-scope
+// fence that fold needs writes back the whole L2 in every block.
+__global__ __launch_bounds__(kPrepBlock) void k_prep_final(BuildBufs b, uint32_t nblocks) {
+  __shared__ double s_part[kPrepBlock / 64][6];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double r[6];
   for (int c = 0; c < 3; ++c) {
     r[c] = 1.7976931348623157e308;
     r[3 + c] = -1.7976931348623157e308;
   }
-  for (unsigned blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x)
+  for (unsigned blk = threadIdx.x; blk < nblocks; blk += blockDim.x)
     for (int c = 0; c < 3; ++c) {
-      const double a = __hip_atomic_load(&b.partials[blk * 6 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const double z = __hip_atomic_load(&b.partials[blk * 6 + 3 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      r[c] = sel_min(r[c], a);
-      r[3 + c] = sel_max(r[3 + c], z);
+      r[c] = sel_min(r[c], b.partials[blk * 6 + c]);
+      r[3 + c] = sel_max(r[3 + c], b.partials[blk * 6 + 3 + c]);
     }
   for (int c = 0; c < 3; ++c) {
     r[c] = wave_min(r[c]);
     r[3 + c] = wave_max(r[3 + c]);
   }
-  __syncthreads();
   if (lane == 0)
     for (int c = 0; c < 6; ++c) s_part[w][c] = r[c];
   __syncthreads();
@@ -181,7 +178,6 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, 
     }
     bd.pad = 0.0;
     *b.bounds = bd;
-    *b.done_counter = 0;  // ready for the next build
   }
 }
 
@@ -429,7 +425,9 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t o
       *b.root = node;
       return;
     }
-    const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
+    const uint32_t dr = (r != n - 1) ? b.delta[r] : 0u;
+    const uint32_t dl = (l != 0) ? b.delta[l - 1] : 0u;
+    const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(s_nfront, 1u);
@@ -458,7 +456,7 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t o
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
     node_store<FP32>(b, parent, kd, l, r, mypatch);
     b.ilast[parent] = r;
-    if (mypatch != kMixed && patch_begin(b.splits, od, mypatch) == l && patch_end(b.splits, od, n, mypatch) == r + 1)
+    if (mypatch != kMixed && (l == 0 || b.delta[l - 1] >= 128) && (r == n - 1 || b.delta[r] >= 128))
       b.patch_root[mypatch] = parent;
     node = parent;
   }
@@ -600,7 +598,9 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
     node_store<FP32>(b, parent, kd, l, r, mypatch);
     b.ilast[parent] = r;
-    if (mypatch != kMixed && patch_begin(b.splits, od, mypatch) == l && patch_end(b.splits, od, n, mypatch) == r + 1)
+    // a one-patch node is its patch's root when a patch boundary (delta >=
+    // 128) lies on both sides
+    if (mypatch != kMixed && (l == 0 || L.delta[l - c0] >= 128) && (r == n - 1 || L.delta[r - c0 + 1] >= 128))
       b.patch_root[mypatch] = parent;
     publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
     node = parent;
@@ -685,10 +685,12 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < L.nfront) ? L.front[t] : kEnd;
 }
 
+// Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one
+// thread per handed-over node (two per sub-chunk).
 template <bool FP32>
-__global__ __launch_bounds__(kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t od, uint64_t chunk,
-                                                        uint64_t nsub, const uint32_t* __restrict__ front_in,
-                                                        uint32_t* __restrict__ front_out) {
+__global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t od, uint64_t chunk,
+                                                            uint64_t nsub, const uint32_t* __restrict__ front_in,
+                                                            uint32_t* __restrict__ front_out) {
   __shared__ uint32_t s_front[2];
   __shared__ uint32_t s_nfront;
   const uint32_t t = threadIdx.x;
@@ -697,15 +699,12 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n,
   const uint32_t c1 = (uint32_t)(c0l + chunk < n ? c0l + chunk : n);
   if (t == 0) s_nfront = 0;
   __syncthreads();
-  const uint64_t sub = (uint64_t)blockIdx.x * kTreeChunk + t;
-  if (sub < nsub) {
-    for (int i = 0; i < 2; ++i) {
-      const uint32_t node = front_in[2 * sub + i];
-      if (node == kEnd) continue;
-      if (node >= 2 * n - 1) {
-        atomicOr(b.status, kErrTree);
-        continue;
-      }
+  const uint64_t slot = (uint64_t)blockIdx.x * (2 * kTreeChunk) + t;  // = 2 * sub + i
+  if (slot < 2 * nsub) {
+    const uint32_t node = front_in[slot];
+    if (node != kEnd && node >= 2 * n - 1) {
+      atomicOr(b.status, kErrTree);
+    } else if (node != kEnd) {
       typename Slab<FP32>::T kd[kSlabs];
       uint32_t l, r, pk;
       node_load<FP32>(b, node, kd, l, r, pk);
@@ -769,6 +768,9 @@ hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s) {
   } else {
     prep_dispatch<kSnapshot, 1>(in, b, zl, s);
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_prep_final, dim3(1), dim3(kPrepBlock), 0, s, b, (uint32_t)prep_blocks(in.n));
   return hipGetLastError();
 }
 
@@ -872,8 +874,8 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     const uint64_t nsub = (n + chunk - 1) / chunk;
     const uint64_t next = chunk * kTreeChunk;
     const unsigned blocks = (unsigned)((n + next - 1) / next);
-    if (fp32) hipLaunchKernelGGL(k_tree_up<true>, dim3(blocks), dim3(kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
-    else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
+    if (fp32) hipLaunchKernelGGL(k_tree_up<true>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
+    else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t* tmp = fin;

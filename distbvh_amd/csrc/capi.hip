@@ -18,6 +18,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <atomic>
 
 #include "../../include/distbvh.h"
 #include "kernels.hpp"
@@ -48,6 +49,11 @@ bvh_status fail(bvh_status s, const std::string& msg) {
   } while (0)
 
 // grow-only device buffer
+// Every device (re)allocation bumps the generation; a graph captured at an
+// older generation may hold freed pointers and is refused.
+std::atomic<uint64_t> g_alloc_gen{0};
+thread_local bool t_capturing = false;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -59,6 +65,9 @@ struct DevBuf {
   }
   bvh_status ensure(size_t want) {
     if (want <= bytes && p) return BVH_OK;
+    if (t_capturing)
+      return fail(BVH_E_STATE, "a device buffer would grow inside a capture: run the step once before capturing");
+    g_alloc_gen.fetch_add(1);
     if (p) {
       (void)hipFree(p);
       p = nullptr;
@@ -103,7 +112,10 @@ struct bvh_world_s {
   DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
   // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
   DevBuf frame;
-  unsigned long long* h_count = nullptr;  // pinned
+  // stream capture (batching form): objects whose queries were captured
+  bool capturing = false;
+  std::vector<bvh_object_s*> captured;
+  unsigned long long* h_count = nullptr;  // pinned, 16 words: query counters and status words
   bool iteration_open = false;
 
   hipEvent_t take_event() {
@@ -119,7 +131,7 @@ struct bvh_world_s {
     events_used = 0;
   }
   void stage_begin(const char* name) {
-    if (!(cfg.flags & BVH_FLAG_TIMING) || stages.size() >= 512) return;
+    if (!(cfg.flags & BVH_FLAG_TIMING) || capturing || stages.size() >= 512) return;
     Stage s{name, take_event(), take_event()};
     if (s.a && s.b) {
       (void)hipEventRecord(s.a, stream);
@@ -127,7 +139,7 @@ struct bvh_world_s {
     }
   }
   void stage_end() {
-    if (!(cfg.flags & BVH_FLAG_TIMING) || stages.empty()) return;
+    if (!(cfg.flags & BVH_FLAG_TIMING) || capturing || stages.empty()) return;
     (void)hipEventRecord(stages.back().b, stream);
   }
 };
@@ -152,7 +164,7 @@ struct bvh_object_s {
   int bits = 63;
   int fp32 = 0;
   // results
-  DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count, recheck;
+  DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count, recheck, digest;
   uint64_t npairs = 0;
   uint64_t npatch_pairs = 0;
   uint64_t nvisits = 0;
@@ -163,6 +175,10 @@ struct bvh_object_s {
   bvh_object_s* result_other = nullptr;
   bool self_result = false;
   bool have_results = false;
+  bool results_pending = false;  // a captured query: counters not read yet
+  // the last launched query: kind and the list capacities it ran with
+  bool q_self = false, q_narrow = false;
+  uint64_t q_pcap = 0, q_cap = 0, q_rcap = 0;
 };
 
 // ---- helpers ---------------------------------------------------------------------------
@@ -222,7 +238,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->counters.ensure(n1 * 4));
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
-  BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, last-block counter, max slab error, -]
+  BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, -, max fp32 slab error, -]
   BVH_TRY(o->rightc.ensure(n1 * 4));
   BVH_TRY(o->ropes.ensure((2 * n1 - 1) * 4));
   BVH_TRY(o->ilast.ensure(n1 * 4));
@@ -237,7 +253,6 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BuildBufs& b = o->bb;
   b.centroid = o->centroid.as<double>();
   b.partials = o->partials.as<double>();
-  b.done_counter = o->status.as<uint32_t>() + 1;
   b.bounds = o->bounds.as<Bounds>();
   b.codes0 = o->codes0.p;
   b.codes_a = o->codes_a.p;
@@ -278,7 +293,7 @@ static bvh_status object_build(bvh_object_s* o) {
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n;
   BVH_TRY(object_alloc(o));
-  HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, bounds counter, slab error
+  HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, fp32 slab error
   if (n == 0) {
     // splits are all 0 (split/cluster.hpp:101-105), every patch empty
     HIP_TRY(launch_fill_u32(o->bb.splits, w->od > 1 ? w->od - 1 : 0, 0u, s));
@@ -371,6 +386,8 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   o->fp32 = w->cfg.precision == BVH_PREC_FP32_OUTWARD ? 1 : 0;
   size_t planes = kind == kPoly ? (size_t)nv * 3 : (kind == kSphere ? 4 : kSlabs);
   size_t bytes = planes * n * sizeof(double);
+  if (w->capturing && (mem == BVH_MEM_HOST || gids))
+    return fail(BVH_E_STATE, "a capture takes device element data without global ids");
   if (mem == BVH_MEM_HOST && n > 0) {
     w->stage_begin("upload");
     BVH_TRY(o->input.ensure(bytes));
@@ -400,24 +417,32 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   return object_build(o);
 }
 
-// Reads the count a kernel left in `dcount` and the device error words of
-// both objects (one synchronization).
-static bvh_status read_count(bvh_object_s* o, bvh_object_s* other, unsigned long long* dcount, uint64_t* out) {
+// Counter words of a query (device, o->count): pairs, node visits, fp32
+// candidates, fp32 re-checks, patch pairs.
+constexpr int kCntPairs = 0, kCntVisits = 1, kCntCand = 2, kCntRecheck = 3, kCntPatch = 4, kCntWords = 5;
+
+// Reads the counters of the last query of `o` and the device error words of
+// both objects (one synchronization).  Sets *overflow when a list outgrew
+// its buffer (the query must run again with larger buffers).
+static bvh_status read_counts(bvh_object_s* o, bvh_object_s* other, bool* overflow) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
-  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[4], o->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[6], other->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  const unsigned long long* dcount = o->count.as<unsigned long long>();
+  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, kCntWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[kCntWords], o->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&w->h_count[kCntWords + 2], other->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  o->nvisits = w->h_count[1];
-  o->ncandidates = w->h_count[2];
-  o->nrecheck = w->h_count[3];
+  o->npairs = o->q_narrow ? w->h_count[kCntPairs] : 0;
+  o->nvisits = w->h_count[kCntVisits];
+  o->ncandidates = w->h_count[kCntCand];
+  o->nrecheck = w->h_count[kCntRecheck];
+  o->npatch_pairs = o->q_self ? 0 : w->h_count[kCntPatch];
   uint32_t sw[8];
-  std::memcpy(sw, &w->h_count[4], sizeof(sw));
+  std::memcpy(sw, &w->h_count[kCntWords], sizeof(sw));
   o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
   const uint32_t st = sw[0] | sw[4];
   if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
-  *out = w->h_count[0];
+  *overflow = o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck > o->q_rcap);
   return BVH_OK;
 }
 
@@ -451,71 +476,91 @@ static bvh_status sort_results(bvh_object_s* o) {
   return BVH_OK;
 }
 
-static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true) {
+// Launches a query with the current buffer sizes; no synchronization (so it
+// can be captured into a graph).  The counters stay on the device.
+static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
-  if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
-  BVH_TRY(o->count.ensure(4 * sizeof(unsigned long long)));
   unsigned long long* dcount = o->count.as<unsigned long long>();
   const TreeView qv = view_of(o), tv = view_of(other);
-  // patch broadphase
+  o->q_self = self;
+  o->q_narrow = narrow;
+  o->q_pcap = o->ppairs.bytes / 8;
+  o->q_cap = o->pairs.bytes / 8;
+  o->q_rcap = o->recheck.bytes / 8;
+  HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntWords, 0u, s));
   if (!self) {
     w->stage_begin("broadphase");
-    const uint64_t pcap = std::max<uint64_t>(1024, 4 * w->od);
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      uint64_t cap = o->ppairs.bytes / 8;
-      if (cap < pcap) {
-        BVH_TRY(o->ppairs.ensure(pcap * 8));
-        cap = pcap;
-      }
-      HIP_TRY(launch_fill_u32((uint32_t*)dcount, 8, 0u, s));
-      PairOut po{o->ppairs.as<uint64_t>(), cap, dcount, nullptr, nullptr, nullptr, 0, nullptr};
-      HIP_TRY(launch_patch_broadphase(qv, tv, o->fp32, po, s));
-      BVH_TRY(read_count(o, other, dcount, &o->npatch_pairs));
-      if (o->npatch_pairs <= cap) break;
-      BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024));
-    }
+    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr};
+    HIP_TRY(launch_patch_broadphase(qv, tv, o->fp32, po, s));
     w->stage_end();
-  } else {
-    o->npatch_pairs = 0;
   }
-  if (!narrow) {
-    o->npairs = 0;
-    o->nvisits = 0;
-    o->result_ptr = o->pairs.as<uint64_t>();
-    o->result_other = other;
-    o->self_result = false;
+  if (narrow) {
+    w->stage_begin("narrowphase");
+    PairOut po{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
+               o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntRecheck};
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, po, s));
+    w->stage_end();
+  }
+  o->result_ptr = o->pairs.as<uint64_t>();
+  o->result_other = other;
+  o->self_result = self;
+  return BVH_OK;
+}
+
+static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true) {
+  bvh_world w = o->w;
+  if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
+  o->have_results = false;
+  o->results_pending = false;
+  if (w->capturing && (w->cfg.flags & BVH_FLAG_SORT_RESULTS))
+    return fail(BVH_E_STATE, "sort_results cannot be captured (it needs the pair count on the host)");
+  // buffer sizes: grown here, outside any capture (a captured query uses them as they are)
+  BVH_TRY(o->count.ensure(2 * kCntWords * sizeof(unsigned long long)));
+  BVH_TRY(o->ppairs.ensure(std::max<uint64_t>(1024, 4 * w->od) * 8));
+  if (narrow) {
+    BVH_TRY(ensure_pairs(o, std::max<uint64_t>(1u << 20, 8 * o->in.n)));
+    if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n / 8) * 8));
+  }
+  if (w->capturing) {
+    BVH_TRY(launch_query(o, other, self, narrow));
     o->have_results = true;
+    o->results_pending = true;
+    w->captured.push_back(o);
     return BVH_OK;
   }
-  // element narrowphase
-  w->stage_begin("narrowphase");
-  uint64_t want = std::max<uint64_t>(1u << 20, 8 * o->in.n);
-  if (o->pairs.bytes / 8 < want) BVH_TRY(ensure_pairs(o, want));
-  if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n / 8) * 8));
   for (int attempt = 0; attempt < 2; ++attempt) {
-    const uint64_t cap = o->pairs.bytes / 8;
-    const uint64_t rcap = o->recheck.bytes / 8;
-    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 8, 0u, s));
-    PairOut po{o->pairs.as<uint64_t>(), cap, dcount, dcount + 1, dcount + 2,
-               o->recheck.as<uint64_t>(), rcap, dcount + 3};
-    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, po, s));
-    BVH_TRY(read_count(o, other, dcount, &o->npairs));
-    if (o->npairs <= cap && (!o->fp32 || o->nrecheck <= rcap)) break;
-    // overflow: grow to the exact counts and run again
-    if (o->npairs > cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
-    if (o->fp32 && o->nrecheck > rcap) BVH_TRY(o->recheck.ensure((o->nrecheck + (o->nrecheck >> 3) + 1024) * 8));
+    BVH_TRY(launch_query(o, other, self, narrow));
+    bool overflow = false;
+    BVH_TRY(read_counts(o, other, &overflow));
+    if (!overflow) break;
+    if (attempt == 1) return fail(BVH_E_OVERFLOW, "result lists still overflow after growing");
+    // grow to the exact counts and run again
+    if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
+    if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+    if (o->fp32 && o->nrecheck > o->q_rcap) BVH_TRY(o->recheck.ensure((o->nrecheck + (o->nrecheck >> 3) + 1024) * 8));
   }
-  w->stage_end();
-  o->result_ptr = o->pairs.as<uint64_t>();
-  if (w->cfg.flags & BVH_FLAG_SORT_RESULTS) {
+  if (narrow && (w->cfg.flags & BVH_FLAG_SORT_RESULTS)) {
     w->stage_begin("sort_results");
     BVH_TRY(sort_results(o));
     w->stage_end();
   }
-  o->result_other = other;
-  o->self_result = self;
   o->have_results = true;
+  return BVH_OK;
+}
+
+// Results of a captured query become known on the host at the first access
+// after the graph ran.
+static bvh_status results_ready(bvh_object_s* o) {
+  if (!o->have_results || !o->results_pending) return BVH_OK;
+  if (o->w->capturing) return fail(BVH_E_STATE, "results are read after the capture ends");
+  bool overflow = false;
+  BVH_TRY(read_counts(o, o->result_other, &overflow));
+  if (overflow) {
+    o->have_results = false;
+    return fail(BVH_E_OVERFLOW, "a captured query outgrew its buffers; run the step once outside the graph");
+  }
+  o->results_pending = false;
   return BVH_OK;
 }
 
@@ -607,7 +652,7 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
     HIP_TRY(launch_fill_u32(w->frame.as<uint32_t>(), 8, 0u, w->stream));
   }
   void* hc = nullptr;
-  HIP_TRY(hipHostMalloc(&hc, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(&hc, 16 * sizeof(unsigned long long), hipHostMallocDefault));
   w->h_count = (unsigned long long*)hc;
   *out = w.release();
   return BVH_OK;
@@ -719,6 +764,7 @@ bvh_status bvh_refit(bvh_object o, const double* data, uint32_t mem) {
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   const uint64_t n = o->in.n;
   if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
+  if (o->w->capturing && mem == BVH_MEM_HOST) return fail(BVH_E_STATE, "a capture takes device element data");
   o->have_results = false;
   if (n > 0) {
     const size_t planes = o->in.kind == kPoly ? (size_t)o->in.nv * 3 : (o->in.kind == kSphere ? 4 : kSlabs);
@@ -785,12 +831,14 @@ bvh_status bvh_end_phase(bvh_object o) {
 
 bvh_status bvh_result_count(bvh_object o, uint64_t* count) {
   if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(results_ready(o));
   *count = o->have_results ? o->npairs : 0;
   return BVH_OK;
 }
 
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_t* count) {
   if (!o || !pairs || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(results_ready(o));
   *pairs = o->have_results ? o->result_ptr : nullptr;
   *count = o->have_results ? o->npairs : 0;
   return BVH_OK;
@@ -798,10 +846,11 @@ bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_
 
 bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]) {
   if (!o || !digest) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   hipStream_t s = o->w->stream;
-  BVH_TRY(o->count.ensure(4 * sizeof(unsigned long long)));
-  unsigned long long* d = o->count.as<unsigned long long>();
+  BVH_TRY(o->digest.ensure(2 * sizeof(unsigned long long)));
+  unsigned long long* d = o->digest.as<unsigned long long>();
   HIP_TRY(launch_fill_u32((uint32_t*)d, 4, 0u, s));
   HIP_TRY(launch_digest(o->result_ptr, n, d, s));
   return copy_out(digest, d, 2 * sizeof(uint64_t), BVH_MEM_HOST, s);
@@ -809,6 +858,7 @@ bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]) {
 
 bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t mem) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "result buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
   if (n && !out) return fail(BVH_E_INVALID, "null output");
@@ -817,6 +867,7 @@ bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t 
 
 bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_t cap, uint32_t mem) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "record buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
   if (n == 0) return BVH_OK;
@@ -833,6 +884,7 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_
 bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs, uint64_t* candidates,
                             uint64_t* rechecked, double* max_slab_error_ulps) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(results_ready(o));
   if (node_visits) *node_visits = o->have_results ? o->nvisits : 0;
   if (patch_pairs) *patch_pairs = o->have_results ? o->npatch_pairs : 0;
   if (candidates) *candidates = o->have_results ? (o->fp32 ? o->ncandidates : o->npairs) : 0;
@@ -847,12 +899,14 @@ bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch
 
 bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {
   if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(results_ready(o));
   *count = o->have_results ? o->npatch_pairs : 0;
   return BVH_OK;
 }
 
 bvh_status bvh_patch_pairs(bvh_object o, uint64_t* out, uint64_t cap) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npatch_pairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "patch pair buffer too small");
   if (n == 0) return BVH_OK;
@@ -1024,6 +1078,116 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
   HIP_TRY(launch_fill_u32(w->s_ctl.as<uint32_t>(), 1, 0u, s));
   HIP_TRY(launch_scan_u32(vals, n, w->s_lb.as<uint64_t>(), w->s_ctl.as<uint32_t>(), s));
   HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+// ---- autotuner ---------------------------------------------------------------------------
+bvh_status bvh_autotune_leaf_size(bvh_object o, bvh_object other, uint32_t* best, double* ms) {
+  if (!o || !other || !best) return fail(BVH_E_INVALID, "null argument");
+  if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
+  bvh_world w = o->w;
+  if (w->capturing) return fail(BVH_E_STATE, "autotune inside a capture");
+  const bool self = o == other;
+  static const uint32_t kLeaf[4] = {1, 2, 4, 8};
+  const uint32_t keep = w->cfg.leaf_size;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best_ms = 0.0;
+  uint32_t best_leaf = keep;
+  bvh_status st = BVH_OK;
+  for (int k = 0; k < 4 && st == BVH_OK; ++k) {
+    w->cfg.leaf_size = kLeaf[k];
+    st = run_query(o, other, self);  // sizes the buffers
+    double t = 0.0;
+    for (int r = 0; r < 3 && st == BVH_OK; ++r) {
+      (void)hipEventRecord(e0, w->stream);
+      st = run_query(o, other, self);
+      (void)hipEventRecord(e1, w->stream);
+      (void)hipEventSynchronize(e1);
+      float f = 0.0f;
+      (void)hipEventElapsedTime(&f, e0, e1);
+      t = (r == 0 || f < t) ? f : t;
+    }
+    if (ms) ms[k] = t;
+    if (st == BVH_OK && (k == 0 || t < best_ms)) {
+      best_ms = t;
+      best_leaf = kLeaf[k];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  w->cfg.leaf_size = st == BVH_OK ? best_leaf : keep;
+  if (st != BVH_OK) return st;
+  *best = best_leaf;
+  return run_query(o, other, self);  // leave the results of the chosen setting
+}
+
+// ---- batching form: whole steps captured into a HIP graph ------------------------------
+struct bvh_graph_s {
+  bvh_world w = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t gen = 0;
+  std::vector<bvh_object_s*> objects;
+};
+
+bvh_status bvh_capture_begin(bvh_world w) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  if (w->capturing || t_capturing) return fail(BVH_E_STATE, "a capture is already open on this thread");
+  HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+  w->capturing = true;
+  t_capturing = true;
+  w->captured.clear();
+  return BVH_OK;
+}
+
+bvh_status bvh_capture_end(bvh_world w, bvh_graph* out) {
+  if (!w || !out) return fail(BVH_E_INVALID, "null argument");
+  *out = nullptr;
+  if (!w->capturing) return fail(BVH_E_STATE, "no capture open");
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(w->stream, &graph);
+  w->capturing = false;
+  t_capturing = false;
+  if (e != hipSuccess) {
+    w->captured.clear();
+    return fail(BVH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  auto g = std::make_unique<bvh_graph_s>();
+  g->w = w;
+  g->graph = graph;
+  g->gen = g_alloc_gen.load();
+  g->objects = w->captured;
+  w->captured.clear();
+  const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(BVH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  *out = g.release();
+  return BVH_OK;
+}
+
+bvh_status bvh_graph_launch(bvh_graph g) {
+  if (!g) return fail(BVH_E_INVALID, "null graph");
+  if (g->w->capturing) return fail(BVH_E_STATE, "graph launch inside a capture");
+  if (g->gen != g_alloc_gen.load())
+    return fail(BVH_E_STATE, "device buffers were reallocated since the capture; capture again");
+  HIP_TRY(hipGraphLaunch(g->exec, g->w->stream));
+  for (bvh_object_s* o : g->objects) {
+    o->have_results = true;
+    o->results_pending = true;
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_graph_destroy(bvh_graph g) {
+  if (!g) return BVH_OK;
+  (void)hipStreamSynchronize(g->w->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
   return BVH_OK;
 }
 

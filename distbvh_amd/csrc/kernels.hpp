@@ -49,7 +49,6 @@ struct BuildBufs {
   // original order
   double* centroid;        // 3 planes of n (kPoly)
   double* partials;        // per-block cardinal min/max (6 per block)
-  uint32_t* done_counter;  // last-block counter
   Bounds* bounds;
   // sort
   void* codes0;            // Morton codes, input order (u64 or u32)

@@ -155,3 +155,41 @@ def touching_spheres(n=4096, radius=1e-3, shift=(1e3, -1e3, 1e3), max_ulps=4, se
         x = np.where(k < -s, np.nextafter(x, -np.inf), x)
     b[0] = x
     return a, b
+
+
+def patch_batch(npatch=4096, median=2048, lo=64, hi=65536, sigma=1.32, lattice=16, overlap=0.1,
+                height=0.05, seed=0xC4):
+    """A batch of independent height-field meshes: patch k sits in cell
+    (k % L, (k // L) % L, k // L^2) of an L^3 lattice (cell 1/L), widened by
+    overlap/2 of a cell on each side in x and y so that neighbours overlap by
+    `overlap`; its vertex heights are cell_z + U[0, height].  Patch sizes are
+    lognormal (median `median` triangles, log-sd `sigma`) clipped to [lo, hi];
+    a patch of s triangles is an m x m quad grid, m = max(2, round(sqrt(s/2))).
+    Returns (planes (3, 3, N) float64, per-patch triangle counts)."""
+    cell = 1.0 / lattice
+    z = normal(seed, npatch)
+    sizes = np.clip(np.rint(median * np.exp(sigma * z)), lo, hi).astype(np.int64)
+    ms = np.maximum(2, np.rint(np.sqrt(sizes / 2.0))).astype(np.int64)
+    counts = 2 * ms * ms
+    total = int(counts.sum())
+    out = np.empty((3, 3, total), np.float64)
+    off = 0
+    hoff = 0
+    hseed = seed ^ 0x5A5A
+    for k in range(npatch):
+        m = int(ms[k])
+        ci, cj, cl = k % lattice, (k // lattice) % lattice, k // (lattice * lattice)
+        x0 = (ci - overlap / 2) * cell
+        y0 = (cj - overlap / 2) * cell
+        w = (1.0 + overlap) * cell
+        nvtx = (m + 1) * (m + 1)
+        hz = cl * cell + height * uniform(hseed, nvtx, offset=hoff).reshape(m + 1, m + 1)
+        hoff += nvtx
+        g = np.arange(m + 1) / m
+        vx = np.broadcast_to(x0 + w * g[None, :], (m + 1, m + 1))
+        vy = np.broadcast_to(y0 + w * g[:, None], (m + 1, m + 1))
+        t = grid_triangles(vx, vy, hz, m, m)
+        out[:, :, off:off + t.shape[2]] = t
+        off += t.shape[2]
+    assert off == total
+    return out, counts

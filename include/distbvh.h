@@ -73,6 +73,7 @@ typedef struct bvh_world_config {
 
 typedef struct bvh_world_s *bvh_world;
 typedef struct bvh_object_s *bvh_object;
+typedef struct bvh_graph_s *bvh_graph;
 
 /* detailed_narrowphase_result, tests/collision_object_test.cpp:318-324: the
  * 32-byte record the built-in narrowphase emits, read back through
@@ -222,6 +223,33 @@ bvh_status bvh_sort_u64(bvh_world w, uint64_t *keys, uint32_t *vals, uint64_t n)
 bvh_status bvh_sort_u32(bvh_world w, uint32_t *keys, uint32_t *vals, uint64_t n);
 /* prefix_sum, util/prefix_sum.hpp:41-54 (exclusive), in place */
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
+
+/* ---- autotuner ------------------------------------------------------------------ */
+/* Times the query o x other (self collision when o == other) with leaf sizes
+ * 1, 2, 4 and 8 (best of 3 runs each, ms[4] if not NULL), sets the world's
+ * leaf size to the fastest, returns it in *best and leaves that query's
+ * results in o.  The pair set does not depend on the leaf size.  (No
+ * reference counterpart; the traversal here is stackless and wave-wide, so
+ * the leaf size is its one tuning parameter.) */
+bvh_status bvh_autotune_leaf_size(bvh_object o, bvh_object other, uint32_t *best, double *ms);
+
+/* ---- batching form: a whole step as one HIP graph ----------------------------- */
+/* No reference counterpart (the reference launches per call).  Between
+ * bvh_capture_begin and bvh_capture_end the world's stream is captured:
+ * set_entity_data (device data, no global ids), bvh_refit, init_broadphase,
+ * broadphase, broadphase_patches and self_collision record their device work
+ * without synchronizing.  Run the same step once before capturing so that
+ * every buffer has its size: a call that would grow a buffer fails with
+ * BVH_E_STATE inside a capture, and any later (re)allocation in the process
+ * makes bvh_graph_launch refuse the graph.  A launched graph re-reads the
+ * element data in place; its results are read with the usual calls, which
+ * synchronize; a result list that outgrew its buffer reports
+ * BVH_E_OVERFLOW (run the step once outside the graph and capture again).
+ * sort_results cannot be captured. */
+bvh_status bvh_capture_begin(bvh_world w);
+bvh_status bvh_capture_end(bvh_world w, bvh_graph *graph);
+bvh_status bvh_graph_launch(bvh_graph graph);
+bvh_status bvh_graph_destroy(bvh_graph graph);
 
 #ifdef __cplusplus
 }

@@ -56,3 +56,22 @@ def sort_pairs(p):
         return p
     order = np.lexsort((p[:, 1], p[:, 0]))
     return p[order]
+
+
+def mix64(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+        return z ^ (z >> np.uint64(31))
+
+
+def pair_digest(pairs):
+    """Host restatement of bvh_result_digest over (k, 2) index pairs: the sum
+    and the xor of two 64-bit mixes of key = a << 32 | b."""
+    p = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    keys = (p[:, 0] << np.uint64(32)) | p[:, 1]
+    with np.errstate(over="ignore"):
+        s = int(mix64(keys).sum(dtype=np.uint64)) if len(keys) else 0
+    x = int(np.bitwise_xor.reduce(mix64(keys ^ np.uint64(0x9e3779b97f4a7c15)))) if len(keys) else 0
+    return s, x

@@ -1,0 +1,151 @@
+"""Batching form: a whole step (build + self collision of a batch of meshes)
+captured into one HIP graph and replayed on new element data in place; the
+leaf-size autotuner.  Every replay is checked against brute force on the data
+it ran on, with eager GPU work and synchronizations between replays."""
+import numpy as np
+import pytest
+
+import oracle as O
+from distbvh_amd import CollisionWorld
+from distbvh_amd import capi
+from distbvh_amd import workloads as W
+from helpers import gpu_object, pair_digest, sort_pairs, to_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a GPU"
+    return torch
+
+
+def _batch(seed):
+    a, _ = W.patch_batch(npatch=64, median=256, lo=16, hi=2048, lattice=4, seed=seed)
+    return a
+
+
+def _brute_self(a):
+    kd, _ = O.elements_poly(a)
+    return sort_pairs(O.brute_self(kd))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replays_match_brute_force(torch_cuda, precision):
+    torch = torch_cuda
+    a0 = _batch(0xC4)
+    n = a0.shape[2]
+    # same topology of the batch, new heights: the same element count
+    datas = [a0]
+    for k in range(3):
+        d = a0.copy()
+        d[:, 2, :] += 0.002 * np.sin(np.arange(n) * (0.37 + k))[None, :]
+        datas.append(d)
+    want = [_brute_self(d) for d in datas]
+    assert all(len(w) > 0 for w in want)
+    buf = to_cuda(a0)
+    with CollisionWorld(64, leaf_size=4, precision=precision) as w:
+        o = w.create_collision_object()
+
+        def step():
+            o.set_entity_data_poly(buf)
+            o.init_broadphase()
+            o.self_collision()
+
+        step()  # eager: sizes every buffer
+        assert o.result_count() == len(want[0])
+        assert o.digest() == pair_digest(want[0])
+        with w.capture() as g:
+            step()
+        try:
+            for rep in range(2):
+                for d, wp in zip(datas, want):
+                    buf.copy_(to_cuda(d))
+                    torch.cuda.synchronize()
+                    junk = torch.randn(1 << 16, device="cuda").sum().item()  # eager work between replays
+                    assert junk == junk
+                    g.launch()
+                    assert o.result_count() == len(wp)
+                    assert o.digest() == pair_digest(wp)
+            # the last replay's pairs, element for element
+            assert np.array_equal(sort_pairs(o.pairs()), want[-1])
+        finally:
+            g.close()
+
+
+def test_graph_overflow_is_reported(torch_cuda):
+    torch = torch_cuda
+    a0 = _batch(0xC5)
+    buf = to_cuda(a0)
+    with CollisionWorld(64, leaf_size=4) as w:
+        o = w.create_collision_object()
+        o.set_entity_data_poly(buf)
+        o.init_broadphase()
+        o.self_collision()
+        cap_pairs = max(1 << 20, 8 * a0.shape[2])
+        base = o.result_count()
+        with w.capture() as g:
+            o.set_entity_data_poly(buf)
+            o.init_broadphase()
+            o.self_collision()
+        try:
+            # every element a copy of element 0: all n (n - 1) / 2 pairs
+            # overlap, far more than the pair buffer holds
+            d = np.ascontiguousarray(np.broadcast_to(a0[:, :, :1], a0.shape))
+            n = a0.shape[2]
+            if n * (n - 1) // 2 <= cap_pairs:
+                pytest.skip("batch too small to overflow the pair buffer")
+            buf.copy_(to_cuda(d))
+            torch.cuda.synchronize()
+            g.launch()
+            with pytest.raises(RuntimeError, match="outgrew"):
+                o.result_count()
+            # an eager run grows the buffers and gives the full set
+            o.set_entity_data_poly(buf)
+            o.init_broadphase()
+            o.self_collision()
+            assert o.result_count() > cap_pairs
+            # the capture refers to freed buffers now and is refused
+            with pytest.raises(RuntimeError, match="reallocated"):
+                g.launch()
+        finally:
+            g.close()
+        assert base > 0
+
+
+def test_capture_refuses_buffer_growth(torch_cuda):
+    a, _ = W.wavy_sheets(32)
+    big, _ = W.wavy_sheets(64)
+    with CollisionWorld(16, leaf_size=4) as w:
+        o = w.create_collision_object()
+        o.set_entity_data_poly(to_cuda(a))
+        with pytest.raises(RuntimeError, match="grow inside a capture"):
+            with w.capture():
+                o.set_entity_data_poly(to_cuda(big))
+        # host data cannot be captured either
+        with pytest.raises(RuntimeError, match="device element data"):
+            with w.capture():
+                o.set_entity_data_poly(a)
+        # the world still works eagerly
+        o.set_entity_data_poly(to_cuda(big))
+        o.self_collision()
+        kd, _ = O.elements_poly(big)
+        assert np.array_equal(sort_pairs(o.pairs()), sort_pairs(O.brute_self(kd)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_autotune_leaf_size(torch_cuda, precision):
+    a, b = W.wavy_sheets(64)
+    ka, _ = O.elements_poly(a)
+    kb, _ = O.elements_poly(b)
+    want = sort_pairs(O.brute_pairs(ka, kb))
+    with CollisionWorld(16, leaf_size=1, precision=precision) as w:
+        oa = gpu_object(w, "poly", a, device=True)
+        ob = gpu_object(w, "poly", b, device=True)
+        best, ms = oa.autotune_leaf_size(ob)
+        assert best in (1, 2, 4, 8)
+        assert all(t > 0 for t in ms)
+        assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
+        oa.broadphase(ob)  # the chosen leaf size stays
+        assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
