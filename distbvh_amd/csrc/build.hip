@@ -348,6 +348,12 @@ __device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t o
 //    the global counters of the splits inside its chunk (no other workgroup
 //    touches them in that launch; earlier launches are ordered by the kernel
 //    boundary).  The last phase is one chunk and finds the root.
+//
+// The fp64 form stores every node (Node64: own slabs + links).  The fp32 form
+// stores one record per internal node with both children's slabs (Rec32);
+// each node writes its slabs into its half of the parent's record before it
+// counts its arrival, and the second arriver reads the sibling's half.
+
 template <bool FP32>
 struct Slab {
   using T = double;
@@ -357,56 +363,8 @@ struct Slab<true> {
   using T = float;
 };
 
-template <bool FP32>
-__device__ __forceinline__ void node_set_child(const BuildBufs& b, uint32_t parent, bool is_left, uint32_t child) {
-  if constexpr (FP32) {
-    if (is_left) b.nodes32[parent].left = child;
-    else b.nodes32[parent].right = child;
-  } else {
-    if (is_left) b.nodes[parent].left = child;
-    else b.nodes[parent].right = child;
-  }
-  if (!is_left) b.rightc[parent] = child;
-}
-
-template <bool FP32>
-__device__ __forceinline__ uint32_t node_child(const BuildBufs& b, uint32_t parent, bool left) {
-  if constexpr (FP32) return left ? b.nodes32[parent].left : b.nodes32[parent].right;
-  else return left ? b.nodes[parent].left : b.nodes[parent].right;
-}
-
-template <bool FP32>
-__device__ __forceinline__ void node_load(const BuildBufs& b, uint32_t id, typename Slab<FP32>::T* k,
-                                          uint32_t& first, uint32_t& last, uint32_t& patch) {
-  if constexpr (FP32) {
-    const Node32* nd = &b.nodes32[id];
-    load_slabs32(nd, k);
-    first = nd->first; last = nd->last; patch = nd->patch;
-  } else {
-    const Node64* nd = &b.nodes[id];
-    load_slabs(nd, k);
-    first = nd->first; last = nd->last; patch = nd->patch;
-    // keep all 14 loads in flight: one memory round trip per tree level
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-template <bool FP32>
-__device__ __forceinline__ void node_store(const BuildBufs& b, uint32_t id, const typename Slab<FP32>::T* k,
-                                           uint32_t first, uint32_t last, uint32_t patch) {
-  if constexpr (FP32) {
-    Node32* nd = &b.nodes32[id];
-    store_slabs32(nd, k);
-    nd->first = first; nd->last = last; nd->patch = patch;
-  } else {
-    Node64* nd = &b.nodes[id];
-    store_slabs(nd, k);
-    nd->first = first; nd->last = last; nd->patch = patch;
-  }
-}
-
-template <bool FP32>
-__device__ __forceinline__ void slab_union(typename Slab<FP32>::T* d, const typename Slab<FP32>::T* s) {
+template <typename T>
+__device__ __forceinline__ void slab_union(T* d, const T* s) {
 #pragma unroll
   for (int i = 0; i < kAxes; ++i) {
     d[2 * i] = (d[2 * i] < s[2 * i]) ? d[2 * i] : s[2 * i];
@@ -414,51 +372,138 @@ __device__ __forceinline__ void slab_union(typename Slab<FP32>::T* d, const type
   }
 }
 
-// Climb from a finished node until it is the first arrival at its parent, the
-// root, or a child whose parent splits on the chunk boundary [c0, c1).
-template <bool FP32, bool LDS>
+// ---- fp64 nodes
+__device__ __forceinline__ void node_set_child(const BuildBufs& b, uint32_t parent, bool is_left, uint32_t child) {
+  if (is_left) b.nodes[parent].left = child;
+  else b.nodes[parent].right = child;
+  if (!is_left) b.rightc[parent] = child;
+}
+
+__device__ __forceinline__ void node_load(const BuildBufs& b, uint32_t id, double* k, uint32_t& first,
+                                          uint32_t& last, uint32_t& patch) {
+  const Node64* nd = &b.nodes[id];
+  load_slabs(nd, k);
+  first = nd->first; last = nd->last; patch = nd->patch;
+  // keep all 14 loads in flight: one memory round trip per tree level
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+__device__ __forceinline__ void node_store(const BuildBufs& b, uint32_t id, const double* k, uint32_t first,
+                                           uint32_t last, uint32_t patch) {
+  Node64* nd = &b.nodes[id];
+  store_slabs(nd, k);
+  nd->first = first; nd->last = last; nd->patch = patch;
+}
+
+// ---- fp32 records
+__device__ __forceinline__ void rec_store_half(const BuildBufs& b, uint32_t parent, int side, const float* kf,
+                                               uint32_t child, uint32_t elem) {
+  Rec32* R = b.rec32 + parent;
+  float2* d = (float2*)R->box[side];  // 104-byte halves: 8-byte aligned
+#pragma unroll
+  for (int q = 0; q < kAxes; ++q) d[q] = make_float2(kf[2 * q], kf[2 * q + 1]);
+  R->child[side] = child;
+  R->elem[side] = elem;
+}
+
+__device__ __forceinline__ void rec_load_half(const BuildBufs& b, uint32_t parent, int side, float* kf) {
+  const float2* d = (const float2*)b.rec32[parent].box[side];
+#pragma unroll
+  for (int q = 0; q < kAxes; ++q) {
+    const float2 v = d[q];
+    kf[2 * q] = v.x;
+    kf[2 * q + 1] = v.y;
+  }
+}
+
+__device__ __forceinline__ void store_rootbox(const BuildBufs& b, const float* kf) {
+#pragma unroll
+  for (int i = 0; i < kSlabs; ++i) b.rootbox32[i] = kf[i];
+}
+
+// a one-patch node is its patch's root when a patch boundary (delta >= 128)
+// lies on both sides
+__device__ __forceinline__ bool is_patch_root(uint32_t patch, uint32_t l, uint32_t r, uint64_t n, uint32_t dl,
+                                              uint32_t dr) {
+  return patch != kMixed && (l == 0 || dl >= 128) && (r == n - 1 || dr >= 128);
+}
+
+// Global-memory climb (phases >= 2) from a finished node until it is the
+// first arrival at its parent, the root, or a child whose parent splits on
+// the chunk boundary [c0, c1).  `elem` is the node's input index if it is a
+// leaf (fp32 records), else kEnd.
+template <bool FP32>
 __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
-                                      uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t c0,
-                                      uint32_t c1, uint32_t* s_cnt, uint32_t* s_front, uint32_t* s_nfront) {
+                                      uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t elem,
+                                      uint32_t c0, uint32_t c1, uint32_t* s_front, uint32_t* s_nfront) {
+  (void)od;
   for (;;) {
     if (l == 0 && r == n - 1) {
       *b.root = node;
+      if constexpr (FP32) store_rootbox(b, kd);
       return;
     }
     const uint32_t dr = (r != n - 1) ? b.delta[r] : 0u;
     const uint32_t dl = (l != 0) ? b.delta[l - 1] : 0u;
     const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
+    if constexpr (FP32) rec_store_half(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(s_nfront, 1u);
       if (k < 2) s_front[k] = node;
       else atomicOr(b.status, kErrTree);
       return;
     }
-    node_set_child<FP32>(b, parent, is_left, node);
-    uint32_t old;
-    if constexpr (LDS)
-      old = __hip_atomic_fetch_add(&s_cnt[parent - c0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else
-      old = __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if constexpr (!FP32) node_set_child(b, parent, is_left, node);
+    const uint32_t old =
+        __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) return;  // first arriver: the sibling finishes the parent
-    const uint32_t sib = node_child<FP32>(b, parent, !is_left);
-    if (old != 1 || sib >= 2 * n - 1) {
+    if (old != 1) {
       atomicOr(b.status, kErrTree);
       return;
     }
     typename Slab<FP32>::T ks[kSlabs];
     uint32_t sfirst, slast, spatch;
-    node_load<FP32>(b, sib, ks, sfirst, slast, spatch);
-    slab_union<FP32>(kd, ks);
+    if constexpr (FP32) {
+      const uint32_t sib = b.rec32[parent].child[is_left ? 1 : 0];
+      if (sib >= 2 * n - 1) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      rec_load_half(b, parent, is_left ? 1 : 0, ks);
+      if (sib >= n - 1) {
+        sfirst = slast = sib - (uint32_t)(n - 1);
+        spatch = b.patch_of_pos[sfirst];
+      } else {
+        sfirst = b.rec32[sib].first;
+        slast = b.rec32[sib].last;
+        spatch = b.rec32[sib].patch;
+      }
+    } else {
+      const uint32_t sib = is_left ? b.nodes[parent].right : b.nodes[parent].left;
+      if (sib >= 2 * n - 1) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      node_load(b, sib, ks, sfirst, slast, spatch);
+    }
+    slab_union(kd, ks);
     if (is_left) r = slast;
     else l = sfirst;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    node_store<FP32>(b, parent, kd, l, r, mypatch);
-    b.ilast[parent] = r;
-    if (mypatch != kMixed && (l == 0 || b.delta[l - 1] >= 128) && (r == n - 1 || b.delta[r] >= 128))
+    if constexpr (FP32) {
+      Rec32* R = b.rec32 + parent;
+      R->first = l;
+      R->last = r;
+      R->patch = mypatch;
+    } else {
+      node_store(b, parent, kd, l, r, mypatch);
+      b.ilast[parent] = r;
+    }
+    if (is_patch_root(mypatch, l, r, n, l ? b.delta[l - 1] : 0u, r != n - 1 ? b.delta[r] : 0u))
       b.patch_root[mypatch] = parent;
     node = parent;
+    elem = kEnd;
   }
 }
 
@@ -557,23 +602,26 @@ __device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32
 // stores that nothing in this kernel waits for.
 template <bool FP32>
 __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
-                                            uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t c0,
-                                            uint32_t c1, ChunkLds<FP32>& L) {
+                                            uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd,
+                                            uint32_t elem, uint32_t c0, uint32_t c1, ChunkLds<FP32>& L) {
+  (void)od;
   for (;;) {
     if (l == 0 && r == n - 1) {
       *b.root = node;
+      if constexpr (FP32) store_rootbox(b, kd);
       return;
     }
     // delta[r] = L.delta[r - c0 + 1], delta[l - 1] = L.delta[l - c0]
     const bool is_left = (l == 0) || (r != n - 1 && L.delta[r - c0 + 1] < L.delta[l - c0]);
     const uint32_t parent = is_left ? r : l - 1;
+    if constexpr (FP32) rec_store_half(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(&L.nfront, 1u);
       if (k < 2) L.front[k] = node;
       else atomicOr(b.status, kErrTree);
       return;
     }
-    node_set_child<FP32>(b, parent, is_left, node);
+    if constexpr (!FP32) node_set_child(b, parent, is_left, node);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     const uint32_t old = __hip_atomic_fetch_add(&L.cnt[parent - c0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -592,18 +640,24 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
 #pragma unroll
     for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][ss];
     const uint32_t slast = L.last[ss], spatch = L.patch[ss];
-    slab_union<FP32>(kd, ks);
+    slab_union(kd, ks);
     if (is_left) r = slast;
     else l = c0 + ss;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    node_store<FP32>(b, parent, kd, l, r, mypatch);
-    b.ilast[parent] = r;
-    // a one-patch node is its patch's root when a patch boundary (delta >=
-    // 128) lies on both sides
-    if (mypatch != kMixed && (l == 0 || L.delta[l - c0] >= 128) && (r == n - 1 || L.delta[r - c0 + 1] >= 128))
+    if constexpr (FP32) {
+      Rec32* R = b.rec32 + parent;
+      R->first = l;
+      R->last = r;
+      R->patch = mypatch;
+    } else {
+      node_store(b, parent, kd, l, r, mypatch);
+      b.ilast[parent] = r;
+    }
+    if (is_patch_root(mypatch, l, r, n, l ? L.delta[l - c0] : 0u, r != n - 1 ? L.delta[r - c0 + 1] : 0u))
       b.patch_root[mypatch] = parent;
     publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
     node = parent;
+    elem = kEnd;
   }
 }
 
@@ -659,15 +713,13 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
         err = d > err ? d : err;
       }
       atomicMax(&L.err, __float_as_uint(err));
+      // the query slabs of this leaf, SoA in sorted order
+#pragma unroll
+      for (int i = 0; i < kSlabs; ++i) b.leafbox32[(uint64_t)i * n + p] = ks[i];
     } else {
 #pragma unroll
       for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
-    }
-    node_store<FP32>(b, leaf, ks, p, p, pk);
-    if constexpr (FP32) {
-      b.nodes32[leaf].left = e;
-      b.nodes32[leaf].right = kEnd;
-    } else {
+      node_store(b, leaf, ks, p, p, pk);
       b.nodes[leaf].left = e;
       b.nodes[leaf].right = kEnd;
     }
@@ -675,8 +727,12 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   }
   __syncthreads();  // counters, deltas and leaf slots ready
   if (act) {
-    if (n == 1) *b.root = leaf;
-    else climb_chunk<FP32>(b, n, od, leaf, p, p, pk, ks, c0, c1, L);
+    if (n == 1) {
+      *b.root = leaf;
+      if constexpr (FP32) store_rootbox(b, ks);
+    } else {
+      climb_chunk<FP32>(b, n, od, leaf, p, p, pk, ks, e, c0, c1, L);
+    }
   }
   __syncthreads();
   if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);
@@ -706,9 +762,24 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
       atomicOr(b.status, kErrTree);
     } else if (node != kEnd) {
       typename Slab<FP32>::T kd[kSlabs];
-      uint32_t l, r, pk;
-      node_load<FP32>(b, node, kd, l, r, pk);
-      climb<FP32, false>(b, n, od, node, l, r, pk, kd, c0, c1, nullptr, s_front, &s_nfront);
+      uint32_t l, r, pk, elem = kEnd;
+      if constexpr (FP32) {
+        // a handed-over node's slabs are in its half of the parent's record
+        if (node >= n - 1) {
+          l = r = node - (uint32_t)(n - 1);
+          pk = b.patch_of_pos[l];
+          elem = b.perm[l];
+        } else {
+          l = b.rec32[node].first;
+          r = b.rec32[node].last;
+          pk = b.rec32[node].patch;
+        }
+        const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
+        rec_load_half(b, is_left ? r : l - 1, is_left ? 0 : 1, kd);
+      } else {
+        node_load(b, node, kd, l, r, pk);
+      }
+      climb<FP32>(b, n, od, node, l, r, pk, kd, elem, c0, c1, s_front, &s_nfront);
     }
   }
   __syncthreads();
@@ -716,8 +787,7 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
 }
 
 // rope(node [l, r]) = right child of internal node r (none when r = n-1): the
-// next subtree in depth-first order once this one is done.
-template <bool FP32>
+// next subtree in depth-first order once this one is done (fp64 nodes).
 __global__ __launch_bounds__(256) void k_ropes(BuildBufs b, uint64_t n) {
   const uint64_t total = 2 * n - 1;
   for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
@@ -902,8 +972,8 @@ hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s)
   if (n == 0) return hipSuccess;
   uint64_t rb = (2 * n - 1 + 255) / 256;
   if (rb > 4096) rb = 4096;
-  if (fp32) hipLaunchKernelGGL(k_ropes<true>, dim3((unsigned)rb), dim3(256), 0, s, b, n);
-  else hipLaunchKernelGGL(k_ropes<false>, dim3((unsigned)rb), dim3(256), 0, s, b, n);
+  if (fp32) return hipSuccess;  // the fp32 records are walked with a stack
+  hipLaunchKernelGGL(k_ropes, dim3((unsigned)rb), dim3(256), 0, s, b, n);
   return hipGetLastError();
 }
 

@@ -158,7 +158,7 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, nodes, nodes32, status, rightc, ilast, front_a, front_b, ropes,
+      patch_of_input, counters, patch_root, root, nodes, rec32, leafbox32, rootbox32, status, rightc, ilast, front_a, front_b, ropes,
       patch_k64;
   BuildBufs bb{};
   int bits = 63;
@@ -185,7 +185,9 @@ struct bvh_object_s {
 static TreeView view_of(const bvh_object_s* o) {
   TreeView v{};
   v.nodes = o->nodes.as<Node64>();
-  v.nodes32 = o->nodes32.as<Node32>();
+  v.rec32 = o->rec32.as<Rec32>();
+  v.leafbox32 = o->leafbox32.as<float>();
+  v.rootbox32 = o->rootbox32.as<float>();
   v.root = o->root.as<uint32_t>();
   v.patch_root = o->patch_root.as<uint32_t>();
   v.patch_of_input = o->patch_of_input.as<uint32_t>();
@@ -245,7 +247,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   if (o->fp32) {
-    BVH_TRY(o->nodes32.ensure((2 * n1 - 1) * sizeof(Node32)));
+    BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
+    BVH_TRY(o->leafbox32.ensure((uint64_t)kSlabs * n1 * sizeof(float)));
+    BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(float)));
     BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   } else {
     BVH_TRY(o->nodes.ensure((2 * n1 - 1) * sizeof(Node64)));
@@ -275,7 +279,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.patch_root = o->patch_root.as<uint32_t>();
   b.root = o->root.as<uint32_t>();
   b.nodes = o->fp32 ? nullptr : o->nodes.as<Node64>();
-  b.nodes32 = o->fp32 ? o->nodes32.as<Node32>() : nullptr;
+  b.rec32 = o->fp32 ? o->rec32.as<Rec32>() : nullptr;
+  b.leafbox32 = o->fp32 ? o->leafbox32.as<float>() : nullptr;
+  b.rootbox32 = o->fp32 ? o->rootbox32.as<float>() : nullptr;
   b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
   b.rightc = o->rightc.as<uint32_t>();

@@ -180,11 +180,18 @@ static_assert(sizeof(Node64) == 256, "node layout");
 
 // fp32 node for the outward-rounded precision form: 104 B slabs + links in
 // one 128-byte line.
-struct alignas(128) Node32 {
-  float s[kSlabs];  // 104 B
-  uint32_t left, right, first, last, rope, patch;
+// fp32 form: one record per internal node holding both children's slabs,
+// so a traversal step tests two nodes with one memory round trip.  A node's
+// own slabs live in its parent's record (the root's in a separate array).
+struct alignas(256) Rec32 {
+  float box[2][kSlabs];  // left, right child (208 B)
+  uint32_t child[2];     // child node ids (leaf p = n-1+p)
+  uint32_t elem[2];      // input index of a leaf child
+  uint32_t first, last;  // this node's range of sorted positions
+  uint32_t patch;        // patch id, or kMixed
+  uint32_t pad;
 };
-static_assert(sizeof(Node32) == 128, "node32 layout");
+static_assert(sizeof(Rec32) == 256, "record layout");
 
 
 // Order-preserving map of a double to an unsigned 64-bit integer (x < y ==>
@@ -299,23 +306,6 @@ __device__ __forceinline__ void to_fp32_outward(const double* kd, const double* 
   }
 }
 
-__device__ __forceinline__ void store_slabs32(Node32* nd, const float* kf) {
-  float4* d = (float4*)nd->s;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) d[q] = make_float4(kf[4 * q], kf[4 * q + 1], kf[4 * q + 2], kf[4 * q + 3]);
-  nd->s[24] = kf[24];
-  nd->s[25] = kf[25];
-}
-__device__ __forceinline__ void load_slabs32(const Node32* nd, float* kf) {
-  const float4* d = (const float4*)nd->s;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    float4 t = d[q];
-    kf[4 * q] = t.x; kf[4 * q + 1] = t.y; kf[4 * q + 2] = t.z; kf[4 * q + 3] = t.w;
-  }
-  kf[24] = nd->s[24];
-  kf[25] = nd->s[25];
-}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }

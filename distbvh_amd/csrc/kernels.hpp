@@ -74,7 +74,9 @@ struct BuildBufs {
   uint32_t* patch_root;    // od
   uint32_t* root;          // 1
   Node64* nodes;           // 2n-1
-  Node32* nodes32;         // 2n-1 (fp32 form) or null
+  Rec32* rec32;            // fp32 form: n-1 child-pair records
+  float* leafbox32;        // fp32 form: leaf slabs, SoA [26][n] in sorted order
+  float* rootbox32;        // fp32 form: the root's slabs (26)
   double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
   uint32_t* rightc;        // n-1: right child of each internal node
@@ -109,7 +111,9 @@ uint64_t prep_blocks(uint64_t n);
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {
   const Node64* nodes;
-  const Node32* nodes32;
+  const Rec32* rec32;
+  const float* leafbox32;
+  const float* rootbox32;
   const uint32_t* root;
   const uint32_t* patch_root;
   const uint32_t* patch_of_input;

@@ -184,7 +184,13 @@ __global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf
 // slab are emitted; the rest go to a re-check list whose pairs are tested on
 // fp64 k-DOPs recomputed from element data (k_recheck).  The pairs are
 // exactly those of the fp64 form.
-typedef __attribute__((address_space(4))) const Node32 CNode32;
+//
+// The tree is stored as child-pair records: visiting an internal node loads
+// both children's slabs with one round of scalar loads, tests both, handles
+// leaf children at once and pushes the internal children some lane overlaps
+// on a per-wave LDS stack (left on top: depth-first order).
+typedef __attribute__((address_space(4))) const Rec32 CRec32;
+typedef __attribute__((address_space(4))) const float CFloat;
 
 // ordered-integer image of a float: monotone, consecutive floats differ by 1,
 // ord(-0) = ord(+0) = 0
@@ -193,18 +199,23 @@ __device__ __forceinline__ int32_t ord_f32(float f) {
   return (u >> 31) ? -(int32_t)(u & 0x7fffffffu) : (int32_t)u;
 }
 constexpr int32_t kInnerUlps = 6;
+// stack entries per wave: above the height of any radix tree over the
+// augmented keys (14 patch bits + 63 code bits + 30 position bits)
+constexpr uint32_t kStack = 128;
 
-__device__ __forceinline__ bool overlap_cnode32(const float* q, CNode32* nd) {
+__device__ __forceinline__ bool overlap_c32(const float* q, CFloat* box) {
   bool o = true;
 #pragma unroll
-  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < nd->s[2 * i + 1]) & (nd->s[2 * i] < q[2 * i + 1]);
+  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < box[2 * i + 1]) & (box[2 * i] < q[2 * i + 1]);
   return o;
 }
 
 template <bool SELF>
-__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int leaf_size, PairOut out) {
+__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
   uint64_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t* stk = s_stack[threadIdx.x / kWave];
   uint32_t fill = 0;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
@@ -213,10 +224,9 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int le
   int32_t qin[kSlabs];  // ord(qf_lo) + kInnerUlps, ord(qf_hi) - kInnerUlps
   uint32_t qa = 0;
   if (valid) {
-    // q and t share the world's fp32 frame: the query slabs are q's leaf slabs
-    const Node32* qn = &q.nodes32[q.n - 1 + p];
-    load_slabs32(qn, qf);
-    qa = qn->left;
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) qf[i] = q.leafbox32[(uint64_t)i * q.n + p];
+    qa = q.perm[p];
   } else {
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {
@@ -231,24 +241,22 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int le
     qin[2 * a] = (int32_t)((uint32_t)ord_f32(qf[2 * a]) + (uint32_t)kInnerUlps);
     qin[2 * a + 1] = (int32_t)((uint32_t)ord_f32(qf[2 * a + 1]) - (uint32_t)kInnerUlps);
   }
-  const uint32_t tleaf0 = (uint32_t)(t.n - 1);
-  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
-  CNode32* nodes = (CNode32*)t.nodes32;
-  const __attribute__((address_space(4))) uint32_t* ropes = (const __attribute__((address_space(4))) uint32_t*)t.ropes;
-  uint32_t node = uniform(*t.root);
+  const uint32_t n = (uint32_t)t.n;
+  const uint32_t tleaf0 = n - 1;
+  CRec32* recs = (CRec32*)t.rec32;
   uint32_t steps = 0;
   uint32_t ncand = 0;
   // fp32 candidates at one (wave-uniform) leaf: emit what the inner bound
   // proves, queue the rest for the fp64 re-check
-  auto leaf_check = [&](CNode32* ln, bool h) {
+  auto leaf_check = [&](CFloat* box, uint32_t elem, bool h) {
     const uint64_t m = __ballot(h);
     if (m == 0) return;
     ncand += (uint32_t)__popcll(m);
     bool sure = true;
 #pragma unroll
     for (int a = 0; a < kAxes; ++a)
-      sure = sure && (qin[2 * a] < ord_f32(ln->s[2 * a + 1])) && (ord_f32(ln->s[2 * a]) < qin[2 * a + 1]);
-    const uint64_t key = pair_key(qa, ln->left, SELF);
+      sure = sure && (qin[2 * a] < ord_f32(box[2 * a + 1])) && (ord_f32(box[2 * a]) < qin[2 * a + 1]);
+    const uint64_t key = pair_key(qa, elem, SELF);
     wave_emit(buf, fill, out, h && sure, key);
     const bool unsure = h && !sure;
     const uint64_t mu = __ballot(unsure);
@@ -264,33 +272,48 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int le
       }
     }
   };
-  while (node != kEnd) {
-    if (node >= nnodes || ++steps > nnodes) {
+  uint32_t sp = 0;
+  {
+    CFloat* rb = (CFloat*)t.rootbox32;
+    bool h = overlap_c32(qf, rb);
+    if (SELF) h = h && (uint64_t)(n - 1) > p;  // the root's last position
+    if (n == 1) {
+      if (!SELF) leaf_check(rb, uniform(t.perm[0]), h);
+    } else if (__ballot(h) != 0) {
+      stk[0] = uniform(*t.root);
+      sp = 1;
+    }
+  }
+  while (sp > 0) {
+    --sp;
+    const uint32_t x = uniform(stk[sp]);
+    if (x >= tleaf0 || ++steps > n) {
       if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
-    CNode32* nd = nodes + node;
-    const uint32_t first = nd->first, last = nd->last, rope = ropes[node];
-    bool hit = overlap_cnode32(qf, nd);
-    if (SELF) hit = hit && last > (uint32_t)p;
-    if (__ballot(hit) == 0) {
-      node = rope;
-      continue;
+    CRec32* R = recs + x;
+    const uint32_t cl = R->child[0], cr = R->child[1];
+    bool hl = overlap_c32(qf, R->box[0]);
+    bool hr = overlap_c32(qf, R->box[1]);
+    if (SELF) {
+      hl = hl && (uint64_t)x > p;        // the left child ends at position x
+      hr = hr && (uint64_t)R->last > p;
     }
-    if (node >= tleaf0) {
-      leaf_check(nd, hit);
-      node = rope;
-    } else if (last - first < (uint32_t)leaf_size) {
-      for (uint32_t pos = first; pos <= last; ++pos) {
-        CNode32* ln = nodes + (tleaf0 + pos);
-        bool h = hit && overlap_cnode32(qf, ln);
-        if (SELF) h = h && pos > (uint32_t)p;
-        leaf_check(ln, h);
-      }
-      node = rope;
-    } else {
-      node = nd->left;
+    bool push_l = false, push_r = false;
+    if (__ballot(hl) != 0) {
+      if (cl >= tleaf0) leaf_check(R->box[0], R->elem[0], SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
+      else push_l = true;
     }
+    if (__ballot(hr) != 0) {
+      if (cr >= tleaf0) leaf_check(R->box[1], R->elem[1], SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
+      else push_r = true;
+    }
+    if (sp + (push_l ? 1u : 0u) + (push_r ? 1u : 0u) > kStack) {
+      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    if (push_r) stk[sp++] = cr;
+    if (push_l) stk[sp++] = cl;
   }
   wave_flush(buf, fill, out);
   if (lane_id() == 0) {
@@ -430,10 +453,11 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   if (q.n == 0 || t.n == 0) return hipSuccess;
   const unsigned blocks = (unsigned)((q.n + 255) / 256);
   if (fp32) {
-    if (!q.nodes32 || !t.nodes32 || !t.origin || q.origin != t.origin) return hipErrorInvalidValue;
+    if (!q.leafbox32 || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin) return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
-    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
-    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+    // the fp32 walk tests leaves from their parents' records: no leaf clusters
+    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(256), 0, s, q, t, out);
+    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(256), 0, s, q, t, out);
     // in self mode both keys index the one object (q == t)
     hipLaunchKernelGGL(k_recheck, dim3(512), dim3(256), 0, s, q, t, out);
     return hipGetLastError();
