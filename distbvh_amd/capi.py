@@ -36,6 +36,11 @@ class Record(C.Structure):
                 ("element_q", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("leaf_size", C.c_uint32), ("block_size", C.c_uint32), ("xcd_remap", C.c_uint32),
+                ("tried", C.c_uint32), ("ms", C.c_double), ("tried_ms", C.c_double * 24)]
+
+
 class PatchInfo(C.Structure):
     _fields_ = [("global_id", C.c_uint64), ("size", C.c_uint64), ("kdop", C.c_double * 26),
                 ("centroid", C.c_double * 3)]
@@ -68,7 +73,9 @@ SIGNATURES = {
     "bvh_set_entity_data_sphere": [_vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_set_entity_data_snapshot": [_vp, _vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_refit": [_vp, _vp, _u32],
-    "bvh_autotune_leaf_size": [_vp, _vp, _vp, _vp],
+    "bvh_object_device_bytes": [_vp, C.POINTER(_u64)],
+    "bvh_object_root_kdop": [_vp, _vp],
+    "bvh_autotune": [_vp, _vp, _vp],
     "bvh_capture_begin": [_vp],
     "bvh_capture_end": [_vp, _vp],
     "bvh_graph_launch": [_vp],

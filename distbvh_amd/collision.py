@@ -202,12 +202,23 @@ class CollisionObject:
         self._keep = (keep,) + tuple(self._keep[1:])
         call("bvh_refit", self.handle, p, mem)
 
-    def autotune_leaf_size(self, other=None):
-        """bvh_autotune_leaf_size: (best leaf size, ms for leaf 1, 2, 4, 8)."""
-        best = C.c_uint32()
-        ms = (C.c_double * 4)()
-        call("bvh_autotune_leaf_size", self.handle, (other or self).handle, C.byref(best), ms)
-        return best.value, list(ms)
+    def device_bytes(self):
+        v = C.c_uint64()
+        call("bvh_object_device_bytes", self.handle, C.byref(v))
+        return v.value
+
+    def root_kdop(self):
+        k = np.zeros(26)
+        call("bvh_object_root_kdop", self.handle, C.c_void_p(k.ctypes.data))
+        return k
+
+    def autotune(self, other=None):
+        """bvh_autotune: the fastest leaf size / block size / XCD remap for
+        the query self x other (self collision when other is None)."""
+        t = capi.Tuning()
+        call("bvh_autotune", self.handle, (other or self).handle, C.byref(t))
+        return {"leaf_size": t.leaf_size, "block_size": t.block_size, "xcd_remap": bool(t.xcd_remap),
+                "ms": t.ms, "tried_ms": list(t.tried_ms[:t.tried])}
 
     def init_broadphase(self):
         call("bvh_init_broadphase", self.handle)

@@ -14,7 +14,6 @@
 //  * patches (collision_object.cpp:134-141) and the tree
 //      -> k_delta + k_tree (Apetrei-style bottom-up build: each leaf climbs,
 //         the second child to reach a parent merges both children's slabs)
-//      -> k_ropes (escape links for stackless traversal)
 //
 // The tree is the binary radix tree of the augmented keys
 // (patch id, Morton code, sorted position): every patch is a subtree, the
@@ -349,19 +348,22 @@ __device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t o
 //    touches them in that launch; earlier launches are ordered by the kernel
 //    boundary).  The last phase is one chunk and finds the root.
 //
-// The fp64 form stores every node (Node64: own slabs + links).  The fp32 form
-// stores one record per internal node with both children's slabs (Rec32);
-// each node writes its slabs into its half of the parent's record before it
-// counts its arrival, and the second arriver reads the sibling's half.
+// Both forms store one record per internal node with both children's slabs
+// (Rec32 / Rec64).  Each node writes its slabs into its half of the parent's
+// record before it counts its arrival, and the second arriver reads the
+// sibling's half; the root's slabs go to a separate array, and every leaf's
+// slabs also to the SoA leaf array the queries read.
 
 template <bool FP32>
-struct Slab {
-  using T = double;
-};
-template <>
-struct Slab<true> {
-  using T = float;
-};
+using SlabT = typename Form<FP32>::T;
+template <bool FP32>
+using RecT = typename Form<FP32>::Rec;
+
+template <bool FP32>
+__device__ __forceinline__ RecT<FP32>* recs_of(const BuildBufs& b) {
+  if constexpr (FP32) return b.rec32;
+  else return b.rec64;
+}
 
 template <typename T>
 __device__ __forceinline__ void slab_union(T* d, const T* s) {
@@ -372,53 +374,54 @@ __device__ __forceinline__ void slab_union(T* d, const T* s) {
   }
 }
 
-// ---- fp64 nodes
-__device__ __forceinline__ void node_set_child(const BuildBufs& b, uint32_t parent, bool is_left, uint32_t child) {
-  if (is_left) b.nodes[parent].left = child;
-  else b.nodes[parent].right = child;
-  if (!is_left) b.rightc[parent] = child;
-}
-
-__device__ __forceinline__ void node_load(const BuildBufs& b, uint32_t id, double* k, uint32_t& first,
-                                          uint32_t& last, uint32_t& patch) {
-  const Node64* nd = &b.nodes[id];
-  load_slabs(nd, k);
-  first = nd->first; last = nd->last; patch = nd->patch;
-  // keep all 14 loads in flight: one memory round trip per tree level
-  __builtin_amdgcn_sched_barrier(0);
-}
-
-__device__ __forceinline__ void node_store(const BuildBufs& b, uint32_t id, const double* k, uint32_t first,
-                                           uint32_t last, uint32_t patch) {
-  Node64* nd = &b.nodes[id];
-  store_slabs(nd, k);
-  nd->first = first; nd->last = last; nd->patch = patch;
-}
-
-// ---- fp32 records
-__device__ __forceinline__ void rec_store_half(const BuildBufs& b, uint32_t parent, int side, const float* kf,
+template <bool FP32>
+__device__ __forceinline__ void rec_store_half(const BuildBufs& b, uint32_t parent, int side, const SlabT<FP32>* k,
                                                uint32_t child, uint32_t elem) {
-  Rec32* R = b.rec32 + parent;
-  float2* d = (float2*)R->box[side];  // 104-byte halves: 8-byte aligned
+  RecT<FP32>* R = recs_of<FP32>(b) + parent;
+  if constexpr (FP32) {
+    float2* d = (float2*)R->box[side];  // 104-byte halves: 8-byte aligned
 #pragma unroll
-  for (int q = 0; q < kAxes; ++q) d[q] = make_float2(kf[2 * q], kf[2 * q + 1]);
+    for (int q = 0; q < kAxes; ++q) d[q] = make_float2(k[2 * q], k[2 * q + 1]);
+  } else {
+    double2* d = (double2*)R->box[side];  // 208-byte halves: 16-byte aligned
+#pragma unroll
+    for (int q = 0; q < kAxes; ++q) d[q] = make_double2(k[2 * q], k[2 * q + 1]);
+  }
   R->child[side] = child;
   R->elem[side] = elem;
 }
 
-__device__ __forceinline__ void rec_load_half(const BuildBufs& b, uint32_t parent, int side, float* kf) {
-  const float2* d = (const float2*)b.rec32[parent].box[side];
+template <bool FP32>
+__device__ __forceinline__ void rec_load_half(const BuildBufs& b, uint32_t parent, int side, SlabT<FP32>* k) {
+  const RecT<FP32>* R = recs_of<FP32>(b) + parent;
+  if constexpr (FP32) {
+    const float2* d = (const float2*)R->box[side];
 #pragma unroll
-  for (int q = 0; q < kAxes; ++q) {
-    const float2 v = d[q];
-    kf[2 * q] = v.x;
-    kf[2 * q + 1] = v.y;
+    for (int q = 0; q < kAxes; ++q) {
+      const float2 v = d[q];
+      k[2 * q] = v.x;
+      k[2 * q + 1] = v.y;
+    }
+  } else {
+    const double2* d = (const double2*)R->box[side];
+#pragma unroll
+    for (int q = 0; q < kAxes; ++q) {
+      const double2 v = d[q];
+      k[2 * q] = v.x;
+      k[2 * q + 1] = v.y;
+    }
+    // keep all loads in flight: one memory round trip per tree level
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-__device__ __forceinline__ void store_rootbox(const BuildBufs& b, const float* kf) {
+template <bool FP32>
+__device__ __forceinline__ void store_rootbox(const BuildBufs& b, const SlabT<FP32>* k) {
 #pragma unroll
-  for (int i = 0; i < kSlabs; ++i) b.rootbox32[i] = kf[i];
+  for (int i = 0; i < kSlabs; ++i) {
+    if constexpr (FP32) b.rootbox32[i] = k[i];
+    else b.rootbox64[i] = k[i];
+  }
 }
 
 // a one-patch node is its patch's root when a patch boundary (delta >= 128)
@@ -431,75 +434,55 @@ __device__ __forceinline__ bool is_patch_root(uint32_t patch, uint32_t l, uint32
 // Global-memory climb (phases >= 2) from a finished node until it is the
 // first arrival at its parent, the root, or a child whose parent splits on
 // the chunk boundary [c0, c1).  `elem` is the node's input index if it is a
-// leaf (fp32 records), else kEnd.
+// leaf, else kEnd.
 template <bool FP32>
-__device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
-                                      uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd, uint32_t elem,
-                                      uint32_t c0, uint32_t c1, uint32_t* s_front, uint32_t* s_nfront) {
-  (void)od;
+__device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t node, uint32_t l, uint32_t r,
+                                      uint32_t mypatch, SlabT<FP32>* kd, uint32_t elem, uint32_t c0, uint32_t c1,
+                                      uint32_t* s_front, uint32_t* s_nfront) {
+  RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
     if (l == 0 && r == n - 1) {
       *b.root = node;
-      if constexpr (FP32) store_rootbox(b, kd);
+      store_rootbox<FP32>(b, kd);
       return;
     }
     const uint32_t dr = (r != n - 1) ? b.delta[r] : 0u;
     const uint32_t dl = (l != 0) ? b.delta[l - 1] : 0u;
     const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
-    if constexpr (FP32) rec_store_half(b, parent, is_left ? 0 : 1, kd, node, elem);
+    rec_store_half<FP32>(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(s_nfront, 1u);
       if (k < 2) s_front[k] = node;
       else atomicOr(b.status, kErrTree);
       return;
     }
-    if constexpr (!FP32) node_set_child(b, parent, is_left, node);
     const uint32_t old =
         __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) return;  // first arriver: the sibling finishes the parent
-    if (old != 1) {
+    const uint32_t sib = recs[parent].child[is_left ? 1 : 0];
+    if (old != 1 || sib >= 2 * n - 1) {
       atomicOr(b.status, kErrTree);
       return;
     }
-    typename Slab<FP32>::T ks[kSlabs];
+    SlabT<FP32> ks[kSlabs];
+    rec_load_half<FP32>(b, parent, is_left ? 1 : 0, ks);
     uint32_t sfirst, slast, spatch;
-    if constexpr (FP32) {
-      const uint32_t sib = b.rec32[parent].child[is_left ? 1 : 0];
-      if (sib >= 2 * n - 1) {
-        atomicOr(b.status, kErrTree);
-        return;
-      }
-      rec_load_half(b, parent, is_left ? 1 : 0, ks);
-      if (sib >= n - 1) {
-        sfirst = slast = sib - (uint32_t)(n - 1);
-        spatch = b.patch_of_pos[sfirst];
-      } else {
-        sfirst = b.rec32[sib].first;
-        slast = b.rec32[sib].last;
-        spatch = b.rec32[sib].patch;
-      }
+    if (sib >= n - 1) {
+      sfirst = slast = sib - (uint32_t)(n - 1);
+      spatch = b.patch_of_pos[sfirst];
     } else {
-      const uint32_t sib = is_left ? b.nodes[parent].right : b.nodes[parent].left;
-      if (sib >= 2 * n - 1) {
-        atomicOr(b.status, kErrTree);
-        return;
-      }
-      node_load(b, sib, ks, sfirst, slast, spatch);
+      sfirst = recs[sib].first;
+      slast = recs[sib].last;
+      spatch = recs[sib].patch;
     }
     slab_union(kd, ks);
     if (is_left) r = slast;
     else l = sfirst;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    if constexpr (FP32) {
-      Rec32* R = b.rec32 + parent;
-      R->first = l;
-      R->last = r;
-      R->patch = mypatch;
-    } else {
-      node_store(b, parent, kd, l, r, mypatch);
-      b.ilast[parent] = r;
-    }
+    recs[parent].first = l;
+    recs[parent].last = r;
+    recs[parent].patch = mypatch;
     if (is_patch_root(mypatch, l, r, n, l ? b.delta[l - 1] : 0u, r != n - 1 ? b.delta[r] : 0u))
       b.patch_root[mypatch] = parent;
     node = parent;
@@ -517,8 +500,8 @@ __device__ __forceinline__ void empty_kdop(double* kd) {
   }
 }
 
-// fp32 form: exact (fp64) patch slabs, the union of the patch's element
-// k-DOPs, kept beside the fp32 tree for the patch broadphase.  Called by every
+// Exact (fp64) patch slabs, the union of the patch's element k-DOPs, for the
+// patch broadphase and local_patches (in both forms).  Called by every
 // lane of the wave; lanes without an element pass pk = kEnd and the empty
 // k-DOP.  A wave whose elements all lie in one patch reduces in registers and
 // lane 0 issues the 26 atomics; otherwise each lane issues its own.
@@ -572,7 +555,7 @@ __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
 // a slot is only overwritten by the thread that has just read it.
 template <bool FP32>
 struct ChunkLds {
-  typename Slab<FP32>::T slab[kSlabs][kTreeChunk];  // SoA: conflict-free stores
+  SlabT<FP32> slab[kSlabs][kTreeChunk];  // SoA: conflict-free stores
   uint32_t cnt[kTreeChunk];      // arrivals at internal node c0 + j
   uint32_t id[kTreeChunk];
   uint32_t last[kTreeChunk];
@@ -585,9 +568,8 @@ struct ChunkLds {
 };
 
 template <bool FP32>
-__device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32_t endslot,
-                                        const typename Slab<FP32>::T* kd, uint32_t id, uint32_t last,
-                                        uint32_t patch) {
+__device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32_t endslot, const SlabT<FP32>* kd,
+                                        uint32_t id, uint32_t last, uint32_t patch) {
 #pragma unroll
   for (int i = 0; i < kSlabs; ++i) L.slab[i][slot] = kd[i];
   L.id[slot] = id;
@@ -598,30 +580,29 @@ __device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32
 
 // Phase-1 climb inside the chunk [c0, c1): the sibling's slabs, id and range
 // come from LDS and arrivals are LDS atomics whose fences order LDS only, so a
-// level costs LDS latency; node links and slabs go to global memory with
-// stores that nothing in this kernel waits for.
+// level costs LDS latency; record halves go to global memory with stores that
+// nothing in this kernel waits for.
 template <bool FP32>
-__device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint64_t od, uint32_t node, uint32_t l,
-                                            uint32_t r, uint32_t mypatch, typename Slab<FP32>::T* kd,
-                                            uint32_t elem, uint32_t c0, uint32_t c1, ChunkLds<FP32>& L) {
-  (void)od;
+__device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint32_t node, uint32_t l, uint32_t r,
+                                            uint32_t mypatch, SlabT<FP32>* kd, uint32_t elem, uint32_t c0,
+                                            uint32_t c1, ChunkLds<FP32>& L) {
+  RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
     if (l == 0 && r == n - 1) {
       *b.root = node;
-      if constexpr (FP32) store_rootbox(b, kd);
+      store_rootbox<FP32>(b, kd);
       return;
     }
     // delta[r] = L.delta[r - c0 + 1], delta[l - 1] = L.delta[l - c0]
     const bool is_left = (l == 0) || (r != n - 1 && L.delta[r - c0 + 1] < L.delta[l - c0]);
     const uint32_t parent = is_left ? r : l - 1;
-    if constexpr (FP32) rec_store_half(b, parent, is_left ? 0 : 1, kd, node, elem);
+    rec_store_half<FP32>(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(&L.nfront, 1u);
       if (k < 2) L.front[k] = node;
       else atomicOr(b.status, kErrTree);
       return;
     }
-    if constexpr (!FP32) node_set_child(b, parent, is_left, node);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     const uint32_t old = __hip_atomic_fetch_add(&L.cnt[parent - c0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -636,7 +617,7 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
       atomicOr(b.status, kErrTree);
       return;
     }
-    typename Slab<FP32>::T ks[kSlabs];
+    SlabT<FP32> ks[kSlabs];
 #pragma unroll
     for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][ss];
     const uint32_t slast = L.last[ss], spatch = L.patch[ss];
@@ -644,15 +625,9 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     if (is_left) r = slast;
     else l = c0 + ss;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    if constexpr (FP32) {
-      Rec32* R = b.rec32 + parent;
-      R->first = l;
-      R->last = r;
-      R->patch = mypatch;
-    } else {
-      node_store(b, parent, kd, l, r, mypatch);
-      b.ilast[parent] = r;
-    }
+    recs[parent].first = l;
+    recs[parent].last = r;
+    recs[parent].patch = mypatch;
     if (is_patch_root(mypatch, l, r, n, l ? L.delta[l - c0] : 0u, r != n - 1 ? L.delta[r - c0 + 1] : 0u))
       b.patch_root[mypatch] = parent;
     publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
@@ -692,8 +667,9 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   } else {
     empty_kdop(kd);
   }
-  if constexpr (FP32) patch_union64(b.patch_k64, pk, kd);
-  typename Slab<FP32>::T ks[kSlabs];
+  // exact patch slabs for the patch broadphase
+  patch_union64(b.patch_k64, pk, kd);
+  SlabT<FP32> ks[kSlabs];
   const uint32_t leaf = (uint32_t)(n - 1 + p);
   if (act) {
     b.patch_of_input[e] = pk;
@@ -713,15 +689,14 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
         err = d > err ? d : err;
       }
       atomicMax(&L.err, __float_as_uint(err));
-      // the query slabs of this leaf, SoA in sorted order
 #pragma unroll
       for (int i = 0; i < kSlabs; ++i) b.leafbox32[(uint64_t)i * n + p] = ks[i];
     } else {
 #pragma unroll
-      for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
-      node_store(b, leaf, ks, p, p, pk);
-      b.nodes[leaf].left = e;
-      b.nodes[leaf].right = kEnd;
+      for (int i = 0; i < kSlabs; ++i) {
+        ks[i] = kd[i];
+        b.leafbox64[(uint64_t)i * n + p] = kd[i];
+      }
     }
     publish<FP32>(L, t, t, ks, leaf, p, pk);
   }
@@ -729,9 +704,9 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   if (act) {
     if (n == 1) {
       *b.root = leaf;
-      if constexpr (FP32) store_rootbox(b, ks);
+      store_rootbox<FP32>(b, ks);
     } else {
-      climb_chunk<FP32>(b, n, od, leaf, p, p, pk, ks, e, c0, c1, L);
+      climb_chunk<FP32>(b, n, leaf, p, p, pk, ks, e, c0, c1, L);
     }
   }
   __syncthreads();
@@ -742,10 +717,11 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
 }
 
 // Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one
-// thread per handed-over node (two per sub-chunk).
+// thread per handed-over node (two per sub-chunk).  A handed-over node's
+// slabs are in its half of its parent's record.
 template <bool FP32>
-__global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t od, uint64_t chunk,
-                                                            uint64_t nsub, const uint32_t* __restrict__ front_in,
+__global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t chunk, uint64_t nsub,
+                                                            const uint32_t* __restrict__ front_in,
                                                             uint32_t* __restrict__ front_out) {
   __shared__ uint32_t s_front[2];
   __shared__ uint32_t s_nfront;
@@ -761,47 +737,33 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
     if (node != kEnd && node >= 2 * n - 1) {
       atomicOr(b.status, kErrTree);
     } else if (node != kEnd) {
-      typename Slab<FP32>::T kd[kSlabs];
+      const RecT<FP32>* recs = recs_of<FP32>(b);
+      SlabT<FP32> kd[kSlabs];
       uint32_t l, r, pk, elem = kEnd;
-      if constexpr (FP32) {
-        // a handed-over node's slabs are in its half of the parent's record
-        if (node >= n - 1) {
-          l = r = node - (uint32_t)(n - 1);
-          pk = b.patch_of_pos[l];
-          elem = b.perm[l];
-        } else {
-          l = b.rec32[node].first;
-          r = b.rec32[node].last;
-          pk = b.rec32[node].patch;
-        }
-        const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
-        rec_load_half(b, is_left ? r : l - 1, is_left ? 0 : 1, kd);
+      if (node >= n - 1) {
+        l = r = node - (uint32_t)(n - 1);
+        pk = b.patch_of_pos[l];
+        elem = b.perm[l];
       } else {
-        node_load(b, node, kd, l, r, pk);
+        l = recs[node].first;
+        r = recs[node].last;
+        pk = recs[node].patch;
       }
-      climb<FP32>(b, n, od, node, l, r, pk, kd, elem, c0, c1, s_front, &s_nfront);
+      const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
+      rec_load_half<FP32>(b, is_left ? r : l - 1, is_left ? 0 : 1, kd);
+      climb<FP32>(b, n, node, l, r, pk, kd, elem, c0, c1, s_front, &s_nfront);
     }
   }
   __syncthreads();
   if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
 }
 
-// rope(node [l, r]) = right child of internal node r (none when r = n-1): the
-// next subtree in depth-first order once this one is done (fp64 nodes).
-__global__ __launch_bounds__(256) void k_ropes(BuildBufs b, uint64_t n) {
-  const uint64_t total = 2 * n - 1;
-  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < total;
-       id += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t last = id < n - 1 ? b.ilast[id] : (uint32_t)(id - (n - 1));
-    uint32_t rope = kEnd;
-    if (last < n - 1) {
-      rope = b.rightc[last];
-      if (rope >= total) {
-        atomicOr(b.status, kErrTree);
-        rope = kEnd;
-      }
-    }
-    b.ropes[id] = rope;
+// sorted leaf slabs, SoA [26][n] -> AoS n x 26 (the snapshot k-DOPs in sorted order)
+__global__ __launch_bounds__(256) void k_leaf_aos(const double* __restrict__ soa, uint64_t n, double* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kSlabs;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t p = i / kSlabs, k = i - p * kSlabs;
+    out[i] = soa[k * n + p];
   }
 }
 
@@ -917,6 +879,11 @@ hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint6
 hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
   const uint64_t n = in.n;
   if (n == 0) return hipSuccess;
+  // every buffer the kernels write must exist for this form
+  if (!b.patch_k64 || !b.patch_root || !b.counters || !b.front_a || !b.front_b || !b.delta || !b.perm)
+    return hipErrorInvalidValue;
+  if (fp32 ? (!b.rec32 || !b.leafbox32 || !b.rootbox32 || !b.origin) : (!b.rec64 || !b.leafbox64 || !b.rootbox64))
+    return hipErrorInvalidValue;
   if (in.kind == kPoly) {
     switch (in.nv) {
       case 1: tree_dispatch<kPoly, 1>(in, b, od, fp32, s); break;
@@ -944,8 +911,8 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     const uint64_t nsub = (n + chunk - 1) / chunk;
     const uint64_t next = chunk * kTreeChunk;
     const unsigned blocks = (unsigned)((n + next - 1) / next);
-    if (fp32) hipLaunchKernelGGL(k_tree_up<true>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
-    else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, od, next, nsub, fin, fout);
+    if (fp32) hipLaunchKernelGGL(k_tree_up<true>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, next, nsub, fin, fout);
+    else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, next, nsub, fin, fout);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint32_t* tmp = fin;
@@ -964,16 +931,16 @@ hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s) 
 hipError_t launch_patch_k64_init(unsigned long long* pk64, uint64_t od, hipStream_t s) {
   const uint64_t count = od * kSlabs;
   if (count == 0) return hipSuccess;
+  if (!pk64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_patch_k64_init, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, pk64, count);
   return hipGetLastError();
 }
 
-hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s) {
+hipError_t launch_leaf_aos(const double* soa, uint64_t n, double* out, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  uint64_t rb = (2 * n - 1 + 255) / 256;
-  if (rb > 4096) rb = 4096;
-  if (fp32) return hipSuccess;  // the fp32 records are walked with a stack
-  hipLaunchKernelGGL(k_ropes, dim3((unsigned)rb), dim3(256), 0, s, b, n);
+  uint64_t blocks = (n * kSlabs + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_leaf_aos, dim3((unsigned)blocks), dim3(256), 0, s, soa, n, out);
   return hipGetLastError();
 }
 

@@ -112,6 +112,7 @@ struct bvh_world_s {
   DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
   // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
   DevBuf frame;
+  LaunchShape shape;  // narrowphase launch shape (autotuned)
   // stream capture (batching form): objects whose queries were captured
   bool capturing = false;
   std::vector<bvh_object_s*> captured;
@@ -158,8 +159,8 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, nodes, rec32, leafbox32, rootbox32, status, rightc, ilast, front_a, front_b, ropes,
-      patch_k64;
+      patch_of_input, counters, patch_root, root, rec64, leafbox64, rootbox64, rec32, leafbox32, rootbox32, status,
+      front_a, front_b, patch_k64;
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;
@@ -179,12 +180,26 @@ struct bvh_object_s {
   // the last launched query: kind and the list capacities it ran with
   bool q_self = false, q_narrow = false;
   uint64_t q_pcap = 0, q_cap = 0, q_rcap = 0;
+
+  size_t device_bytes() const {
+    const DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
+                           &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
+                           &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
+                           &root, &rec64, &leafbox64, &rootbox64, &rec32, &leafbox32, &rootbox32, &status,
+                           &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
+                           &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest};
+    size_t t = 0;
+    for (const DevBuf* d : all) t += d->bytes;
+    return t;
+  }
 };
 
 // ---- helpers ---------------------------------------------------------------------------
 static TreeView view_of(const bvh_object_s* o) {
   TreeView v{};
-  v.nodes = o->nodes.as<Node64>();
+  v.rec64 = o->rec64.as<Rec64>();
+  v.leafbox64 = o->leafbox64.as<double>();
+  v.rootbox64 = o->rootbox64.as<double>();
   v.rec32 = o->rec32.as<Rec32>();
   v.leafbox32 = o->leafbox32.as<float>();
   v.rootbox32 = o->rootbox32.as<float>();
@@ -195,10 +210,9 @@ static TreeView view_of(const bvh_object_s* o) {
   v.n = o->in.n;
   v.od = o->w->od;
   v.status = o->status.as<uint32_t>();
-  v.ropes = o->ropes.as<uint32_t>();
   v.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   v.elem = o->in;
-  v.patch_k64 = o->fp32 ? o->patch_k64.as<unsigned long long>() : nullptr;
+  v.patch_k64 = o->patch_k64.as<unsigned long long>();
   return v;
 }
 
@@ -241,18 +255,17 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
   BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, -, max fp32 slab error, -]
-  BVH_TRY(o->rightc.ensure(n1 * 4));
-  BVH_TRY(o->ropes.ensure((2 * n1 - 1) * 4));
-  BVH_TRY(o->ilast.ensure(n1 * 4));
   BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
   BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
+  BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
     BVH_TRY(o->leafbox32.ensure((uint64_t)kSlabs * n1 * sizeof(float)));
     BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(float)));
-    BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   } else {
-    BVH_TRY(o->nodes.ensure((2 * n1 - 1) * sizeof(Node64)));
+    BVH_TRY(o->rec64.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec64)));
+    BVH_TRY(o->leafbox64.ensure((uint64_t)kSlabs * n1 * sizeof(double)));
+    BVH_TRY(o->rootbox64.ensure(kSlabs * sizeof(double)));
   }
   BuildBufs& b = o->bb;
   b.centroid = o->centroid.as<double>();
@@ -278,18 +291,17 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.counters = o->counters.as<uint32_t>();
   b.patch_root = o->patch_root.as<uint32_t>();
   b.root = o->root.as<uint32_t>();
-  b.nodes = o->fp32 ? nullptr : o->nodes.as<Node64>();
+  b.rec64 = o->fp32 ? nullptr : o->rec64.as<Rec64>();
+  b.leafbox64 = o->fp32 ? nullptr : o->leafbox64.as<double>();
+  b.rootbox64 = o->fp32 ? nullptr : o->rootbox64.as<double>();
   b.rec32 = o->fp32 ? o->rec32.as<Rec32>() : nullptr;
   b.leafbox32 = o->fp32 ? o->leafbox32.as<float>() : nullptr;
   b.rootbox32 = o->fp32 ? o->rootbox32.as<float>() : nullptr;
   b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
-  b.rightc = o->rightc.as<uint32_t>();
-  b.ilast = o->ilast.as<uint32_t>();
   b.front_a = o->front_a.as<uint32_t>();
   b.front_b = o->front_b.as<uint32_t>();
-  b.ropes = o->ropes.as<uint32_t>();
-  b.patch_k64 = o->fp32 ? o->patch_k64.as<unsigned long long>() : nullptr;
+  b.patch_k64 = o->patch_k64.as<unsigned long long>();
   return BVH_OK;
 }
 
@@ -336,16 +348,13 @@ static bvh_status object_build(bvh_object_s* o) {
   if (o->fp32) {
     // the world's fp32 frame: the Morton bounds minimum of its first non-empty build
     HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
-    HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   }
+  HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   w->stage_begin("delta");
   HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
   w->stage_end();
   w->stage_begin("tree");
   HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
-  w->stage_end();
-  w->stage_begin("ropes");
-  HIP_TRY(launch_ropes(o->bb, n, o->fp32, s));
   w->stage_end();
   o->built = true;
   return BVH_OK;
@@ -364,7 +373,7 @@ static bvh_status object_refit(bvh_object_s* o) {
   if (n > 1) HIP_TRY(launch_fill_u32(o->bb.counters, n - 1, 0u, s));
   HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
   HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
-  if (o->fp32) HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
+  HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
   w->stage_end();
   return BVH_OK;
@@ -505,7 +514,7 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
     w->stage_begin("narrowphase");
     PairOut po{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
                o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntRecheck};
-    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, po, s));
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, w->shape, po, s));
     w->stage_end();
   }
   o->result_ptr = o->pairs.as<uint64_t>();
@@ -741,6 +750,41 @@ bvh_status bvh_object_id(bvh_object o, size_t* id) {
   return BVH_OK;
 }
 
+bvh_status bvh_object_device_bytes(bvh_object o, uint64_t* bytes) {
+  if (!o || !bytes) return fail(BVH_E_INVALID, "null argument");
+  *bytes = o->device_bytes();
+  return BVH_OK;
+}
+
+bvh_status bvh_object_root_kdop(bvh_object o, double* kdop) {
+  if (!o || !kdop) return fail(BVH_E_INVALID, "null argument");
+  if (!o->built) return fail(BVH_E_STATE, "object not built");
+  for (int i = 0; i < 26; ++i) kdop[i] = (i & 1) ? -1.7976931348623157e308 : 1.7976931348623157e308;
+  if (o->in.n == 0) return BVH_OK;
+  hipStream_t s = o->w->stream;
+  if (o->fp32) {
+    float f[kSlabs];
+    double org[3];
+    BVH_TRY(copy_out(f, o->rootbox32.p, sizeof(f), BVH_MEM_HOST, s));
+    BVH_TRY(copy_out(org, o->w->frame.p, sizeof(org), BVH_MEM_HOST, s));
+    static const double N[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                    {0.5773502691896258, 0.5773502691896258, 0.5773502691896258},
+                                    {0.5773502691896258, -0.5773502691896258, 0.5773502691896258},
+                                    {0.5773502691896258, 0.5773502691896258, -0.5773502691896258},
+                                    {0.5773502691896258, -0.5773502691896258, -0.5773502691896258},
+                                    {0.7071067811865475, 0.7071067811865475, 0}, {0.7071067811865475, 0, 0.7071067811865475},
+                                    {0, 0.7071067811865475, 0.7071067811865475}, {0.7071067811865475, -0.7071067811865475, 0},
+                                    {0.7071067811865475, 0, -0.7071067811865475}, {0, 0.7071067811865475, -0.7071067811865475}};
+    for (int a = 0; a < kAxes; ++a) {
+      const double po = org[0] * N[a][0] + org[1] * N[a][1] + org[2] * N[a][2];
+      kdop[2 * a] = (double)f[2 * a] + po;
+      kdop[2 * a + 1] = (double)f[2 * a + 1] + po;
+    }
+    return BVH_OK;
+  }
+  return copy_out(kdop, o->rootbox64.p, 26 * sizeof(double), BVH_MEM_HOST, s);
+}
+
 bvh_status bvh_object_overdecomposition(bvh_object o, size_t* od) {
   if (!o || !od) return fail(BVH_E_INVALID, "null argument");
   *od = o->w->od;
@@ -973,12 +1017,16 @@ bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
-  // leaves n-1 .. 2n-2: copy the 26 slabs out of each 256-byte node
-  HIP_TRY(hipMemcpy2DAsync(kdops, kSlabs * sizeof(double), o->nodes.as<Node64>() + (n - 1), sizeof(Node64),
-                           kSlabs * sizeof(double), n,
-                           mem == BVH_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return BVH_OK;
+  // the SoA leaf slabs, transposed to n x 26
+  if (mem == BVH_MEM_DEVICE) {
+    HIP_TRY(launch_leaf_aos(o->leafbox64.as<double>(), n, kdops, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BVH_OK;
+  }
+  DevBuf tmp;
+  BVH_TRY(tmp.ensure(n * kSlabs * sizeof(double)));
+  HIP_TRY(launch_leaf_aos(o->leafbox64.as<double>(), n, tmp.as<double>(), s));
+  return copy_out(kdops, tmp.p, n * kSlabs * sizeof(double), BVH_MEM_HOST, s);
 }
 
 bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
@@ -1011,14 +1059,10 @@ bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
       pi.kdop[2 * a] = 1.7976931348623157e308;
       pi.kdop[2 * a + 1] = -1.7976931348623157e308;
     }
-    if (roots[k] != kEnd && o->fp32) {
+    if (roots[k] != kEnd) {
       uint64_t enc[kSlabs];
       BVH_TRY(copy_out(enc, o->patch_k64.as<unsigned long long>() + k * kSlabs, sizeof(enc), BVH_MEM_HOST, s));
       for (int i = 0; i < 26; ++i) pi.kdop[i] = from_ordered_u64(enc[i]);
-    } else if (roots[k] != kEnd) {
-      Node64 nd;
-      BVH_TRY(copy_out(&nd, o->nodes.as<Node64>() + roots[k], sizeof(Node64), BVH_MEM_HOST, s));
-      for (int i = 0; i < 26; ++i) pi.kdop[i] = nd.s[i];
     }
     for (int c = 0; c < 3; ++c) pi.centroid[c] = cent[3 * k + c];
   }
@@ -1055,18 +1099,31 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
   if (root) *root = (r == kEnd) ? (size_t)-1 : r;
   if (!kdops && !children && !ranges) return BVH_OK;
   if (cap < nn) return fail(BVH_E_OVERFLOW, "tree buffer too small");
-  std::vector<Node64> nodes(nn);
-  if (nn) BVH_TRY(copy_out(nodes.data(), o->nodes.p, nn * sizeof(Node64), BVH_MEM_HOST, o->w->stream));
-  for (size_t i = 0; i < nn; ++i) {
-    const bool leaf = i >= n - 1;
-    if (kdops) std::memcpy(kdops + 26 * i, nodes[i].s, 26 * sizeof(double));
+  if (nn == 0) return BVH_OK;
+  // a node's slabs are in its parent's record (the root's in rootbox)
+  std::vector<Rec64> recs(n - 1);
+  double rootk[kSlabs];
+  if (n > 1) BVH_TRY(copy_out(recs.data(), o->rec64.p, (n - 1) * sizeof(Rec64), BVH_MEM_HOST, o->w->stream));
+  BVH_TRY(copy_out(rootk, o->rootbox64.p, sizeof(rootk), BVH_MEM_HOST, o->w->stream));
+  if (kdops && r < nn) std::memcpy(kdops + 26 * (size_t)r, rootk, sizeof(rootk));
+  for (size_t i = n - 1; i < nn; ++i) {
+    if (children) children[2 * i] = children[2 * i + 1] = -1;
+    if (ranges) ranges[2 * i] = ranges[2 * i + 1] = i - (n - 1);
+  }
+  for (size_t x = 0; x + 1 < n; ++x) {
+    const Rec64& R = recs[x];
     if (children) {
-      children[2 * i] = leaf ? -1 : (int64_t)nodes[i].left;
-      children[2 * i + 1] = leaf ? -1 : (int64_t)nodes[i].right;
+      children[2 * x] = R.child[0];
+      children[2 * x + 1] = R.child[1];
     }
     if (ranges) {
-      ranges[2 * i] = nodes[i].first;
-      ranges[2 * i + 1] = nodes[i].last;
+      ranges[2 * x] = R.first;
+      ranges[2 * x + 1] = R.last;
+    }
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t c = R.child[side];
+      if (c >= nn) return fail(BVH_E_HIP, "tree record holds an invalid child id");
+      if (kdops) std::memcpy(kdops + 26 * (size_t)c, R.box[side], 26 * sizeof(double));
     }
   }
   return BVH_OK;
@@ -1088,44 +1145,63 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
 }
 
 // ---- autotuner ---------------------------------------------------------------------------
-bvh_status bvh_autotune_leaf_size(bvh_object o, bvh_object other, uint32_t* best, double* ms) {
+bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   if (!o || !other || !best) return fail(BVH_E_INVALID, "null argument");
   if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "autotune inside a capture");
   const bool self = o == other;
   static const uint32_t kLeaf[4] = {1, 2, 4, 8};
-  const uint32_t keep = w->cfg.leaf_size;
+  static const uint32_t kBlock[3] = {64, 128, 256};
+  const uint32_t keep_leaf = w->cfg.leaf_size;
+  const LaunchShape keep_shape = w->shape;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  double best_ms = 0.0;
-  uint32_t best_leaf = keep;
+  bvh_tuning b{};
+  b.ms = -1.0;
   bvh_status st = BVH_OK;
-  for (int k = 0; k < 4 && st == BVH_OK; ++k) {
-    w->cfg.leaf_size = kLeaf[k];
-    st = run_query(o, other, self);  // sizes the buffers
-    double t = 0.0;
-    for (int r = 0; r < 3 && st == BVH_OK; ++r) {
-      (void)hipEventRecord(e0, w->stream);
-      st = run_query(o, other, self);
-      (void)hipEventRecord(e1, w->stream);
-      (void)hipEventSynchronize(e1);
-      float f = 0.0f;
-      (void)hipEventElapsedTime(&f, e0, e1);
-      t = (r == 0 || f < t) ? f : t;
-    }
-    if (ms) ms[k] = t;
-    if (st == BVH_OK && (k == 0 || t < best_ms)) {
-      best_ms = t;
-      best_leaf = kLeaf[k];
-    }
-  }
+  // the fp32 walk has no leaf clusters: only the launch shape matters there
+  const int nleaf = o->fp32 ? 1 : 4;
+  uint32_t k = 0;
+  for (int li = 0; li < nleaf && st == BVH_OK; ++li)
+    for (int bi = 0; bi < 3 && st == BVH_OK; ++bi)
+      for (int xi = 0; xi < 2 && st == BVH_OK; ++xi) {
+        w->cfg.leaf_size = o->fp32 ? keep_leaf : kLeaf[li];
+        w->shape.block = kBlock[bi];
+        w->shape.xcd_remap = xi == 1;
+        st = run_query(o, other, self);  // sizes the buffers
+        double t = 0.0;
+        for (int r = 0; r < 3 && st == BVH_OK; ++r) {
+          (void)hipEventRecord(e0, w->stream);
+          st = run_query(o, other, self);
+          (void)hipEventRecord(e1, w->stream);
+          (void)hipEventSynchronize(e1);
+          float f = 0.0f;
+          (void)hipEventElapsedTime(&f, e0, e1);
+          t = (r == 0 || f < t) ? f : t;
+        }
+        if (k < 24) b.tried_ms[k] = t;
+        ++k;
+        if (st == BVH_OK && (b.ms < 0.0 || t < b.ms)) {
+          b.ms = t;
+          b.leaf_size = w->cfg.leaf_size;
+          b.block_size = w->shape.block;
+          b.xcd_remap = w->shape.xcd_remap ? 1 : 0;
+        }
+      }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  w->cfg.leaf_size = st == BVH_OK ? best_leaf : keep;
-  if (st != BVH_OK) return st;
-  *best = best_leaf;
+  if (st != BVH_OK) {
+    w->cfg.leaf_size = keep_leaf;
+    w->shape = keep_shape;
+    return st;
+  }
+  b.tried = k;
+  w->cfg.leaf_size = b.leaf_size;
+  w->shape.block = b.block_size;
+  w->shape.xcd_remap = b.xcd_remap != 0;
+  *best = b;
   return run_query(o, other, self);  // leave the results of the chosen setting
 }
 

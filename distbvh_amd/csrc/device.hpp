@@ -161,28 +161,14 @@ __device__ __forceinline__ uint32_t morton30(double x, double y, double z, const
   return (expand32(q[2]) << 2) + (expand32(q[1]) << 1) + expand32(q[0]);
 }
 
-// ---- tree nodes -----------------------------------------------------------------
-// One 256-byte node per tree node, AoS: traversal reads nodes at random, so a
-// node's 26 slabs and its links share two 128-byte cache lines.
+// ---- tree records ---------------------------------------------------------------
 // Internal node k (k in [0, n-2]) is the split between sorted positions k and
 // k+1; leaf p is node n-1+p.
-struct alignas(256) Node64 {
-  double s[kSlabs];  // 208 B
-  uint32_t left;     // internal: left child id; leaf: input index of the element
-  uint32_t right;    // internal: right child id; leaf: unused
-  uint32_t first;    // covered sorted positions [first, last]
-  uint32_t last;
-  uint32_t rope;     // next node in depth-first order after this subtree (kEnd: none)
-  uint32_t patch;    // patch id if the subtree lies in one patch, else kMixed
-  uint32_t pad[6];
-};
-static_assert(sizeof(Node64) == 256, "node layout");
-
-// fp32 node for the outward-rounded precision form: 104 B slabs + links in
-// one 128-byte line.
-// fp32 form: one record per internal node holding both children's slabs,
-// so a traversal step tests two nodes with one memory round trip.  A node's
-// own slabs live in its parent's record (the root's in a separate array).
+// One record per internal node holding both children's slabs, so a
+// traversal step tests two nodes with one memory round trip.  A node's own
+// slabs live in its parent's record (the root's in a separate array).
+// fp32 form: 256 B, read with scalar loads; fp64 form: 448 B, staged through
+// LDS by a wave-wide vector load.
 struct alignas(256) Rec32 {
   float box[2][kSlabs];  // left, right child (208 B)
   uint32_t child[2];     // child node ids (leaf p = n-1+p)
@@ -192,6 +178,27 @@ struct alignas(256) Rec32 {
   uint32_t pad;
 };
 static_assert(sizeof(Rec32) == 256, "record layout");
+
+struct alignas(64) Rec64 {
+  double box[2][kSlabs];  // left, right child (416 B)
+  uint32_t child[2];
+  uint32_t elem[2];
+  uint32_t first, last;
+  uint32_t patch;
+  uint32_t pad;
+};
+static_assert(sizeof(Rec64) == 448, "record layout");
+
+template <bool FP32>
+struct Form {
+  using T = double;
+  using Rec = Rec64;
+};
+template <>
+struct Form<true> {
+  using T = float;
+  using Rec = Rec32;
+};
 
 
 // Order-preserving map of a double to an unsigned 64-bit integer (x < y ==>
@@ -278,21 +285,6 @@ __device__ __forceinline__ void element_kdop_any(const ElemInput& in, uint32_t e
       case 7: element_kdop<kPoly, 7>(in, e, kd); break;
       default: element_kdop<kPoly, 8>(in, e, kd); break;
     }
-  }
-}
-
-__device__ __forceinline__ void store_slabs(Node64* nd, const double* kd) {
-  double2* d = (double2*)nd->s;
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) d[a] = make_double2(kd[2 * a], kd[2 * a + 1]);
-}
-__device__ __forceinline__ void load_slabs(const Node64* nd, double* kd) {
-  const double2* d = (const double2*)nd->s;
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) {
-    double2 t = d[a];
-    kd[2 * a] = t.x;
-    kd[2 * a + 1] = t.y;
   }
 }
 

@@ -73,18 +73,17 @@ struct BuildBufs {
   uint32_t* counters;      // n-1 arrival counters
   uint32_t* patch_root;    // od
   uint32_t* root;          // 1
-  Node64* nodes;           // 2n-1
+  Rec64* rec64;            // fp64 form: n-1 child-pair records
+  double* leafbox64;       // fp64 form: leaf slabs, SoA [26][n] in sorted order
+  double* rootbox64;       // fp64 form: the root's slabs (26)
   Rec32* rec32;            // fp32 form: n-1 child-pair records
   float* leafbox32;        // fp32 form: leaf slabs, SoA [26][n] in sorted order
   float* rootbox32;        // fp32 form: the root's slabs (26)
   double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
-  uint32_t* rightc;        // n-1: right child of each internal node
-  uint32_t* ilast;         // n-1: last covered position of each internal node
   uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
   uint32_t* front_b;
-  uint32_t* ropes;         // 2n-1: escape link of every node
-  unsigned long long* patch_k64;  // fp32 form: od x 26 exact patch slabs (ordered_u64 encoding)
+  unsigned long long* patch_k64;  // od x 26 exact (fp64) patch slabs, ordered_u64 encoding
 };
 
 // device error bits: set instead of faulting when an invariant breaks
@@ -102,15 +101,17 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
 // fp32 form: set the world frame (3 doubles, then a flag word) from the
 // bounds minimum unless already set; non-finite components become 0
 hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s);
-// fp32 form: patch_k64 set to the empty k-DOP before the leaf kernel
+// patch_k64 set to the empty k-DOP before the leaf kernel
 hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hipStream_t s);
-// k_ropes (escape links)
-hipError_t launch_ropes(const BuildBufs& b, uint64_t n, int fp32, hipStream_t s);
+// sorted leaf slabs SoA [26][n] -> n x 26
+hipError_t launch_leaf_aos(const double* soa, uint64_t n, double* out, hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
 
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {
-  const Node64* nodes;
+  const Rec64* rec64;
+  const double* leafbox64;
+  const double* rootbox64;
   const Rec32* rec32;
   const float* leafbox32;
   const float* rootbox32;
@@ -121,10 +122,9 @@ struct TreeView {
   uint64_t n;
   uint64_t od;
   uint32_t* status;
-  const uint32_t* ropes;
   const double* origin;  // fp32 form: frame origin (3 doubles)
   ElemInput elem;        // the object's element data (fp32 form re-checks in fp64)
-  const unsigned long long* patch_k64;  // fp32 form: exact patch slabs (ordered_u64)
+  const unsigned long long* patch_k64;  // exact patch slabs (ordered_u64)
 };
 
 struct PairOut {
@@ -139,10 +139,18 @@ struct PairOut {
   unsigned long long* rcount;
 };
 
+// launch shape of the narrowphase (tuned by the autotuner): threads per
+// block (64, 128 or 256) and whether blocks are remapped so each XCD walks a
+// contiguous range of query tiles
+struct LaunchShape {
+  unsigned block = 256;
+  bool xcd_remap = true;
+};
+
 // element narrowphase: every leaf of `q` (queries, in its sorted order)
 // against tree `t`; self != 0 emits each unordered pair of one tree once.
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self,
-                              int fp32, const PairOut& out, hipStream_t s);
+                              int fp32, const LaunchShape& shape, const PairOut& out, hipStream_t s);
 // patch broadphase: patch roots of `q` against the patch level of `t`
 hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
                                    hipStream_t s);

@@ -1,11 +1,12 @@
 // query.hip -- broadphase and narrowphase traversal on gfx950.
 //
-//  * k_narrow: one lane per query element (the query object's leaves in its
-//    sorted order, so a wave holds 64 spatially adjacent elements).  Each lane
-//    walks the other object's tree without a stack: on overlap it descends to
-//    the left child, otherwise (or after a leaf) it follows the node's rope.
-//    Nodes whose range holds <= leaf_size elements are tested element by
-//    element.  Hits are appended with one atomic per wave and step.
+//  * k_narrow64 / k_narrow32: one lane per query element (the query object's
+//    leaves in its sorted order, so a wave holds 64 spatially adjacent
+//    elements).  The wave walks the other object's tree with a wave-uniform
+//    node: each internal node's record holds both children's slabs, so one
+//    memory round trip tests two nodes; leaf children are tested at once and
+//    internal children that some lane overlaps go on a per-wave LDS stack.
+//    Hits are staged per wave in LDS and appended with one atomic per flush.
 //    Replaces query_tree / query_node_impl (collision_query.hpp:166-213) as
 //    used by a narrowphase functor per patch pair, and self_collision_set
 //    (collision_query.hpp:331-368) in self mode (each unordered pair once).
@@ -32,16 +33,6 @@ __device__ __forceinline__ void emit(const PairOut& out, bool pred, uint64_t key
   }
 }
 
-__device__ __forceinline__ void load_node_slabs(const Node64* nd, double* k) {
-  const double2* d = (const double2*)nd->s;
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) {
-    const double2 t = d[a];
-    k[2 * a] = t.x;
-    k[2 * a + 1] = t.y;
-  }
-}
-
 __device__ __forceinline__ uint64_t pair_key(uint32_t a, uint32_t b, bool self) {
   if (self && b < a) {
     const uint32_t t = a;
@@ -51,20 +42,7 @@ __device__ __forceinline__ uint64_t pair_key(uint32_t a, uint32_t b, bool self) 
   return ((uint64_t)a << 32) | b;
 }
 
-// Node slabs and links read through the scalar cache: the traversal below
-// keeps the node index wave-uniform, so a constant-address-space pointer makes
-// every node load an s_load into SGPRs, and each lane compares its own query
-// slabs (VGPRs) against them.
-typedef __attribute__((address_space(4))) const Node64 CNode64;
-
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ bool overlap_cnode(const double* q, CNode64* nd) {
-  bool o = true;
-#pragma unroll
-  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < nd->s[2 * i + 1]) & (nd->s[2 * i] < q[2 * i + 1]);
-  return o;
-}
 
 __device__ __forceinline__ void empty_slabs(double* k) {
 #pragma unroll
@@ -85,7 +63,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // Per-wave pair staging in LDS: hits are appended with a ballot (all lanes of
 // the wave are converged in the uniform traversal loop) and written to
 // global memory in coalesced runs with one atomic per flush.
-constexpr uint32_t kWaveBuf = 128;
+constexpr uint32_t kWaveBuf = 512;
 
 __device__ __forceinline__ void wave_flush(uint64_t* buf, uint32_t& fill, const PairOut& out) {
   if (fill == 0) return;
@@ -109,64 +87,6 @@ __device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const P
   if (fill + cnt > kWaveBuf) wave_flush(buf, fill, out);
   if (pred) buf[fill + (uint32_t)__popcll(m & lanemask_lt())] = key;
   fill += cnt;
-}
-
-// Element narrowphase: lane = query element at sorted position p of q; the
-// wave walks t depth first (left child on overlap, rope otherwise) visiting a
-// node when any lane's query overlaps it.
-template <bool SELF>
-__global__ __launch_bounds__(256) void k_narrow(TreeView q, TreeView t, int leaf_size, PairOut out) {
-  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
-  uint64_t* buf = s_buf[threadIdx.x / kWave];
-  uint32_t fill = 0;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
-  const bool valid = p < q.n;
-  double qk[kSlabs];
-  uint32_t qa = 0;
-  if (valid) {
-    const Node64* qn = &q.nodes[q.n - 1 + p];
-    load_node_slabs(qn, qk);
-    qa = qn->left;
-  } else {
-    empty_slabs(qk);
-  }
-  const uint32_t tleaf0 = (uint32_t)(t.n - 1);
-  const uint32_t nnodes = (uint32_t)(2 * t.n - 1);
-  CNode64* nodes = (CNode64*)t.nodes;
-  const __attribute__((address_space(4))) uint32_t* ropes = (const __attribute__((address_space(4))) uint32_t*)t.ropes;
-  uint32_t node = uniform(*t.root);
-  uint32_t steps = 0;
-  while (node != kEnd) {
-    if (node >= nnodes || ++steps > nnodes) {
-      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
-      break;
-    }
-    CNode64* nd = nodes + node;
-    const uint32_t first = nd->first, last = nd->last, rope = ropes[node];
-    bool hit = overlap_cnode(qk, nd);
-    if (SELF) hit = hit && last > (uint32_t)p;
-    if (__ballot(hit) == 0) {
-      node = rope;
-      continue;
-    }
-    if (node >= tleaf0) {
-      wave_emit(buf, fill, out, hit, pair_key(qa, nd->left, SELF));
-      node = rope;
-    } else if (last - first < (uint32_t)leaf_size) {
-      for (uint32_t pos = first; pos <= last; ++pos) {
-        CNode64* ln = nodes + (tleaf0 + pos);
-        bool h = hit && overlap_cnode(qk, ln);
-        if (SELF) h = h && pos > (uint32_t)p;
-        wave_emit(buf, fill, out, h, pair_key(qa, ln->left, SELF));
-      }
-      node = rope;
-    } else {
-      node = nd->left;
-    }
-  }
-  wave_flush(buf, fill, out);
-  if (out.visits && lane_id() == 0) atomicAdd(out.visits, (unsigned long long)steps);
 }
 
 // fp32-outward form.  Node slabs are fp32 in the frame of the world's origin P
@@ -211,13 +131,13 @@ __device__ __forceinline__ bool overlap_c32(const float* q, CFloat* box) {
 }
 
 template <bool SELF>
-__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, PairOut out) {
+__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xcd, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
   __shared__ uint32_t s_stack[256 / kWave][kStack];
   uint64_t* buf = s_buf[threadIdx.x / kWave];
   uint32_t* stk = s_stack[threadIdx.x / kWave];
   uint32_t fill = 0;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
   float qf[kSlabs];
@@ -322,6 +242,93 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, PairOu
   }
 }
 
+// fp64 form: the same record walk.  Two fp64 boxes (416 B) do not fit the
+// scalar registers, so a 448-byte record is staged in LDS with one 8-byte
+// vector load per lane (56 lanes) and every lane reads both boxes as LDS
+// broadcasts.  Leaf tests are exact, so hits are emitted directly.
+template <bool SELF>
+__global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xcd, PairOut out) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  __shared__ Rec64 s_rec[256 / kWave];
+  const uint32_t wv = threadIdx.x / kWave;
+  uint64_t* buf = s_buf[wv];
+  uint32_t* stk = s_stack[wv];
+  Rec64& R = s_rec[wv];
+  uint32_t fill = 0;
+  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
+  const bool valid = p < q.n;
+  double qk[kSlabs];
+  uint32_t qa = 0;
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) qk[i] = q.leafbox64[(uint64_t)i * q.n + p];
+    qa = q.perm[p];
+  } else {
+    empty_slabs(qk);
+  }
+  const uint32_t n = (uint32_t)t.n;
+  const uint32_t tleaf0 = n - 1;
+  uint32_t steps = 0;
+  uint32_t sp = 0;
+  {
+    // the root's slabs through scalar loads (wave-uniform)
+    const __attribute__((address_space(4))) double* rb = (const __attribute__((address_space(4))) double*)t.rootbox64;
+    bool h = true;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) h = h & (qk[2 * a] < rb[2 * a + 1]) & (rb[2 * a] < qk[2 * a + 1]);
+    if (SELF) h = h && (uint64_t)(n - 1) > p;  // the root's last position
+    if (n == 1) {
+      if (!SELF) wave_emit(buf, fill, out, h, pair_key(qa, uniform(t.perm[0]), SELF));
+    } else if (__ballot(h) != 0) {
+      stk[0] = uniform(*t.root);
+      sp = 1;
+    }
+  }
+  const uint32_t lane = lane_id();
+  while (sp > 0) {
+    --sp;
+    const uint32_t x = uniform(stk[sp]);
+    if (x >= tleaf0 || ++steps > n) {
+      if (lane == 0) atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    {
+      const uint64_t* src = (const uint64_t*)(t.rec64 + x);
+      if (lane < sizeof(Rec64) / 8) ((uint64_t*)&R)[lane] = src[lane];
+      __builtin_amdgcn_wave_barrier();
+    }
+    const uint32_t cl = uniform(R.child[0]), cr = uniform(R.child[1]);
+    const uint32_t el = uniform(R.elem[0]), er = uniform(R.elem[1]);
+    const uint32_t rlast = uniform(R.last);
+    bool hl = overlap(qk, R.box[0]);
+    bool hr = overlap(qk, R.box[1]);
+    __builtin_amdgcn_wave_barrier();  // every lane has read the record
+    if (SELF) {
+      hl = hl && (uint64_t)x > p;  // the left child ends at position x
+      hr = hr && (uint64_t)rlast > p;
+    }
+    bool push_l = false, push_r = false;
+    if (__ballot(hl) != 0) {
+      if (cl >= tleaf0) wave_emit(buf, fill, out, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl, pair_key(qa, el, SELF));
+      else push_l = true;
+    }
+    if (__ballot(hr) != 0) {
+      if (cr >= tleaf0) wave_emit(buf, fill, out, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr, pair_key(qa, er, SELF));
+      else push_r = true;
+    }
+    if (sp + (push_l ? 1u : 0u) + (push_r ? 1u : 0u) > kStack) {
+      if (lane == 0) atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    if (push_r) stk[sp++] = cr;
+    if (push_l) stk[sp++] = cl;
+  }
+  wave_flush(buf, fill, out);
+  if (lane == 0 && out.visits) atomicAdd(out.visits, (unsigned long long)steps);
+}
+
 // fp64 re-check of the fp32 candidates that the inner bound left open: both
 // elements' k-DOPs recomputed from element data, the pair emitted on overlap.
 // The list length is read on the device, so no host round trip separates the
@@ -345,16 +352,11 @@ __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut
   }
 }
 
-// exact patch slabs: the patch root node (fp64 form) or the fp64 patch slabs
-// kept beside an fp32 tree
-__device__ __forceinline__ void patch_slabs(const TreeView& v, uint64_t k, uint32_t r, double* out) {
-  if (v.patch_k64) {
-    const unsigned long long* e = v.patch_k64 + k * kSlabs;
+// exact patch slabs (the union of the patch's element k-DOPs)
+__device__ __forceinline__ void patch_slabs(const TreeView& v, uint64_t k, double* out) {
+  const unsigned long long* e = v.patch_k64 + k * kSlabs;
 #pragma unroll
-    for (int i = 0; i < kSlabs; ++i) out[i] = from_ordered_u64(e[i]);
-  } else {
-    load_node_slabs(&v.nodes[r], out);
-  }
+  for (int i = 0; i < kSlabs; ++i) out[i] = from_ordered_u64(e[i]);
 }
 
 // Patch broadphase over all patch pairs: one lane per (patch of q, patch of
@@ -369,8 +371,8 @@ __global__ __launch_bounds__(256) void k_patch_pairs(TreeView q, TreeView t, Pai
     const uint32_t rq = q.patch_root[qp], rt = t.patch_root[tp];
     if (rq != kEnd && rq < 2 * q.n - 1 && rt != kEnd && rt < 2 * t.n - 1) {
       double qk[kSlabs], tk[kSlabs];
-      patch_slabs(q, qp, rq, qk);
-      patch_slabs(t, tp, rt, tk);
+      patch_slabs(q, qp, qk);
+      patch_slabs(t, tp, tk);
       hit = overlap(qk, tk);
     }
   }
@@ -449,28 +451,35 @@ __global__ __launch_bounds__(256) void k_digest(const uint64_t* __restrict__ pai
 
 // ---- launchers ---------------------------------------------------------------------
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
-                              const PairOut& out, hipStream_t s) {
+                              const LaunchShape& shape, const PairOut& out, hipStream_t s) {
+  (void)leaf_size;  // the record walk tests every leaf from its parent's record
   if (q.n == 0 || t.n == 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((q.n + 255) / 256);
+  const unsigned bs = shape.block;
+  if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
+  if (!q.perm || !t.perm || !t.root || !t.status || !out.pairs || !out.count) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)((q.n + bs - 1) / bs);
+  const int xcd = shape.xcd_remap ? 1 : 0;
   if (fp32) {
     if (!q.leafbox32 || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin) return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
-    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(256), 0, s, q, t, out);
-    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(256), 0, s, q, t, out);
+    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     // in self mode both keys index the one object (q == t)
     hipLaunchKernelGGL(k_recheck, dim3(512), dim3(256), 0, s, q, t, out);
     return hipGetLastError();
   }
-  if (self) hipLaunchKernelGGL(k_narrow<true>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
-  else hipLaunchKernelGGL(k_narrow<false>, dim3(blocks), dim3(256), 0, s, q, t, leaf_size, out);
+  if (!q.leafbox64 || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
+  if (self) hipLaunchKernelGGL(k_narrow64<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+  else hipLaunchKernelGGL(k_narrow64<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
   return hipGetLastError();
 }
 
 hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
                                    hipStream_t s) {
   if (q.od == 0 || t.od == 0 || t.n == 0 || q.n == 0) return hipSuccess;
-  if (fp32 && (!q.patch_k64 || !t.patch_k64)) return hipErrorInvalidValue;
+  if (!q.patch_k64 || !t.patch_k64) return hipErrorInvalidValue;
+  (void)fp32;
   const uint64_t total = q.od * t.od;
   hipLaunchKernelGGL(k_patch_pairs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, q, t, out);
   return hipGetLastError();

@@ -196,6 +196,11 @@ bvh_status bvh_patch_pairs(bvh_object o, uint64_t *out, uint64_t cap);
 /* local_patches(), collision_object.hpp:189; overdecomposition_factor() */
 bvh_status bvh_local_patches(bvh_object o, bvh_patch_info *out, size_t cap);
 bvh_status bvh_object_overdecomposition(bvh_object o, size_t *od);
+/* device memory the object holds, in bytes (its build and result buffers) */
+bvh_status bvh_object_device_bytes(bvh_object o, uint64_t *bytes);
+/* the root node's 26-DOP (fp32 form: the outward fp32 slabs shifted back to
+ * world coordinates); for refit-or-rebuild decisions */
+bvh_status bvh_object_root_kdop(bvh_object o, double *kdop);
 /* the input indices of the elements of one patch in patch order (the bytes
  * a narrowphase functor receives, collision_object/impl.hpp:112-118) */
 bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t *indices, uint64_t cap,
@@ -225,13 +230,21 @@ bvh_status bvh_sort_u32(bvh_world w, uint32_t *keys, uint32_t *vals, uint64_t n)
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 
 /* ---- autotuner ------------------------------------------------------------------ */
-/* Times the query o x other (self collision when o == other) with leaf sizes
- * 1, 2, 4 and 8 (best of 3 runs each, ms[4] if not NULL), sets the world's
- * leaf size to the fastest, returns it in *best and leaves that query's
- * results in o.  The pair set does not depend on the leaf size.  (No
- * reference counterpart; the traversal here is stackless and wave-wide, so
- * the leaf size is its one tuning parameter.) */
-bvh_status bvh_autotune_leaf_size(bvh_object o, bvh_object other, uint32_t *best, double *ms);
+typedef struct bvh_tuning {
+  uint32_t leaf_size;   /* elements tested leaf by leaf (fp64 walk only) */
+  uint32_t block_size;  /* narrowphase threads per block: 64, 128 or 256 */
+  uint32_t xcd_remap;   /* 1: each XCD walks a contiguous range of query tiles */
+  uint32_t tried;       /* configurations timed (entries of tried_ms) */
+  double ms;            /* best query time */
+  double tried_ms[24];  /* in the order leaf (1, 2, 4, 8) x block (64, 128, 256) x remap (off, on) */
+} bvh_tuning;
+/* Times the query o x other (self collision when o == other) over leaf size
+ * (fp64 form: 1, 2, 4, 8), block size (64, 128, 256) and XCD remap (off, on),
+ * best of 3 runs each, keeps the fastest setting for the world and leaves
+ * that query's results in o.  The pair set does not depend on the setting.
+ * (No reference counterpart: the walk here is wave-wide and stackless or
+ * record-based, and these are its tuning parameters.) */
+bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning *best);
 
 /* ---- batching form: a whole step as one HIP graph ----------------------------- */
 /* No reference counterpart (the reference launches per call).  Between

@@ -1,6 +1,6 @@
 """Batching form: a whole step (build + self collision of a batch of meshes)
 captured into one HIP graph and replayed on new element data in place; the
-leaf-size autotuner.  Every replay is checked against brute force on the data
+autotuner.  Every replay is checked against brute force on the data
 it ran on, with eager GPU work and synchronizations between replays."""
 import numpy as np
 import pytest
@@ -135,7 +135,7 @@ def test_capture_refuses_buffer_growth(torch_cuda):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_autotune_leaf_size(torch_cuda, precision):
+def test_autotune(torch_cuda, precision):
     a, b = W.wavy_sheets(64)
     ka, _ = O.elements_poly(a)
     kb, _ = O.elements_poly(b)
@@ -143,9 +143,18 @@ def test_autotune_leaf_size(torch_cuda, precision):
     with CollisionWorld(16, leaf_size=1, precision=precision) as w:
         oa = gpu_object(w, "poly", a, device=True)
         ob = gpu_object(w, "poly", b, device=True)
-        best, ms = oa.autotune_leaf_size(ob)
-        assert best in (1, 2, 4, 8)
-        assert all(t > 0 for t in ms)
+        t = oa.autotune(ob)
+        assert t["leaf_size"] in (1, 2, 4, 8)
+        assert t["block_size"] in (64, 128, 256)
+        assert len(t["tried_ms"]) == (6 if precision == "fp32" else 24)
+        assert all(x > 0 for x in t["tried_ms"]) and t["ms"] == min(t["tried_ms"])
         assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
-        oa.broadphase(ob)  # the chosen leaf size stays
+        oa.broadphase(ob)  # the chosen setting stays
         assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
+    # every launch shape gives the same pairs (self collision of the second sheet)
+    kd = kb
+    want_self = sort_pairs(O.brute_self(kd))
+    with CollisionWorld(16, leaf_size=2, precision=precision) as w:
+        o = gpu_object(w, "poly", b, device=True)
+        t = o.autotune()
+        assert np.array_equal(sort_pairs(o.pairs()), want_self)
