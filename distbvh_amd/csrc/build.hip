@@ -379,9 +379,9 @@ __device__ __forceinline__ void rec_store_half(const BuildBufs& b, uint32_t pare
                                                uint32_t child, uint32_t elem) {
   RecT<FP32>* R = recs_of<FP32>(b) + parent;
   if constexpr (FP32) {
-    float2* d = (float2*)R->box[side];  // 104-byte halves: 8-byte aligned
+    int2* d = (int2*)R->box[side];  // 104-byte halves: 8-byte aligned
 #pragma unroll
-    for (int q = 0; q < kAxes; ++q) d[q] = make_float2(k[2 * q], k[2 * q + 1]);
+    for (int q = 0; q < kAxes; ++q) d[q] = make_int2(k[2 * q], k[2 * q + 1]);
   } else {
     double2* d = (double2*)R->box[side];  // 208-byte halves: 16-byte aligned
 #pragma unroll
@@ -395,10 +395,10 @@ template <bool FP32>
 __device__ __forceinline__ void rec_load_half(const BuildBufs& b, uint32_t parent, int side, SlabT<FP32>* k) {
   const RecT<FP32>* R = recs_of<FP32>(b) + parent;
   if constexpr (FP32) {
-    const float2* d = (const float2*)R->box[side];
+    const int2* d = (const int2*)R->box[side];
 #pragma unroll
     for (int q = 0; q < kAxes; ++q) {
-      const float2 v = d[q];
+      const int2 v = d[q];
       k[2 * q] = v.x;
       k[2 * q + 1] = v.y;
     }
@@ -421,6 +421,17 @@ __device__ __forceinline__ void store_rootbox(const BuildBufs& b, const SlabT<FP
   for (int i = 0; i < kSlabs; ++i) {
     if constexpr (FP32) b.rootbox32[i] = k[i];
     else b.rootbox64[i] = k[i];
+  }
+}
+
+// fp64 form: a patch root's slabs are the patch's exact slabs (one writer
+// per patch); the fp32 form accumulates them from the elements instead
+template <bool FP32>
+__device__ __forceinline__ void store_patch_box(const BuildBufs& b, uint32_t patch, const SlabT<FP32>* k) {
+  if constexpr (!FP32) {
+    unsigned long long* d = b.patch_k64 + (uint64_t)patch * kSlabs;
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) d[i] = ordered_u64(k[i]);
   }
 }
 
@@ -483,8 +494,10 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t n
     recs[parent].first = l;
     recs[parent].last = r;
     recs[parent].patch = mypatch;
-    if (is_patch_root(mypatch, l, r, n, l ? b.delta[l - 1] : 0u, r != n - 1 ? b.delta[r] : 0u))
+    if (is_patch_root(mypatch, l, r, n, l ? b.delta[l - 1] : 0u, r != n - 1 ? b.delta[r] : 0u)) {
       b.patch_root[mypatch] = parent;
+      store_patch_box<FP32>(b, mypatch, kd);
+    }
     node = parent;
     elem = kEnd;
   }
@@ -628,8 +641,10 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     recs[parent].first = l;
     recs[parent].last = r;
     recs[parent].patch = mypatch;
-    if (is_patch_root(mypatch, l, r, n, l ? L.delta[l - c0] : 0u, r != n - 1 ? L.delta[r - c0 + 1] : 0u))
+    if (is_patch_root(mypatch, l, r, n, l ? L.delta[l - c0] : 0u, r != n - 1 ? L.delta[r - c0 + 1] : 0u)) {
       b.patch_root[mypatch] = parent;
+      store_patch_box<FP32>(b, mypatch, kd);
+    }
     publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
     node = parent;
     elem = kEnd;
@@ -667,36 +682,36 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   } else {
     empty_kdop(kd);
   }
-  // exact patch slabs for the patch broadphase
-  patch_union64(b.patch_k64, pk, kd);
+  // exact patch slabs for the patch broadphase (fp32 form; the fp64 form
+  // takes them from the patch roots)
+  if constexpr (FP32) patch_union64(b.patch_k64, pk, kd);
   SlabT<FP32> ks[kSlabs];
   const uint32_t leaf = (uint32_t)(n - 1 + p);
   if (act) {
     b.patch_of_input[e] = pk;
-    if (patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1) b.patch_root[pk] = leaf;
+    const bool single = patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1;
+    if (single) b.patch_root[pk] = leaf;
     if constexpr (FP32) {
       double po[kAxes];
       proj_all(b.origin[0], b.origin[1], b.origin[2], po);
-      to_fp32_outward(kd, po, ks);
+      float kf[kSlabs];
+      to_fp32_outward(kd, po, kf);
       // largest distance of an fp32 slab from its fp64 value, in fp32 ulps
       float err = 0.0f;
 #pragma unroll
       for (int i = 0; i < kSlabs; ++i) {
         const double exact = kd[i] - po[i / 2];
-        const float mag = __builtin_fabsf(ks[i]);
+        const float mag = __builtin_fabsf(kf[i]);
         const double ulp = (double)(next_up(mag) - mag);
-        const float d = (float)(__builtin_fabs((double)ks[i] - exact) / ulp);
+        const float d = (float)(__builtin_fabs((double)kf[i] - exact) / ulp);
         err = d > err ? d : err;
       }
       atomicMax(&L.err, __float_as_uint(err));
-#pragma unroll
-      for (int i = 0; i < kSlabs; ++i) b.leafbox32[(uint64_t)i * n + p] = ks[i];
+      ord32_slabs(kf, ks);
     } else {
 #pragma unroll
-      for (int i = 0; i < kSlabs; ++i) {
-        ks[i] = kd[i];
-        b.leafbox64[(uint64_t)i * n + p] = kd[i];
-      }
+      for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
+      if (single) store_patch_box<false>(b, pk, kd);
     }
     publish<FP32>(L, t, t, ks, leaf, p, pk);
   }
@@ -758,12 +773,21 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
   if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
 }
 
-// sorted leaf slabs, SoA [26][n] -> AoS n x 26 (the snapshot k-DOPs in sorted order)
-__global__ __launch_bounds__(256) void k_leaf_aos(const double* __restrict__ soa, uint64_t n, double* __restrict__ out) {
+// fp64 form: leaf p's slabs from its parent's record (n x 26, sorted order)
+__global__ __launch_bounds__(256) void k_sorted_kdops(const Rec64* __restrict__ rec, const double* __restrict__ rootbox,
+                                                      const uint8_t* __restrict__ delta, uint64_t n,
+                                                      double* __restrict__ out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kSlabs;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t p = i / kSlabs, k = i - p * kSlabs;
-    out[i] = soa[k * n + p];
+    if (n == 1) {
+      out[i] = rootbox[k];
+    } else {
+      uint32_t parent;
+      int side;
+      leaf_slot(delta, n, p, parent, side);
+      out[i] = rec[parent].box[side][k];
+    }
   }
 }
 
@@ -882,7 +906,7 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
   // every buffer the kernels write must exist for this form
   if (!b.patch_k64 || !b.patch_root || !b.counters || !b.front_a || !b.front_b || !b.delta || !b.perm)
     return hipErrorInvalidValue;
-  if (fp32 ? (!b.rec32 || !b.leafbox32 || !b.rootbox32 || !b.origin) : (!b.rec64 || !b.leafbox64 || !b.rootbox64))
+  if (fp32 ? (!b.rec32 || !b.rootbox32 || !b.origin) : (!b.rec64 || !b.rootbox64))
     return hipErrorInvalidValue;
   if (in.kind == kPoly) {
     switch (in.nv) {
@@ -936,11 +960,13 @@ hipError_t launch_patch_k64_init(unsigned long long* pk64, uint64_t od, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_leaf_aos(const double* soa, uint64_t n, double* out, hipStream_t s) {
+hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
+                               hipStream_t s) {
   if (n == 0) return hipSuccess;
+  if (!rec || !rootbox || !delta || !out) return hipErrorInvalidValue;
   uint64_t blocks = (n * kSlabs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_leaf_aos, dim3((unsigned)blocks), dim3(256), 0, s, soa, n, out);
+  hipLaunchKernelGGL(k_sorted_kdops, dim3((unsigned)blocks), dim3(256), 0, s, rec, rootbox, delta, n, out);
   return hipGetLastError();
 }
 

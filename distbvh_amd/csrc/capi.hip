@@ -159,7 +159,7 @@ struct bvh_object_s {
   // build buffers
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
-      patch_of_input, counters, patch_root, root, rec64, leafbox64, rootbox64, rec32, leafbox32, rootbox32, status,
+      patch_of_input, counters, patch_root, root, rec64, rootbox64, rec32, rootbox32, status,
       front_a, front_b, patch_k64;
   BuildBufs bb{};
   int bits = 63;
@@ -185,7 +185,7 @@ struct bvh_object_s {
     const DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
                            &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
                            &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
-                           &root, &rec64, &leafbox64, &rootbox64, &rec32, &leafbox32, &rootbox32, &status,
+                           &root, &rec64, &rootbox64, &rec32, &rootbox32, &status,
                            &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
                            &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest};
     size_t t = 0;
@@ -198,11 +198,10 @@ struct bvh_object_s {
 static TreeView view_of(const bvh_object_s* o) {
   TreeView v{};
   v.rec64 = o->rec64.as<Rec64>();
-  v.leafbox64 = o->leafbox64.as<double>();
   v.rootbox64 = o->rootbox64.as<double>();
   v.rec32 = o->rec32.as<Rec32>();
-  v.leafbox32 = o->leafbox32.as<float>();
-  v.rootbox32 = o->rootbox32.as<float>();
+  v.delta = o->delta.as<uint8_t>();
+  v.rootbox32 = o->rootbox32.as<int32_t>();
   v.root = o->root.as<uint32_t>();
   v.patch_root = o->patch_root.as<uint32_t>();
   v.patch_of_input = o->patch_of_input.as<uint32_t>();
@@ -260,11 +259,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
-    BVH_TRY(o->leafbox32.ensure((uint64_t)kSlabs * n1 * sizeof(float)));
-    BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(float)));
+    BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(int32_t)));
   } else {
     BVH_TRY(o->rec64.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec64)));
-    BVH_TRY(o->leafbox64.ensure((uint64_t)kSlabs * n1 * sizeof(double)));
     BVH_TRY(o->rootbox64.ensure(kSlabs * sizeof(double)));
   }
   BuildBufs& b = o->bb;
@@ -292,11 +289,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.patch_root = o->patch_root.as<uint32_t>();
   b.root = o->root.as<uint32_t>();
   b.rec64 = o->fp32 ? nullptr : o->rec64.as<Rec64>();
-  b.leafbox64 = o->fp32 ? nullptr : o->leafbox64.as<double>();
   b.rootbox64 = o->fp32 ? nullptr : o->rootbox64.as<double>();
   b.rec32 = o->fp32 ? o->rec32.as<Rec32>() : nullptr;
-  b.leafbox32 = o->fp32 ? o->leafbox32.as<float>() : nullptr;
-  b.rootbox32 = o->fp32 ? o->rootbox32.as<float>() : nullptr;
+  b.rootbox32 = o->fp32 ? o->rootbox32.as<int32_t>() : nullptr;
   b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
   b.front_a = o->front_a.as<uint32_t>();
@@ -763,9 +758,14 @@ bvh_status bvh_object_root_kdop(bvh_object o, double* kdop) {
   if (o->in.n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
   if (o->fp32) {
+    int32_t ko[kSlabs];
     float f[kSlabs];
     double org[3];
-    BVH_TRY(copy_out(f, o->rootbox32.p, sizeof(f), BVH_MEM_HOST, s));
+    BVH_TRY(copy_out(ko, o->rootbox32.p, sizeof(ko), BVH_MEM_HOST, s));
+    for (int i = 0; i < kSlabs; ++i) {  // inverse of the ordered-integer image
+      const uint32_t u = ko[i] >= 0 ? (uint32_t)ko[i] : (0x80000000u | (uint32_t)(-(int64_t)ko[i]));
+      std::memcpy(&f[i], &u, 4);
+    }
     BVH_TRY(copy_out(org, o->w->frame.p, sizeof(org), BVH_MEM_HOST, s));
     static const double N[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},
                                     {0.5773502691896258, 0.5773502691896258, 0.5773502691896258},
@@ -1017,15 +1017,16 @@ bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
-  // the SoA leaf slabs, transposed to n x 26
+  // each leaf's slabs from its parent's record
   if (mem == BVH_MEM_DEVICE) {
-    HIP_TRY(launch_leaf_aos(o->leafbox64.as<double>(), n, kdops, s));
+    HIP_TRY(launch_sorted_kdops(o->rec64.as<Rec64>(), o->rootbox64.as<double>(), o->delta.as<uint8_t>(), n, kdops, s));
     HIP_TRY(hipStreamSynchronize(s));
     return BVH_OK;
   }
   DevBuf tmp;
   BVH_TRY(tmp.ensure(n * kSlabs * sizeof(double)));
-  HIP_TRY(launch_leaf_aos(o->leafbox64.as<double>(), n, tmp.as<double>(), s));
+  HIP_TRY(launch_sorted_kdops(o->rec64.as<Rec64>(), o->rootbox64.as<double>(), o->delta.as<uint8_t>(), n,
+                              tmp.as<double>(), s));
   return copy_out(kdops, tmp.p, n * kSlabs * sizeof(double), BVH_MEM_HOST, s);
 }
 

@@ -170,7 +170,7 @@ __device__ __forceinline__ uint32_t morton30(double x, double y, double z, const
 // fp32 form: 256 B, read with scalar loads; fp64 form: 448 B, staged through
 // LDS by a wave-wide vector load.
 struct alignas(256) Rec32 {
-  float box[2][kSlabs];  // left, right child (208 B)
+  int32_t box[2][kSlabs];  // left, right child: ordered images of fp32 slabs (208 B)
   uint32_t child[2];     // child node ids (leaf p = n-1+p)
   uint32_t elem[2];      // input index of a leaf child
   uint32_t first, last;  // this node's range of sorted positions
@@ -189,6 +189,17 @@ struct alignas(64) Rec64 {
 };
 static_assert(sizeof(Rec64) == 448, "record layout");
 
+// Where leaf p's slabs live: the half `side` of internal node `parent`'s
+// record.  Leaf p is the left child of internal node p when delta(p) <
+// delta(p-1) (or p == 0), else the right child of internal node p-1 (the
+// bottom-up build's rule for a one-position node).  Requires n >= 2.
+__device__ __forceinline__ void leaf_slot(const uint8_t* delta, uint64_t n, uint64_t p, uint32_t& parent,
+                                          int& side) {
+  const bool is_left = (p == 0) || (p != n - 1 && delta[p] < delta[p - 1]);
+  parent = (uint32_t)(is_left ? p : p - 1);
+  side = is_left ? 0 : 1;
+}
+
 template <bool FP32>
 struct Form {
   using T = double;
@@ -196,7 +207,7 @@ struct Form {
 };
 template <>
 struct Form<true> {
-  using T = float;
+  using T = int32_t;  // ord32 image of an outward-rounded fp32 slab
   using Rec = Rec32;
 };
 
@@ -290,6 +301,24 @@ __device__ __forceinline__ void element_kdop_any(const ElemInput& in, uint32_t e
 
 // fp32 outward slabs relative to the object origin: the fp64 slab minus the
 // origin's projection (fp64), rounded to nearest fp32 and moved one ulp out.
+// Ordered-integer image of a float: monotone, consecutive floats differ by
+// 1, ord(-0) = ord(+0) = 0, so integer compares and min / max act as float
+// compares and min / max.  The fp32 form stores its slabs this way; a NaN
+// lower slab becomes INT32_MAX and a NaN upper slab INT32_MIN, so that (like
+// the fp64 compare with NaN) no overlap test involving it holds.
+__device__ __forceinline__ int32_t ord32(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u >> 31) ? -(int32_t)(u & 0x7fffffffu) : (int32_t)u;
+}
+__device__ __forceinline__ void ord32_slabs(const float* kf, int32_t* ko) {
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    const float lo = kf[2 * a], hi = kf[2 * a + 1];
+    ko[2 * a] = (lo == lo) ? ord32(lo) : 0x7fffffff;
+    ko[2 * a + 1] = (hi == hi) ? ord32(hi) : (int32_t)0x80000000;
+  }
+}
+
 __device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {
 #pragma unroll
   for (int a = 0; a < kAxes; ++a) {

@@ -74,11 +74,9 @@ struct BuildBufs {
   uint32_t* patch_root;    // od
   uint32_t* root;          // 1
   Rec64* rec64;            // fp64 form: n-1 child-pair records
-  double* leafbox64;       // fp64 form: leaf slabs, SoA [26][n] in sorted order
   double* rootbox64;       // fp64 form: the root's slabs (26)
   Rec32* rec32;            // fp32 form: n-1 child-pair records
-  float* leafbox32;        // fp32 form: leaf slabs, SoA [26][n] in sorted order
-  float* rootbox32;        // fp32 form: the root's slabs (26)
+  int32_t* rootbox32;      // fp32 form: the root's slabs (26, ord32 images)
   double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
   uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
@@ -103,18 +101,19 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
 hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s);
 // patch_k64 set to the empty k-DOP before the leaf kernel
 hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hipStream_t s);
-// sorted leaf slabs SoA [26][n] -> n x 26
-hipError_t launch_leaf_aos(const double* soa, uint64_t n, double* out, hipStream_t s);
+// fp64 form: the leaves' slabs in sorted order (n x 26), read from their
+// parents' records
+hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
+                               hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
 
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {
   const Rec64* rec64;
-  const double* leafbox64;
   const double* rootbox64;
   const Rec32* rec32;
-  const float* leafbox32;
-  const float* rootbox32;
+  const int32_t* rootbox32;
+  const uint8_t* delta;  // split heights (locate a leaf's slabs in its parent's record)
   const uint32_t* root;
   const uint32_t* patch_root;
   const uint32_t* patch_of_input;

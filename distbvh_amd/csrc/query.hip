@@ -98,9 +98,11 @@ __device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const P
 // A leaf that passes the fp32 test is a candidate.  The rounding is also
 // bounded from the inside: a leaf slab f = next_down(rn(fl64(x - P))) has
 // x - P < next_up(next_up(f)) (rn moves by at most half an ulp and fl64 by far
-// less), and likewise for upper slabs.  In the ordered-integer image ord() of
-// the floats, where next_up adds one, ord(f(t_hi)) - ord(f(q_lo)) > 6 on a
-// slab pair therefore proves q_lo < t_hi in fp64.  Candidates proven on every
+// less), and likewise for upper slabs.  The records hold the ordered-integer
+// images ord32() of the fp32 slabs, where next_up adds one and integer
+// compares order like float compares, so ord(f(t_hi)) - ord(f(q_lo)) > 6 on
+// a slab pair proves q_lo < t_hi in fp64, and both the overlap test and this
+// bound are integer compares against scalar registers.  Candidates proven on every
 // slab are emitted; the rest go to a re-check list whose pairs are tested on
 // fp64 k-DOPs recomputed from element data (k_recheck).  The pairs are
 // exactly those of the fp64 form.
@@ -110,24 +112,29 @@ __device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const P
 // leaf children at once and pushes the internal children some lane overlaps
 // on a per-wave LDS stack (left on top: depth-first order).
 typedef __attribute__((address_space(4))) const Rec32 CRec32;
-typedef __attribute__((address_space(4))) const float CFloat;
+typedef __attribute__((address_space(4))) const int32_t CInt;
 
-// ordered-integer image of a float: monotone, consecutive floats differ by 1,
-// ord(-0) = ord(+0) = 0
-__device__ __forceinline__ int32_t ord_f32(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u >> 31) ? -(int32_t)(u & 0x7fffffffu) : (int32_t)u;
-}
 constexpr int32_t kInnerUlps = 6;
+
 // stack entries per wave: above the height of any radix tree over the
 // augmented keys (14 patch bits + 63 code bits + 30 position bits)
 constexpr uint32_t kStack = 128;
 
-__device__ __forceinline__ bool overlap_c32(const float* q, CFloat* box) {
-  bool o = true;
+
+// Smallest gap over the 13 slab pairs between query q and box b, in ordered
+// fp32 images: min(b_hi - q_lo, q_hi - b_lo) with saturating subtraction
+// (which keeps every sign).  The boxes overlap strictly iff the gap is > 0,
+// and the fp64 overlap of the exact slabs is proven iff it is > kInnerUlps.
+template <typename P>
+__device__ __forceinline__ int32_t min_gap(const int32_t* q, P b) {
+  int32_t m = 0x7fffffff;
 #pragma unroll
-  for (int i = 0; i < kAxes; ++i) o = o & (q[2 * i] < box[2 * i + 1]) & (box[2 * i] < q[2 * i + 1]);
-  return o;
+  for (int a = 0; a < kAxes; ++a) {
+    const int32_t d1 = __builtin_elementwise_sub_sat(b[2 * a + 1], q[2 * a]);
+    const int32_t d2 = __builtin_elementwise_sub_sat(q[2 * a + 1], b[2 * a]);
+    m = min(m, min(d1, d2));
+  }
+  return m;
 }
 
 template <bool SELF>
@@ -140,49 +147,51 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
-  float qf[kSlabs];
-  int32_t qin[kSlabs];  // ord(qf_lo) + kInnerUlps, ord(qf_hi) - kInnerUlps
+  int32_t qf[kSlabs];  // ord32 images of the query's outward fp32 slabs
   uint32_t qa = 0;
   if (valid) {
+    // this leaf's slabs: its half of its parent's record (the root's array if n = 1)
+    const int32_t* src = q.rootbox32;
+    if (q.n > 1) {
+      uint32_t parent;
+      int side;
+      leaf_slot(q.delta, q.n, p, parent, side);
+      src = q.rec32[parent].box[side];
+    }
 #pragma unroll
-    for (int i = 0; i < kSlabs; ++i) qf[i] = q.leafbox32[(uint64_t)i * q.n + p];
+    for (int a = 0; a < kAxes; ++a) {
+      const int2 v = ((const int2*)src)[a];
+      qf[2 * a] = v.x;
+      qf[2 * a + 1] = v.y;
+    }
     qa = q.perm[p];
   } else {
 #pragma unroll
-    for (int a = 0; a < kAxes; ++a) {
-      qf[2 * a] = __builtin_inff();
-      qf[2 * a + 1] = -__builtin_inff();
+    for (int a = 0; a < kAxes; ++a) {  // the empty box overlaps nothing
+      qf[2 * a] = 0x7fffffff;
+      qf[2 * a + 1] = (int32_t)0x80000000;
     }
-  }
-#pragma unroll
-  for (int a = 0; a < kAxes; ++a) {
-    // (wrapping arithmetic: only NaN images come near the int32 range ends,
-    // and NaN slabs fail the fp32 test before this bound is used)
-    qin[2 * a] = (int32_t)((uint32_t)ord_f32(qf[2 * a]) + (uint32_t)kInnerUlps);
-    qin[2 * a + 1] = (int32_t)((uint32_t)ord_f32(qf[2 * a + 1]) - (uint32_t)kInnerUlps);
   }
   const uint32_t n = (uint32_t)t.n;
   const uint32_t tleaf0 = n - 1;
+  const uint32_t lane = lane_id();
   CRec32* recs = (CRec32*)t.rec32;
   uint32_t steps = 0;
   uint32_t ncand = 0;
-  // fp32 candidates at one (wave-uniform) leaf: emit what the inner bound
-  // proves, queue the rest for the fp64 re-check
-  auto leaf_check = [&](CFloat* box, uint32_t elem, bool h) {
+  // fp32 candidates at one (wave-uniform) leaf, gap g per lane: emit what
+  // the inner bound proves, queue the rest for the fp64 re-check
+  auto leaf_check = [&](int32_t g, uint32_t elem, bool h) {
     const uint64_t m = __ballot(h);
     if (m == 0) return;
     ncand += (uint32_t)__popcll(m);
-    bool sure = true;
-#pragma unroll
-    for (int a = 0; a < kAxes; ++a)
-      sure = sure && (qin[2 * a] < ord_f32(box[2 * a + 1])) && (ord_f32(box[2 * a]) < qin[2 * a + 1]);
+    const bool sure = h & (g > kInnerUlps);
     const uint64_t key = pair_key(qa, elem, SELF);
-    wave_emit(buf, fill, out, h && sure, key);
-    const bool unsure = h && !sure;
+    wave_emit(buf, fill, out, sure, key);
+    const bool unsure = h & !sure;
     const uint64_t mu = __ballot(unsure);
     if (mu != 0) {
       unsigned long long base = 0;
-      if (lane_id() == 0) base = atomicAdd(out.rcount, (unsigned long long)__popcll(mu));
+      if (lane == 0) base = atomicAdd(out.rcount, (unsigned long long)__popcll(mu));
       const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
       const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
       base = ((unsigned long long)hi << 32) | lo;
@@ -192,51 +201,69 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
       }
     }
   };
+  // the node to visit next (kEnd: pop the stack)
+  uint32_t x = kEnd;
   uint32_t sp = 0;
   {
-    CFloat* rb = (CFloat*)t.rootbox32;
-    bool h = overlap_c32(qf, rb);
+    const int32_t g = min_gap(qf, (CInt*)t.rootbox32);
+    bool h = g > 0;
     if (SELF) h = h && (uint64_t)(n - 1) > p;  // the root's last position
     if (n == 1) {
-      if (!SELF) leaf_check(rb, uniform(t.perm[0]), h);
+      if (!SELF) leaf_check(g, uniform(t.perm[0]), h);
     } else if (__ballot(h) != 0) {
-      stk[0] = uniform(*t.root);
-      sp = 1;
+      x = uniform(*t.root);
     }
   }
-  while (sp > 0) {
-    --sp;
-    const uint32_t x = uniform(stk[sp]);
+  for (;;) {
+    if (x == kEnd) {
+      if (sp == 0) break;
+      x = uniform(stk[--sp]);
+    }
     if (x >= tleaf0 || ++steps > n) {
-      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
+      if (lane == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
-    CRec32* R = recs + x;
-    const uint32_t cl = R->child[0], cr = R->child[1];
-    bool hl = overlap_c32(qf, R->box[0]);
-    bool hr = overlap_c32(qf, R->box[1]);
+    // the record's first 224 bytes (both boxes, child ids, leaf elements)
+    // and `last`: every scalar load issued before any is used, one memory
+    // round trip per visit
+    CInt* rp = (CInt*)(recs + x);
+    int32_t rw[56];
+#pragma unroll
+    for (int k = 0; k < 56; ++k) rw[k] = rp[k];
+    const uint32_t rlast = SELF ? recs[x].last : 0u;
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t cl = (uint32_t)rw[52], cr = (uint32_t)rw[53];
+    const uint32_t el = (uint32_t)rw[54], er = (uint32_t)rw[55];
+    const int32_t gl = min_gap(qf, rw);
+    const int32_t gr = min_gap(qf, rw + kSlabs);
+    bool hl = gl > 0, hr = gr > 0;
     if (SELF) {
-      hl = hl && (uint64_t)x > p;        // the left child ends at position x
-      hr = hr && (uint64_t)R->last > p;
+      hl = hl && (uint64_t)x > p;  // the left child ends at position x
+      hr = hr && (uint64_t)rlast > p;
     }
-    bool push_l = false, push_r = false;
+    bool go_l = false, go_r = false;
     if (__ballot(hl) != 0) {
-      if (cl >= tleaf0) leaf_check(R->box[0], R->elem[0], SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
-      else push_l = true;
+      if (cl >= tleaf0) leaf_check(gl, el, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
+      else go_l = true;
     }
     if (__ballot(hr) != 0) {
-      if (cr >= tleaf0) leaf_check(R->box[1], R->elem[1], SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
-      else push_r = true;
+      if (cr >= tleaf0) leaf_check(gr, er, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
+      else go_r = true;
     }
-    if (sp + (push_l ? 1u : 0u) + (push_r ? 1u : 0u) > kStack) {
-      if (lane_id() == 0) atomicOr(t.status, kErrTraverse);
-      break;
+    // depth first: continue with the left child, keep the right one
+    if (go_l && go_r) {
+      if (sp >= kStack) {
+        if (lane == 0) atomicOr(t.status, kErrTraverse);
+        break;
+      }
+      stk[sp++] = cr;
+      x = cl;
+    } else {
+      x = go_l ? cl : (go_r ? cr : kEnd);
     }
-    if (push_r) stk[sp++] = cr;
-    if (push_l) stk[sp++] = cl;
   }
   wave_flush(buf, fill, out);
-  if (lane_id() == 0) {
+  if (lane == 0) {
     if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
     if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
   }
@@ -262,8 +289,20 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
   double qk[kSlabs];
   uint32_t qa = 0;
   if (valid) {
+    // this leaf's slabs: its half of its parent's record (the root's array if n = 1)
+    const double* src = q.rootbox64;
+    if (q.n > 1) {
+      uint32_t parent;
+      int side;
+      leaf_slot(q.delta, q.n, p, parent, side);
+      src = q.rec64[parent].box[side];
+    }
 #pragma unroll
-    for (int i = 0; i < kSlabs; ++i) qk[i] = q.leafbox64[(uint64_t)i * q.n + p];
+    for (int a = 0; a < kAxes; ++a) {
+      const double2 v = ((const double2*)src)[a];
+      qk[2 * a] = v.x;
+      qk[2 * a + 1] = v.y;
+    }
     qa = q.perm[p];
   } else {
     empty_slabs(qk);
@@ -272,6 +311,7 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
   const uint32_t tleaf0 = n - 1;
   uint32_t steps = 0;
   uint32_t sp = 0;
+  uint32_t x = kEnd;  // the node to visit next (kEnd: pop the stack)
   {
     // the root's slabs through scalar loads (wave-uniform)
     const __attribute__((address_space(4))) double* rb = (const __attribute__((address_space(4))) double*)t.rootbox64;
@@ -282,14 +322,15 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
     if (n == 1) {
       if (!SELF) wave_emit(buf, fill, out, h, pair_key(qa, uniform(t.perm[0]), SELF));
     } else if (__ballot(h) != 0) {
-      stk[0] = uniform(*t.root);
-      sp = 1;
+      x = uniform(*t.root);
     }
   }
   const uint32_t lane = lane_id();
-  while (sp > 0) {
-    --sp;
-    const uint32_t x = uniform(stk[sp]);
+  for (;;) {
+    if (x == kEnd) {
+      if (sp == 0) break;
+      x = uniform(stk[--sp]);
+    }
     if (x >= tleaf0 || ++steps > n) {
       if (lane == 0) atomicOr(t.status, kErrTraverse);
       break;
@@ -309,21 +350,26 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
       hl = hl && (uint64_t)x > p;  // the left child ends at position x
       hr = hr && (uint64_t)rlast > p;
     }
-    bool push_l = false, push_r = false;
+    bool go_l = false, go_r = false;
     if (__ballot(hl) != 0) {
       if (cl >= tleaf0) wave_emit(buf, fill, out, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl, pair_key(qa, el, SELF));
-      else push_l = true;
+      else go_l = true;
     }
     if (__ballot(hr) != 0) {
       if (cr >= tleaf0) wave_emit(buf, fill, out, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr, pair_key(qa, er, SELF));
-      else push_r = true;
+      else go_r = true;
     }
-    if (sp + (push_l ? 1u : 0u) + (push_r ? 1u : 0u) > kStack) {
-      if (lane == 0) atomicOr(t.status, kErrTraverse);
-      break;
+    // depth first: continue with the left child, keep the right one
+    if (go_l && go_r) {
+      if (sp >= kStack) {
+        if (lane == 0) atomicOr(t.status, kErrTraverse);
+        break;
+      }
+      stk[sp++] = cr;
+      x = cl;
+    } else {
+      x = go_l ? cl : (go_r ? cr : kEnd);
     }
-    if (push_r) stk[sp++] = cr;
-    if (push_l) stk[sp++] = cl;
   }
   wave_flush(buf, fill, out);
   if (lane == 0 && out.visits) atomicAdd(out.visits, (unsigned long long)steps);
@@ -460,7 +506,8 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   const unsigned blocks = (unsigned)((q.n + bs - 1) / bs);
   const int xcd = shape.xcd_remap ? 1 : 0;
   if (fp32) {
-    if (!q.leafbox32 || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin) return hipErrorInvalidValue;
+    if (!q.rec32 || !q.rootbox32 || !q.delta || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin)
+      return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
     if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
@@ -469,7 +516,7 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     hipLaunchKernelGGL(k_recheck, dim3(512), dim3(256), 0, s, q, t, out);
     return hipGetLastError();
   }
-  if (!q.leafbox64 || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
+  if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
   if (self) hipLaunchKernelGGL(k_narrow64<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
   else hipLaunchKernelGGL(k_narrow64<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
   return hipGetLastError();
