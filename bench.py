@@ -245,6 +245,7 @@ def bench_batch(args, torch, stream):
 
     ms_graph, _, _, _, npairs = run_steps(world, replay, args.steps, args.warmup, stream, torch)
     ok = npairs == npairs_eager and o.digest() == digest_eager
+    st = o.result_stats()
     g.close()
     world.close()
     return {"metric": "ms/step batch build + self collision, 4096 meshes (%d triangles), one HIP graph" % n,
@@ -259,6 +260,8 @@ def bench_batch(args, torch, stream):
                                      "query_ms": round(tune["ms"], 4),
                                      "tried_ms": [round(t, 4) for t in tune["tried_ms"]]},
                        "pairs": int(npairs), "pairs_per_s": round(npairs / (ms_graph * 1e-3), 1),
+                       "candidate_pairs": int(st["candidate_pairs"]), "rechecked_pairs": int(st["rechecked_pairs"]),
+                       "node_visits_per_wave": round(st["node_visits"] / ((n + 63) // 64), 1),
                        "eager_ms_per_step": round(ms_eager, 4), "graph_pairs_equal_eager": bool(ok),
                        "parallelism": "single GPU"},
             "cpu_baseline": None}

@@ -530,7 +530,7 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   BVH_TRY(o->ppairs.ensure(std::max<uint64_t>(1024, 4 * w->od) * 8));
   if (narrow) {
     BVH_TRY(ensure_pairs(o, std::max<uint64_t>(1u << 20, 8 * o->in.n)));
-    if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n / 8) * 8));
+    if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n) * 8));
   }
   if (w->capturing) {
     BVH_TRY(launch_query(o, other, self, narrow));

@@ -301,6 +301,69 @@ __device__ __forceinline__ void element_kdop_any(const ElemInput& in, uint32_t e
 
 // fp32 outward slabs relative to the object origin: the fp64 slab minus the
 // origin's projection (fp64), rounded to nearest fp32 and moved one ulp out.
+// Strict overlap of element e (from element data) with the k-DOP kd, one
+// axis at a time so that e's 26 slabs are never live together.  Each slab is
+// computed with the operations and order of element_kdop, so the result is
+// overlap(kd, element_kdop(e)).
+template <int KIND, int NV, int A = 0>
+__device__ __forceinline__ bool overlap_axes(const double (&v)[NV][3], double eps, const double* kd) {
+  double lo, hi;
+  const double p0 = project<A>(v[0][0], v[0][1], v[0][2]);
+  lo = p0 - eps;
+  hi = p0 + eps;
+  if constexpr (KIND == kPoly) {
+#pragma unroll
+    for (int j = 1; j < NV; ++j) {
+      const double pj = project<A>(v[j][0], v[j][1], v[j][2]);
+      lo = sel_min(lo, pj - eps);
+      hi = sel_max(hi, pj + eps);
+    }
+  }
+  const bool o = (kd[2 * A] < hi) & (lo < kd[2 * A + 1]);
+  if constexpr (A + 1 < kAxes) return o & overlap_axes<KIND, NV, A + 1>(v, eps, kd);
+  else return o;
+}
+
+template <int KIND, int NV>
+__device__ __forceinline__ bool element_overlap(const ElemInput& in, uint32_t e, const double* kd) {
+  const uint64_t n = in.n;
+  if constexpr (KIND == kPoly) {
+    double v[NV][3];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[j][c] = in.data[(uint64_t)(3 * j + c) * n + e];
+    return overlap_axes<kPoly, NV>(v, in.eps, kd);
+  } else if constexpr (KIND == kSphere) {
+    const double v[1][3] = {{in.data[e], in.data[n + e], in.data[2 * n + e]}};
+    return overlap_axes<kSphere, 1>(v, in.data[3 * n + e], kd);
+  } else {
+    const double2* k = (const double2*)(in.data + (uint64_t)e * kSlabs);
+    bool o = true;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      const double2 t = k[a];
+      o = o & (kd[2 * a] < t.y) & (t.x < kd[2 * a + 1]);
+    }
+    return o;
+  }
+}
+
+__device__ __forceinline__ bool element_overlap_any(const ElemInput& in, uint32_t e, const double* kd) {
+  if (in.kind == kSphere) return element_overlap<kSphere, 1>(in, e, kd);
+  if (in.kind == kSnapshot) return element_overlap<kSnapshot, 1>(in, e, kd);
+  switch (in.nv) {
+    case 1: return element_overlap<kPoly, 1>(in, e, kd);
+    case 2: return element_overlap<kPoly, 2>(in, e, kd);
+    case 3: return element_overlap<kPoly, 3>(in, e, kd);
+    case 4: return element_overlap<kPoly, 4>(in, e, kd);
+    case 5: return element_overlap<kPoly, 5>(in, e, kd);
+    case 6: return element_overlap<kPoly, 6>(in, e, kd);
+    case 7: return element_overlap<kPoly, 7>(in, e, kd);
+    default: return element_overlap<kPoly, 8>(in, e, kd);
+  }
+}
+
 // Ordered-integer image of a float: monotone, consecutive floats differ by
 // 1, ord(-0) = ord(+0) = 0, so integer compares and min / max act as float
 // compares and min / max.  The fp32 form stores its slabs this way; a NaN

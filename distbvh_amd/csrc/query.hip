@@ -137,13 +137,34 @@ __device__ __forceinline__ int32_t min_gap(const int32_t* q, P b) {
   return m;
 }
 
+// Per-wave LDS staging of the candidates queued for the fp64 re-check: one
+// atomic on the list's counter per flush (mesh self collision queues every
+// pair of elements that share a vertex, so these are many).
+constexpr uint32_t kRecheckBuf = 128;
+
+__device__ __forceinline__ void recheck_flush(uint64_t* buf, uint32_t& fill, const PairOut& out) {
+  if (fill == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(out.rcount, (unsigned long long)fill);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+  base = ((unsigned long long)hi << 32) | lo;
+  for (uint32_t i = lane_id(); i < fill; i += kWave)
+    if (base + i < out.rcap) out.recheck[base + i] = buf[i];
+  __builtin_amdgcn_wave_barrier();
+  fill = 0;
+}
+
 template <bool SELF>
 __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xcd, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint64_t s_rbuf[256 / kWave][kRecheckBuf];
   __shared__ uint32_t s_stack[256 / kWave][kStack];
   uint64_t* buf = s_buf[threadIdx.x / kWave];
+  uint64_t* rbuf = s_rbuf[threadIdx.x / kWave];
   uint32_t* stk = s_stack[threadIdx.x / kWave];
-  uint32_t fill = 0;
+  uint32_t fill = 0, rfill = 0;
   const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
@@ -190,15 +211,10 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     const bool unsure = h & !sure;
     const uint64_t mu = __ballot(unsure);
     if (mu != 0) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(out.rcount, (unsigned long long)__popcll(mu));
-      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
-      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
-      base = ((unsigned long long)hi << 32) | lo;
-      if (unsure) {
-        const unsigned long long idx = base + (unsigned long long)__popcll(mu & lanemask_lt());
-        if (idx < out.rcap) out.recheck[idx] = key;
-      }
+      const uint32_t cnt = (uint32_t)__popcll(mu);
+      if (rfill + cnt > kRecheckBuf) recheck_flush(rbuf, rfill, out);
+      if (unsure) rbuf[rfill + (uint32_t)__popcll(mu & lanemask_lt())] = key;
+      rfill += cnt;
     }
   };
   // the node to visit next (kEnd: pop the stack)
@@ -263,6 +279,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     }
   }
   wave_flush(buf, fill, out);
+  recheck_flush(rbuf, rfill, out);
   if (lane == 0) {
     if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
     if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
@@ -379,6 +396,11 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
 // elements' k-DOPs recomputed from element data, the pair emitted on overlap.
 // The list length is read on the device, so no host round trip separates the
 // two kernels.
+// KIND / NV: both objects' element type (the common case, with registers
+// sized for it); kGeneric: any mix, chosen at run time.
+constexpr int kGeneric = -1;
+
+template <int KIND, int NV>
 __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut out) {
   const unsigned long long cnt0 = *out.rcount;
   const uint64_t cnt = cnt0 < out.rcap ? cnt0 : out.rcap;
@@ -389,13 +411,40 @@ __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut
     uint64_t key = 0;
     if (i < cnt) {
       key = out.recheck[i];
-      double ka[kSlabs], kb[kSlabs];
-      element_kdop_any(q.elem, (uint32_t)(key >> 32), ka);
-      element_kdop_any(t.elem, (uint32_t)key, kb);
-      hit = overlap(ka, kb);
+      double ka[kSlabs];
+      if constexpr (KIND == kGeneric) {
+        element_kdop_any(q.elem, (uint32_t)(key >> 32), ka);
+        hit = element_overlap_any(t.elem, (uint32_t)key, ka);
+      } else {
+        element_kdop<KIND, NV>(q.elem, (uint32_t)(key >> 32), ka);
+        hit = element_overlap<KIND, NV>(t.elem, (uint32_t)key, ka);
+      }
     }
     emit(out, hit, key);
   }
+}
+
+static void launch_recheck(const TreeView& q, const TreeView& t, const PairOut& out, unsigned blocks, hipStream_t s) {
+  const ElemInput& a = q.elem;
+  const ElemInput& b = t.elem;
+  const dim3 g(blocks), th(256);
+  if (a.kind == b.kind && (a.kind != kPoly || a.nv == b.nv)) {
+    if (a.kind == kSphere) {
+      hipLaunchKernelGGL((k_recheck<kSphere, 1>), g, th, 0, s, q, t, out);
+      return;
+    }
+    if (a.kind == kSnapshot) {
+      hipLaunchKernelGGL((k_recheck<kSnapshot, 1>), g, th, 0, s, q, t, out);
+      return;
+    }
+    switch (a.nv) {
+      case 3: hipLaunchKernelGGL((k_recheck<kPoly, 3>), g, th, 0, s, q, t, out); return;
+      case 4: hipLaunchKernelGGL((k_recheck<kPoly, 4>), g, th, 0, s, q, t, out); return;
+      case 8: hipLaunchKernelGGL((k_recheck<kPoly, 8>), g, th, 0, s, q, t, out); return;
+      default: break;
+    }
+  }
+  hipLaunchKernelGGL((k_recheck<kGeneric, 0>), g, th, 0, s, q, t, out);
 }
 
 // exact patch slabs (the union of the patch's element k-DOPs)
@@ -513,7 +562,9 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     // in self mode both keys index the one object (q == t)
-    hipLaunchKernelGGL(k_recheck, dim3(512), dim3(256), 0, s, q, t, out);
+    uint64_t rblocks = (out.rcap + 255) / 256;
+    if (rblocks > 8192) rblocks = 8192;
+    launch_recheck(q, t, out, (unsigned)rblocks, s);
     return hipGetLastError();
   }
   if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
