@@ -30,13 +30,16 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(stage, n, npairs, n_other, fp32=False):
+def kernel_bytes(stage, n, npairs, n_other, fp32=False, elem_bytes=72):
     """Algorithmic (compulsory) HBM bytes of one launch of a stage for an
-    object of n triangles (3 fp64 vertices = 72 B).  Node slabs are 13
-    (min, max) pairs: 208 B in fp64, 104 B in the fp32 form."""
-    SLAB_BYTES = 104 if fp32 else 208
-    if stage == "bounds":      # read vertices, write centroid
-        return n * (72 + 24)
+    object of n elements (a triangle: 3 fp64 vertices = 72 B).  A node's 13
+    (min, max) slab pairs take 208 B in fp64 and 104 B in the fp32 form; the
+    tree is one record per internal node holding both children's slabs plus
+    16 B of links (448 B fp64, 256 B fp32 with padding)."""
+    slab = 104 if fp32 else 208
+    rec = 256 if fp32 else 448
+    if stage == "bounds":      # read elements, write centroid
+        return n * (elem_bytes + 24)
     if stage == "morton":      # read centroid, write code
         return n * (24 + 8)
     if stage == "sort":        # 8 passes x (read + write 8 B key + 4 B index)
@@ -45,16 +48,35 @@ def kernel_bytes(stage, n, npairs, n_other, fp32=False):
         return n * (8 + 4 + 16)
     if stage == "delta":       # read codes, write patch id + split height
         return n * (8 + 4 + 1 + 4)
-    if stage == "tree":        # leaves: gather vertices, write slabs; internal:
-        # read two children's slabs, write one, links + counters
-        return n * (4 + 4 + 72 + SLAB_BYTES + 4) + (n - 1) * (2 * SLAB_BYTES + SLAB_BYTES + 12)
-    if stage == "ropes":
-        return (2 * n - 1) * 12
-    if stage == "narrowphase":  # query slabs once, other tree once, pairs
-        return n * (SLAB_BYTES + 4) + (2 * n_other - 1) * (SLAB_BYTES + 16) + 8 * npairs
+    if stage == "tree":        # gather elements + permutation + patch ids, write
+        # each node's slabs into its parent's record + the parent's links
+        return n * (elem_bytes + 4 + 4 + 1 + slab + 16)
+    if stage == "narrowphase":  # query slabs (from their parents' records), the
+        # other object's records once, pair keys
+        return n * (slab + 4 + 2) + (n_other - 1) * rec + 8 * npairs
     if stage == "broadphase":
         return 0
     return 0
+
+
+# stage -> the kernel whose measured HBM traffic the roofline reports
+STAGE_KERNEL = {"narrowphase": ("k_narrow64<false>", "k_narrow32<false>"),
+                "tree": ("k_tree_leaves<0, 3, false>", "k_tree_leaves<0, 3, true>"),
+                "sort": ("k_sort_pass<unsigned long>", "k_sort_pass<unsigned long>")}
+
+
+def measured_traffic(stage, workload, fp32):
+    """HBM bytes per launch of the stage's main kernel from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE table under profiles/ (None if absent)."""
+    name = STAGE_KERNEL.get(stage, (None, None))[1 if fp32 else 0]
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (workload, "fp32" if fp32 else "fp64"))
+    if name is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.rstrip("\n").split("\t")
+        if len(f) == 3 and f[0] == name:
+            return int(f[1]) + int(f[2])
+    return None
 
 
 def cpu_baseline(a, b, od, leaf):
@@ -117,14 +139,15 @@ def run_steps(world, step, steps, warmup, stream, torch):
     return total_ms / steps, stages, per_launch, counts, ret
 
 
-def roofline_of(stages, per_launch, counts, nbytes_fn, ms_per_step):
+def roofline_of(stages, per_launch, counts, nbytes_fn, ms_per_step, traffic_fn=None):
     dom = max(stages, key=lambda k: stages[k])
     nbytes = nbytes_fn(dom)
     t_launch = per_launch[dom] * 1e-3
     achieved = nbytes / t_launch / 1e9 if t_launch > 0 else 0.0
+    traffic = traffic_fn(dom) if traffic_fn else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": int(nbytes)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": int(nbytes), "launch_ms": round(per_launch[dom], 4)}
     step_bytes = sum(nbytes_fn(k) * counts[k] for k in stages)
     step_frac = step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     return roofline, round(step_frac, 4)
@@ -157,8 +180,10 @@ def bench_sheets(args, torch, stream):
 
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = oa.result_stats()
+        wl = "wavy_sheets_2M" if args.nq == 1024 and args.od == 256 else None
         roof, step_frac = roofline_of(stages, per_launch, counts,
-                                      lambda k: kernel_bytes(k, n, npairs, n, fp32), ms)
+                                      lambda k: kernel_bytes(k, n, npairs, n, fp32), ms,
+                                      (lambda k: measured_traffic(k, wl, fp32)) if wl else None)
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
                      "roofline": roof, "step_hbm_frac": step_frac, "digest": oa.digest()}
         world.close()
@@ -194,10 +219,7 @@ def bench_spheres_self(args, torch, stream):
         def nb(k):
             if k == "bounds":  # read x, y, z, r; write centroid
                 return n * (32 + 24)
-            if k == "tree":
-                slab = 104 if fp32 else 208
-                return n * (4 + 4 + 32 + slab + 4) + (n - 1) * (3 * slab + 12)
-            return kernel_bytes(k, n, npairs, n, fp32)
+            return kernel_bytes(k, n, npairs, n, fp32, elem_bytes=32)
 
         roof, step_frac = roofline_of(stages, per_launch, counts, nb, ms)
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
