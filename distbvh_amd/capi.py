@@ -37,7 +37,7 @@ class Record(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("leaf_size", C.c_uint32), ("block_size", C.c_uint32), ("xcd_remap", C.c_uint32),
+    _fields_ = [("batch", C.c_uint32), ("block_size", C.c_uint32), ("xcd_remap", C.c_uint32),
                 ("tried", C.c_uint32), ("ms", C.c_double), ("tried_ms", C.c_double * 24)]
 
 
@@ -76,6 +76,8 @@ SIGNATURES = {
     "bvh_object_device_bytes": [_vp, C.POINTER(_u64)],
     "bvh_object_root_kdop": [_vp, _vp],
     "bvh_autotune": [_vp, _vp, _vp],
+    "bvh_set_launch": [_vp, _vp],
+    "bvh_get_launch": [_vp, _vp],
     "bvh_capture_begin": [_vp],
     "bvh_capture_end": [_vp, _vp],
     "bvh_graph_launch": [_vp],

@@ -90,6 +90,22 @@ class CollisionWorld:
     def synchronize(self):
         call("bvh_synchronize", self.handle)
 
+    def launch(self):
+        """bvh_get_launch: the narrowphase launch shape."""
+        t = capi.Tuning()
+        call("bvh_get_launch", self.handle, C.byref(t))
+        return {"batch": t.batch, "block_size": t.block_size, "xcd_remap": bool(t.xcd_remap)}
+
+    def set_launch(self, batch=None, block_size=None, xcd_remap=None):
+        """bvh_set_launch: change any of batch (1, 2, 4), block size (64,
+        128, 256) and XCD remap; the others keep their values."""
+        cur = self.launch()
+        t = capi.Tuning()
+        t.batch = cur["batch"] if batch is None else batch
+        t.block_size = cur["block_size"] if block_size is None else block_size
+        t.xcd_remap = int(cur["xcd_remap"] if xcd_remap is None else xcd_remap)
+        call("bvh_set_launch", self.handle, C.byref(t))
+
     def stage_times(self):
         names = (C.c_char_p * 64)()
         ms = (C.c_double * 64)()
@@ -213,11 +229,11 @@ class CollisionObject:
         return k
 
     def autotune(self, other=None):
-        """bvh_autotune: the fastest leaf size / block size / XCD remap for
-        the query self x other (self collision when other is None)."""
+        """bvh_autotune: the fastest batch / block size / XCD remap for the
+        query self x other (self collision when other is None)."""
         t = capi.Tuning()
         call("bvh_autotune", self.handle, (other or self).handle, C.byref(t))
-        return {"leaf_size": t.leaf_size, "block_size": t.block_size, "xcd_remap": bool(t.xcd_remap),
+        return {"batch": t.batch, "block_size": t.block_size, "xcd_remap": bool(t.xcd_remap),
                 "ms": t.ms, "tried_ms": list(t.tried_ms[:t.tried])}
 
     def init_broadphase(self):

@@ -334,25 +334,29 @@ __device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t o
 // ---- bottom-up build with refit -------------------------------------------------
 // A node [l, r] is the left child of internal node r when delta(r) <
 // delta(l-1) (or l == 0), else the right child of internal node l-1.  Each
-// leaf climbs; at a parent, an arrival counter decides: the first child to
-// arrive stops, the second merges both children's slabs, writes the parent and
-// continues.  Every synchronization is workgroup-local:
+// leaf climbs; at a parent, the first child to arrive stops and the second
+// merges both children's slabs, writes the parent and continues.  Every
+// synchronization is workgroup-local:
 //  * phase 1 (k_tree_leaves): a workgroup owns 256 consecutive leaves and
 //    counts arrivals in LDS.  A child whose parent splits exactly at a chunk
 //    boundary (at most the left child of internal node c1-1 and the right
 //    child of internal node c0-1) is handed to the next phase; a first arriver
-//    whose sibling extends past the chunk leaves its count in global memory.
+//    whose sibling extends past the chunk leaves its arrival word in global
+//    memory.
 //  * phase k >= 2 (k_tree_up): chunks 256x larger; a workgroup continues the
-//    handed-over nodes of its 256 sub-chunks with workgroup-scope atomics on
-//    the global counters of the splits inside its chunk (no other workgroup
-//    touches them in that launch; earlier launches are ordered by the kernel
-//    boundary).  The last phase is one chunk and finds the root.
+//    handed-over nodes of its 256 sub-chunks.  An arrival is one
+//    workgroup-scope atomic exchange on the parent's arrival word: the first
+//    arriver deposits its far end (l for a left child, r for a right child)
+//    and its patch, so the second learns the parent's range and patch from
+//    the exchange itself and reads only the two slab halves.  No other
+//    workgroup touches those words in that launch; earlier launches are
+//    ordered by the kernel boundary.  The last phase is one chunk and finds
+//    the root.
 //
 // Both forms store one record per internal node with both children's slabs
 // (Rec32 / Rec64).  Each node writes its slabs into its half of the parent's
 // record before it counts its arrival, and the second arriver reads the
-// sibling's half; the root's slabs go to a separate array, and every leaf's
-// slabs also to the SoA leaf array the queries read.
+// sibling's half; the root's slabs go to a separate array.
 
 template <bool FP32>
 using SlabT = typename Form<FP32>::T;
@@ -442,64 +446,61 @@ __device__ __forceinline__ bool is_patch_root(uint32_t patch, uint32_t l, uint32
   return patch != kMixed && (l == 0 || dl >= 128) && (r == n - 1 || dr >= 128);
 }
 
-// Global-memory climb (phases >= 2) from a finished node until it is the
-// first arrival at its parent, the root, or a child whose parent splits on
-// the chunk boundary [c0, c1).  `elem` is the node's input index if it is a
-// leaf, else kEnd.
+// Global-memory climb (phases >= 2).  Node [l, r] of patch `mypatch` has
+// written its slabs, id and element into its half of its parent's record; dl
+// = delta[l-1] and dr = delta[r] (0 at the ends).  It climbs until it is the
+// first arrival at a parent, finishes the root, or is a child whose parent
+// splits on the chunk boundary [c0, c1).
 template <bool FP32>
-__device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t node, uint32_t l, uint32_t r,
-                                      uint32_t mypatch, SlabT<FP32>* kd, uint32_t elem, uint32_t c0, uint32_t c1,
-                                      uint32_t* s_front, uint32_t* s_nfront) {
+__device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t l, uint32_t r, uint32_t mypatch,
+                                      uint32_t dl, uint32_t dr, uint32_t c0, uint32_t c1, FrontNode* s_front,
+                                      uint32_t* s_nfront) {
   RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
-    if (l == 0 && r == n - 1) {
-      *b.root = node;
-      store_rootbox<FP32>(b, kd);
-      return;
-    }
-    const uint32_t dr = (r != n - 1) ? b.delta[r] : 0u;
-    const uint32_t dl = (l != 0) ? b.delta[l - 1] : 0u;
     const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
-    rec_store_half<FP32>(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(s_nfront, 1u);
-      if (k < 2) s_front[k] = node;
+      if (k < 2) s_front[k] = FrontNode{l, r, mypatch, dl | (dr << 8)};
       else atomicOr(b.status, kErrTree);
       return;
     }
-    const uint32_t old =
-        __hip_atomic_fetch_add(&b.counters[parent], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const unsigned long long mine = ((unsigned long long)((is_left ? l : r) + 1u) << 32) | mypatch;
+    const unsigned long long old =
+        __hip_atomic_exchange(&b.counters[parent], mine, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) return;  // first arriver: the sibling finishes the parent
-    const uint32_t sib = recs[parent].child[is_left ? 1 : 0];
-    if (old != 1 || sib >= 2 * n - 1) {
+    const uint32_t far = (uint32_t)(old >> 32) - 1u, spatch = (uint32_t)old;
+    // a left child's sibling ends past r, a right child's starts before l
+    if (is_left ? (far <= r || far >= n) : (far >= l)) {
       atomicOr(b.status, kErrTree);
       return;
     }
-    SlabT<FP32> ks[kSlabs];
-    rec_load_half<FP32>(b, parent, is_left ? 1 : 0, ks);
-    uint32_t sfirst, slast, spatch;
-    if (sib >= n - 1) {
-      sfirst = slast = sib - (uint32_t)(n - 1);
-      spatch = b.patch_of_pos[sfirst];
+    SlabT<FP32> kd[kSlabs], ks[kSlabs];
+    rec_load_half<FP32>(b, parent, 0, kd);
+    rec_load_half<FP32>(b, parent, 1, ks);
+    if (is_left) {
+      r = far;
+      dr = (r != n - 1) ? b.delta[r] : 0u;
     } else {
-      sfirst = recs[sib].first;
-      slast = recs[sib].last;
-      spatch = recs[sib].patch;
+      l = far;
+      dl = l ? b.delta[l - 1] : 0u;
     }
     slab_union(kd, ks);
-    if (is_left) r = slast;
-    else l = sfirst;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
     recs[parent].first = l;
     recs[parent].last = r;
     recs[parent].patch = mypatch;
-    if (is_patch_root(mypatch, l, r, n, l ? b.delta[l - 1] : 0u, r != n - 1 ? b.delta[r] : 0u)) {
+    if (is_patch_root(mypatch, l, r, n, dl, dr)) {
       b.patch_root[mypatch] = parent;
       store_patch_box<FP32>(b, mypatch, kd);
     }
-    node = parent;
-    elem = kEnd;
+    if (l == 0 && r == n - 1) {
+      *b.root = parent;
+      store_rootbox<FP32>(b, kd);
+      return;
+    }
+    const bool up_left = (l == 0) || (r != n - 1 && dr < dl);
+    rec_store_half<FP32>(b, up_left ? r : l - 1, up_left ? 0 : 1, kd, parent, kEnd);
   }
 }
 
@@ -562,30 +563,32 @@ __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
 }
 
 // Phase-1 state of one 256-leaf chunk, all in LDS.  Slot j holds the newest
-// finished node whose range starts at position c0 + j (its slabs, node id,
-// last position, patch); endslot[j] is the slot of the newest finished node
-// that ends at c0 + j.  A parent is written to its first position's slot, so
-// a slot is only overwritten by the thread that has just read it.
+// finished node whose range starts at position c0 + j (its slabs, last
+// position, patch); endslot[j] is the slot of the newest finished node that
+// ends at c0 + j.  A parent is written to its first position's slot, so a
+// slot is only overwritten by the thread that has just read it.  far[j] and
+// fpatch[j] are the far end and patch of the first child to arrive at
+// internal node c0 + j.
 template <bool FP32>
 struct ChunkLds {
   SlabT<FP32> slab[kSlabs][kTreeChunk];  // SoA: conflict-free stores
   uint32_t cnt[kTreeChunk];      // arrivals at internal node c0 + j
-  uint32_t id[kTreeChunk];
+  uint32_t far[kTreeChunk];
+  uint32_t fpatch[kTreeChunk];
   uint32_t last[kTreeChunk];
   uint32_t patch[kTreeChunk];
   uint32_t endslot[kTreeChunk];
   uint8_t delta[kTreeChunk + 1];  // delta[c0 - 1 + j]
-  uint32_t front[2];
+  FrontNode front[2];
   uint32_t nfront;
   uint32_t err;
 };
 
 template <bool FP32>
 __device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32_t endslot, const SlabT<FP32>* kd,
-                                        uint32_t id, uint32_t last, uint32_t patch) {
+                                        uint32_t last, uint32_t patch) {
 #pragma unroll
   for (int i = 0; i < kSlabs; ++i) L.slab[i][slot] = kd[i];
-  L.id[slot] = id;
   L.last[slot] = last;
   L.patch[slot] = patch;
   L.endslot[endslot] = slot;
@@ -607,19 +610,26 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
       return;
     }
     // delta[r] = L.delta[r - c0 + 1], delta[l - 1] = L.delta[l - c0]
-    const bool is_left = (l == 0) || (r != n - 1 && L.delta[r - c0 + 1] < L.delta[l - c0]);
+    const uint32_t dl = L.delta[l - c0], dr = L.delta[r - c0 + 1];  // delta[l - 1], delta[r]
+    const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
     rec_store_half<FP32>(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(&L.nfront, 1u);
-      if (k < 2) L.front[k] = node;
+      if (k < 2) L.front[k] = FrontNode{l, r, mypatch, (l ? dl : 0u) | ((r != n - 1 ? dr : 0u) << 8)};
       else atomicOr(b.status, kErrTree);
       return;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     const uint32_t old = __hip_atomic_fetch_add(&L.cnt[parent - c0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    if (old == 0) return;  // first arriver: the sibling finishes the parent
+    if (old == 0) {
+      // first arriver: the sibling finishes the parent, in this phase or,
+      // through the parent's arrival word, in a later one
+      L.far[parent - c0] = is_left ? l : r;
+      L.fpatch[parent - c0] = mypatch;
+      return;
+    }
     if (old != 1) {
       atomicOr(b.status, kErrTree);
       return;
@@ -645,7 +655,7 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
       b.patch_root[mypatch] = parent;
       store_patch_box<FP32>(b, mypatch, kd);
     }
-    publish<FP32>(L, l - c0, r - c0, kd, parent, r, mypatch);
+    publish<FP32>(L, l - c0, r - c0, kd, r, mypatch);
     node = parent;
     elem = kEnd;
   }
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
       if (single) store_patch_box<false>(b, pk, kd);
     }
-    publish<FP32>(L, t, t, ks, leaf, p, pk);
+    publish<FP32>(L, t, t, ks, p, pk);
   }
   __syncthreads();  // counters, deltas and leaf slots ready
   if (act) {
@@ -726,19 +736,20 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   }
   __syncthreads();
   if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);
-  // a first arrival whose sibling lies outside the chunk keeps its count
-  if (c0 + t + 1 < c1 && L.cnt[t] == 1) b.counters[c0 + t] = 1;
-  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < L.nfront) ? L.front[t] : kEnd;
+  // a first arrival whose sibling lies outside the chunk leaves its arrival word
+  if (c0 + t + 1 < c1 && L.cnt[t] == 1)
+    b.counters[c0 + t] = ((unsigned long long)(L.far[t] + 1u) << 32) | L.fpatch[t];
+  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < L.nfront) ? L.front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
 }
 
 // Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one
 // thread per handed-over node (two per sub-chunk).  A handed-over node's
-// slabs are in its half of its parent's record.
+// slabs are already in its half of its parent's record.
 template <bool FP32>
 __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t chunk, uint64_t nsub,
-                                                            const uint32_t* __restrict__ front_in,
-                                                            uint32_t* __restrict__ front_out) {
-  __shared__ uint32_t s_front[2];
+                                                            const FrontNode* __restrict__ front_in,
+                                                            FrontNode* __restrict__ front_out) {
+  __shared__ FrontNode s_front[2];
   __shared__ uint32_t s_nfront;
   const uint32_t t = threadIdx.x;
   const uint64_t c0l = (uint64_t)blockIdx.x * chunk;
@@ -748,29 +759,14 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
   __syncthreads();
   const uint64_t slot = (uint64_t)blockIdx.x * (2 * kTreeChunk) + t;  // = 2 * sub + i
   if (slot < 2 * nsub) {
-    const uint32_t node = front_in[slot];
-    if (node != kEnd && node >= 2 * n - 1) {
-      atomicOr(b.status, kErrTree);
-    } else if (node != kEnd) {
-      const RecT<FP32>* recs = recs_of<FP32>(b);
-      SlabT<FP32> kd[kSlabs];
-      uint32_t l, r, pk, elem = kEnd;
-      if (node >= n - 1) {
-        l = r = node - (uint32_t)(n - 1);
-        pk = b.patch_of_pos[l];
-        elem = b.perm[l];
-      } else {
-        l = recs[node].first;
-        r = recs[node].last;
-        pk = recs[node].patch;
-      }
-      const bool is_left = (l == 0) || (r != n - 1 && b.delta[r] < b.delta[l - 1]);
-      rec_load_half<FP32>(b, is_left ? r : l - 1, is_left ? 0 : 1, kd);
-      climb<FP32>(b, n, node, l, r, pk, kd, elem, c0, c1, s_front, &s_nfront);
+    const FrontNode f = front_in[slot];
+    if (f.l != kEnd) {
+      if (f.l > f.r || f.r >= n || (f.l == 0 && f.r == n - 1)) atomicOr(b.status, kErrTree);  // never the root
+      else climb<FP32>(b, n, f.l, f.r, f.patch, f.d & 0xffu, f.d >> 8, c0, c1, s_front, &s_nfront);
     }
   }
   __syncthreads();
-  if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : kEnd;
+  if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
 }
 
 // fp64 form: leaf p's slabs from its parent's record (n x 26, sorted order)
@@ -929,8 +925,8 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
   if (e != hipSuccess) return e;
   // upper phases: chunks 256x larger until one chunk holds every leaf
   uint64_t chunk = kTreeChunk;
-  uint32_t* fin = b.front_a;
-  uint32_t* fout = b.front_b;
+  FrontNode* fin = b.front_a;
+  FrontNode* fout = b.front_b;
   while (chunk < n) {
     const uint64_t nsub = (n + chunk - 1) / chunk;
     const uint64_t next = chunk * kTreeChunk;
@@ -939,7 +935,7 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     else hipLaunchKernelGGL(k_tree_up<false>, dim3(blocks), dim3(2 * kTreeChunk), 0, s, b, n, next, nsub, fin, fout);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    uint32_t* tmp = fin;
+    FrontNode* tmp = fin;
     fin = fout;
     fout = tmp;
     chunk = next;

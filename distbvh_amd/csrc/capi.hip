@@ -250,12 +250,12 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->delta.ensure(n1));
   BVH_TRY(o->patch_of_pos.ensure(n1 * 4));
   BVH_TRY(o->patch_of_input.ensure(n1 * 4));
-  BVH_TRY(o->counters.ensure(n1 * 4));
+  BVH_TRY(o->counters.ensure(n1 * sizeof(unsigned long long)));
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
   BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, -, max fp32 slab error, -]
-  BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
-  BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * 4));
+  BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * sizeof(FrontNode)));
+  BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * sizeof(FrontNode)));
   BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
@@ -285,7 +285,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.delta = o->delta.as<uint8_t>();
   b.patch_of_pos = o->patch_of_pos.as<uint32_t>();
   b.patch_of_input = o->patch_of_input.as<uint32_t>();
-  b.counters = o->counters.as<uint32_t>();
+  b.counters = o->counters.as<unsigned long long>();
   b.patch_root = o->patch_root.as<uint32_t>();
   b.root = o->root.as<uint32_t>();
   b.rec64 = o->fp32 ? nullptr : o->rec64.as<Rec64>();
@@ -294,8 +294,8 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.rootbox32 = o->fp32 ? o->rootbox32.as<int32_t>() : nullptr;
   b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
-  b.front_a = o->front_a.as<uint32_t>();
-  b.front_b = o->front_b.as<uint32_t>();
+  b.front_a = o->front_a.as<FrontNode>();
+  b.front_b = o->front_b.as<FrontNode>();
   b.patch_k64 = o->patch_k64.as<unsigned long long>();
   return BVH_OK;
 }
@@ -356,8 +356,8 @@ static bvh_status object_build(bvh_object_s* o) {
 }
 
 // Refit: new element k-DOPs on the topology of the last build (same
-// permutation, patches and split heights); counters and roots are cleared and
-// the bottom-up pass runs again.  Ropes depend on the topology only.
+// permutation, patches and split heights); arrival words and roots are
+// cleared and the bottom-up pass runs again.
 static bvh_status object_refit(bvh_object_s* o) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
@@ -365,7 +365,7 @@ static bvh_status object_refit(bvh_object_s* o) {
   HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));
   if (n == 0) return BVH_OK;
   w->stage_begin("refit");
-  if (n > 1) HIP_TRY(launch_fill_u32(o->bb.counters, n - 1, 0u, s));
+  if (n > 1) HIP_TRY(launch_fill_u32((uint32_t*)o->bb.counters, 2 * (n - 1), 0u, s));
   HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
   HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
   HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
@@ -1152,9 +1152,8 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "autotune inside a capture");
   const bool self = o == other;
-  static const uint32_t kLeaf[4] = {1, 2, 4, 8};
+  static const uint32_t kBatch[3] = {1, 2, 4};
   static const uint32_t kBlock[3] = {64, 128, 256};
-  const uint32_t keep_leaf = w->cfg.leaf_size;
   const LaunchShape keep_shape = w->shape;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
@@ -1162,13 +1161,13 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   bvh_tuning b{};
   b.ms = -1.0;
   bvh_status st = BVH_OK;
-  // the fp32 walk has no leaf clusters: only the launch shape matters there
-  const int nleaf = o->fp32 ? 1 : 4;
   uint32_t k = 0;
-  for (int li = 0; li < nleaf && st == BVH_OK; ++li)
+  // the fp32 walk takes one record per round trip (scalar registers)
+  const int nbatch = o->fp32 ? 1 : 3;
+  for (int ki = 0; ki < nbatch && st == BVH_OK; ++ki)
     for (int bi = 0; bi < 3 && st == BVH_OK; ++bi)
       for (int xi = 0; xi < 2 && st == BVH_OK; ++xi) {
-        w->cfg.leaf_size = o->fp32 ? keep_leaf : kLeaf[li];
+        w->shape.batch = o->fp32 ? keep_shape.batch : kBatch[ki];
         w->shape.block = kBlock[bi];
         w->shape.xcd_remap = xi == 1;
         st = run_query(o, other, self);  // sizes the buffers
@@ -1186,7 +1185,7 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
         ++k;
         if (st == BVH_OK && (b.ms < 0.0 || t < b.ms)) {
           b.ms = t;
-          b.leaf_size = w->cfg.leaf_size;
+          b.batch = w->shape.batch;
           b.block_size = w->shape.block;
           b.xcd_remap = w->shape.xcd_remap ? 1 : 0;
         }
@@ -1194,16 +1193,36 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != BVH_OK) {
-    w->cfg.leaf_size = keep_leaf;
     w->shape = keep_shape;
     return st;
   }
   b.tried = k;
-  w->cfg.leaf_size = b.leaf_size;
+  w->shape.batch = b.batch;
   w->shape.block = b.block_size;
   w->shape.xcd_remap = b.xcd_remap != 0;
   *best = b;
   return run_query(o, other, self);  // leave the results of the chosen setting
+}
+
+bvh_status bvh_set_launch(bvh_world w, const bvh_tuning* shape) {
+  if (!w || !shape) return fail(BVH_E_INVALID, "null argument");
+  if (w->capturing) return fail(BVH_E_STATE, "launch shape changed inside a capture");
+  const uint32_t bs = shape->block_size, kb = shape->batch;
+  if ((bs != 64 && bs != 128 && bs != 256) || (kb != 1 && kb != 2 && kb != 4) || shape->xcd_remap > 1)
+    return fail(BVH_E_INVALID, "block_size must be 64, 128 or 256, batch 1, 2 or 4, xcd_remap 0 or 1");
+  w->shape.block = bs;
+  w->shape.batch = kb;
+  w->shape.xcd_remap = shape->xcd_remap != 0;
+  return BVH_OK;
+}
+
+bvh_status bvh_get_launch(bvh_world w, bvh_tuning* shape) {
+  if (!w || !shape) return fail(BVH_E_INVALID, "null argument");
+  *shape = bvh_tuning{};
+  shape->block_size = w->shape.block;
+  shape->batch = w->shape.batch;
+  shape->xcd_remap = w->shape.xcd_remap ? 1 : 0;
+  return BVH_OK;
 }
 
 // ---- batching form: whole steps captured into a HIP graph ------------------------------

@@ -45,6 +45,14 @@ hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* h
 hipError_t launch_scan_u32(uint32_t* v, uint64_t n, uint64_t* lb, uint32_t* counter, hipStream_t s);
 
 // ---- object build ---------------------------------------------------------------
+// A node handed from one build phase to the next, after it has written its
+// half of its parent's record: its range [l, r], its patch and the split
+// heights on both sides (d = delta[l-1] | delta[r] << 8, 0 at the ends).
+// l == kEnd marks an empty entry.
+struct alignas(16) FrontNode {
+  uint32_t l, r, patch, d;
+};
+
 struct BuildBufs {
   // original order
   double* centroid;        // 3 planes of n (kPoly)
@@ -70,7 +78,9 @@ struct BuildBufs {
   uint8_t* delta;          // n-1
   uint32_t* patch_of_pos;  // n
   uint32_t* patch_of_input;// n
-  uint32_t* counters;      // n-1 arrival counters
+  // n-1 arrival words: 0 before any child arrives, then the first arriver's
+  // far end + 1 (high word) and its patch (low word)
+  unsigned long long* counters;
   uint32_t* patch_root;    // od
   uint32_t* root;          // 1
   Rec64* rec64;            // fp64 form: n-1 child-pair records
@@ -79,8 +89,8 @@ struct BuildBufs {
   int32_t* rootbox32;      // fp32 form: the root's slabs (26, ord32 images)
   double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
-  uint32_t* front_a;       // 2 per 256-leaf chunk: nodes handed to the next build phase
-  uint32_t* front_b;
+  FrontNode* front_a;      // 2 per 256-leaf chunk: nodes handed to the next build phase
+  FrontNode* front_b;
   unsigned long long* patch_k64;  // od x 26 exact (fp64) patch slabs, ordered_u64 encoding
 };
 
@@ -139,11 +149,14 @@ struct PairOut {
 };
 
 // launch shape of the narrowphase (tuned by the autotuner): threads per
-// block (64, 128 or 256) and whether blocks are remapped so each XCD walks a
-// contiguous range of query tiles
+// block (64, 128 or 256), whether blocks are remapped so each XCD walks a
+// contiguous range of query tiles, and the records a wave of the fp64 walk
+// loads per memory round trip (1, 2 or 4; the fp32 walk takes one, in scalar
+// registers)
 struct LaunchShape {
   unsigned block = 256;
   bool xcd_remap = true;
+  unsigned batch = 4;
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)

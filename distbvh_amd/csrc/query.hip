@@ -286,19 +286,39 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   }
 }
 
-// fp64 form: the same record walk.  Two fp64 boxes (416 B) do not fit the
-// scalar registers, so a 448-byte record is staged in LDS with one 8-byte
-// vector load per lane (56 lanes) and every lane reads both boxes as LDS
-// broadcasts.  Leaf tests are exact, so hits are emitted directly.
-template <bool SELF>
+// Both boxes of an LDS-staged fp64 record against the lane's query slabs,
+// one axis at a time: the scheduling barrier after each axis keeps four of
+// the record's doubles in registers, which holds the walk at 88-94 VGPRs
+// (5 waves per SIMD) where loading both boxes at once needs over 128.
+__device__ __forceinline__ void overlap_pair(const double* qk, const Rec64& r, bool& h0, bool& h1) {
+  bool o0 = true, o1 = true;
+#pragma unroll
+  for (int a = 0; a < kAxes; ++a) {
+    o0 = o0 & (qk[2 * a] < r.box[0][2 * a + 1]) & (r.box[0][2 * a] < qk[2 * a + 1]);
+    o1 = o1 & (qk[2 * a] < r.box[1][2 * a + 1]) & (r.box[1][2 * a] < qk[2 * a + 1]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  h0 = o0;
+  h1 = o1;
+}
+
+// fp64 form: the same record walk, K records per memory round trip.  Two
+// fp64 boxes (416 B) do not fit the scalar registers, so each 448-byte record
+// is staged in LDS with one 8-byte vector load per lane (56 lanes) and every
+// lane reads both boxes as LDS broadcasts.  Each iteration pops up to K nodes
+// off the wave's stack and issues all their loads before it tests any: the
+// set of visited nodes does not depend on K (a node is visited when some lane
+// overlaps it and its parent was visited), only the number of dependent
+// round trips does.  Leaf tests are exact, so hits are emitted directly.
+template <bool SELF, int K>
 __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xcd, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
   __shared__ uint32_t s_stack[256 / kWave][kStack];
-  __shared__ Rec64 s_rec[256 / kWave];
+  __shared__ Rec64 s_rec[256 / kWave][K];
   const uint32_t wv = threadIdx.x / kWave;
   uint64_t* buf = s_buf[wv];
   uint32_t* stk = s_stack[wv];
-  Rec64& R = s_rec[wv];
+  Rec64* R = s_rec[wv];
   uint32_t fill = 0;
   const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
@@ -326,9 +346,10 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
   }
   const uint32_t n = (uint32_t)t.n;
   const uint32_t tleaf0 = n - 1;
+  const uint32_t lane = lane_id();
+  const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
   uint32_t steps = 0;
   uint32_t sp = 0;
-  uint32_t x = kEnd;  // the node to visit next (kEnd: pop the stack)
   {
     // the root's slabs through scalar loads (wave-uniform)
     const __attribute__((address_space(4))) double* rb = (const __attribute__((address_space(4))) double*)t.rootbox64;
@@ -339,53 +360,71 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
     if (n == 1) {
       if (!SELF) wave_emit(buf, fill, out, h, pair_key(qa, uniform(t.perm[0]), SELF));
     } else if (__ballot(h) != 0) {
-      x = uniform(*t.root);
+      stk[0] = uniform(*t.root);
+      sp = 1;
     }
   }
-  const uint32_t lane = lane_id();
-  for (;;) {
-    if (x == kEnd) {
-      if (sp == 0) break;
-      x = uniform(stk[--sp]);
-    }
-    if (x >= tleaf0 || ++steps > n) {
+  while (sp != 0) {
+    const uint32_t m = sp < (uint32_t)K ? sp : (uint32_t)K;
+    uint32_t xs[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) xs[k] = (uint32_t)k < m ? uniform(stk[sp - 1 - k]) : 0u;
+    sp -= m;
+    steps += m;
+    bool bad = steps > n;
+#pragma unroll
+    for (int k = 0; k < K; ++k) bad = bad || ((uint32_t)k < m && xs[k] >= tleaf0);
+    if (bad) {
       if (lane == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
     {
-      const uint64_t* src = (const uint64_t*)(t.rec64 + x);
-      if (lane < sizeof(Rec64) / 8) ((uint64_t*)&R)[lane] = src[lane];
+      // every load in flight before the first LDS store
+      uint64_t v[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if ((uint32_t)k < m && lane < sizeof(Rec64) / 8) v[k] = ((const uint64_t*)(t.rec64 + xs[k]))[lane];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if ((uint32_t)k < m && lane < sizeof(Rec64) / 8) ((uint64_t*)&R[k])[lane] = v[k];
       __builtin_amdgcn_wave_barrier();
     }
-    const uint32_t cl = uniform(R.child[0]), cr = uniform(R.child[1]);
-    const uint32_t el = uniform(R.elem[0]), er = uniform(R.elem[1]);
-    const uint32_t rlast = uniform(R.last);
-    bool hl = overlap(qk, R.box[0]);
-    bool hr = overlap(qk, R.box[1]);
-    __builtin_amdgcn_wave_barrier();  // every lane has read the record
-    if (SELF) {
-      hl = hl && (uint64_t)x > p;  // the left child ends at position x
-      hr = hr && (uint64_t)rlast > p;
-    }
-    bool go_l = false, go_r = false;
-    if (__ballot(hl) != 0) {
-      if (cl >= tleaf0) wave_emit(buf, fill, out, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl, pair_key(qa, el, SELF));
-      else go_l = true;
-    }
-    if (__ballot(hr) != 0) {
-      if (cr >= tleaf0) wave_emit(buf, fill, out, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr, pair_key(qa, er, SELF));
-      else go_r = true;
-    }
-    // depth first: continue with the left child, keep the right one
-    if (go_l && go_r) {
-      if (sp >= kStack) {
-        if (lane == 0) atomicOr(t.status, kErrTraverse);
-        break;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if ((uint32_t)k < m) {
+        const Rec64& Rk = R[k];
+        const uint32_t cl = uniform(Rk.child[0]), cr = uniform(Rk.child[1]);
+        const uint32_t el = uniform(Rk.elem[0]), er = uniform(Rk.elem[1]);
+        bool hl, hr;
+        overlap_pair(qk, Rk, hl, hr);
+        if (SELF) {
+          // position bounds (non-short-circuit &: no divergent branch)
+          hl = hl & (xs[k] > p32);  // the left child ends at position x
+          hr = hr & (uniform(Rk.last) > p32);
+        }
+        // ballots and readfirstlane values: every branch below is wave-uniform,
+        // so the stack pointer and counters stay in scalar registers
+        const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
+        const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
+        if (any_l && leaf_l)
+          wave_emit(buf, fill, out, SELF ? (hl & (cl - tleaf0 > p32)) : hl, pair_key(qa, el, SELF));
+        if (any_r && leaf_r)
+          wave_emit(buf, fill, out, SELF ? (hr & (cr - tleaf0 > p32)) : hr, pair_key(qa, er, SELF));
+        // the left child ends on top of the stack
+        if (sp + 2 > kStack) {
+          bad = true;
+        } else {
+          if (any_r && !leaf_r) stk[sp++] = cr;
+          if (any_l && !leaf_l) stk[sp++] = cl;
+        }
       }
-      stk[sp++] = cr;
-      x = cl;
-    } else {
-      x = go_l ? cl : (go_r ? cr : kEnd);
+      // one record's boxes in registers at a time
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_wave_barrier();  // every lane has read the records
+    if (bad) {
+      if (lane == 0) atomicOr(t.status, kErrTraverse);
+      break;
     }
   }
   wave_flush(buf, fill, out);
@@ -545,6 +584,13 @@ __global__ __launch_bounds__(256) void k_digest(const uint64_t* __restrict__ pai
 }
 
 // ---- launchers ---------------------------------------------------------------------
+template <int K>
+static void narrow64_dispatch(const TreeView& q, const TreeView& t, int self, int xcd, unsigned blocks, unsigned bs,
+                              const PairOut& out, hipStream_t s) {
+  if (self) hipLaunchKernelGGL((k_narrow64<true, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+  else hipLaunchKernelGGL((k_narrow64<false, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+}
+
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
                               const LaunchShape& shape, const PairOut& out, hipStream_t s) {
   (void)leaf_size;  // the record walk tests every leaf from its parent's record
@@ -559,6 +605,8 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
       return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
+    // one record per round trip: it arrives in scalar registers (the batch
+    // applies to the fp64 walk)
     if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     // in self mode both keys index the one object (q == t)
@@ -568,8 +616,12 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     return hipGetLastError();
   }
   if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
-  if (self) hipLaunchKernelGGL(k_narrow64<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
-  else hipLaunchKernelGGL(k_narrow64<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+  switch (shape.batch) {
+    case 1: narrow64_dispatch<1>(q, t, self, xcd, blocks, bs, out, s); break;
+    case 2: narrow64_dispatch<2>(q, t, self, xcd, blocks, bs, out, s); break;
+    case 4: narrow64_dispatch<4>(q, t, self, xcd, blocks, bs, out, s); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

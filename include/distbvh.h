@@ -231,20 +231,26 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
-  uint32_t leaf_size;   /* elements tested leaf by leaf (fp64 walk only) */
+  uint32_t batch;       /* records a wave of the fp64 walk loads per memory round trip: 1, 2 or 4 */
   uint32_t block_size;  /* narrowphase threads per block: 64, 128 or 256 */
   uint32_t xcd_remap;   /* 1: each XCD walks a contiguous range of query tiles */
   uint32_t tried;       /* configurations timed (entries of tried_ms) */
   double ms;            /* best query time */
-  double tried_ms[24];  /* in the order leaf (1, 2, 4, 8) x block (64, 128, 256) x remap (off, on) */
+  double tried_ms[24];  /* in the order batch (1, 2, 4) x block (64, 128, 256) x remap (off, on) */
 } bvh_tuning;
-/* Times the query o x other (self collision when o == other) over leaf size
- * (fp64 form: 1, 2, 4, 8), block size (64, 128, 256) and XCD remap (off, on),
- * best of 3 runs each, keeps the fastest setting for the world and leaves
- * that query's results in o.  The pair set does not depend on the setting.
- * (No reference counterpart: the walk here is wave-wide and stackless or
- * record-based, and these are its tuning parameters.) */
+/* Times the query o x other (self collision when o == other) over the batch
+ * (1, 2, 4; fp64 form only: the fp32 walk takes one record per round trip),
+ * block size (64, 128, 256) and XCD remap (off, on), best of 3 runs each,
+ * keeps the fastest setting for the world and leaves that query's results in
+ * o.  The pair set does not depend on the setting.
+ * (No reference counterpart: the walk here is wave-wide and record-based,
+ * and these are its tuning parameters.) */
 bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning *best);
+/* Sets / reads the world's narrowphase launch shape (batch, block_size,
+ * xcd_remap of a bvh_tuning; the other fields are ignored / zero).  Default:
+ * batch 4, block 256, remap on.  No reference counterpart. */
+bvh_status bvh_set_launch(bvh_world w, const bvh_tuning *shape);
+bvh_status bvh_get_launch(bvh_world w, bvh_tuning *shape);
 
 /* ---- batching form: a whole step as one HIP graph ----------------------------- */
 /* No reference counterpart (the reference launches per call).  Between

@@ -8,6 +8,7 @@ import pytest
 import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import capi
+from distbvh_amd.capi import BvhError
 from distbvh_amd import workloads as W
 from helpers import gpu_object, pair_digest, sort_pairs, to_cuda
 
@@ -144,9 +145,9 @@ def test_autotune(torch_cuda, precision):
         oa = gpu_object(w, "poly", a, device=True)
         ob = gpu_object(w, "poly", b, device=True)
         t = oa.autotune(ob)
-        assert t["leaf_size"] in (1, 2, 4, 8)
+        assert t["batch"] in (1, 2, 4)
         assert t["block_size"] in (64, 128, 256)
-        assert len(t["tried_ms"]) == (6 if precision == "fp32" else 24)
+        assert len(t["tried_ms"]) == (6 if precision == "fp32" else 18)
         assert all(x > 0 for x in t["tried_ms"]) and t["ms"] == min(t["tried_ms"])
         assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
         oa.broadphase(ob)  # the chosen setting stays
@@ -158,3 +159,58 @@ def test_autotune(torch_cuda, precision):
         o = gpu_object(w, "poly", b, device=True)
         t = o.autotune()
         assert np.array_equal(sort_pairs(o.pairs()), want_self)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_launch_shapes(torch_cuda, precision):
+    """Every batch (records per round trip), block size and XCD remap gives
+    the oracle's pairs, for two objects and for self collision."""
+    a, b = W.wavy_sheets(48)
+    ka, _ = O.elements_poly(a)
+    kb, _ = O.elements_poly(b)
+    want = sort_pairs(O.brute_pairs(ka, kb))
+    want_self = sort_pairs(O.brute_self(kb))
+    assert len(want) > 0 and len(want_self) > 0
+    with CollisionWorld(16, leaf_size=1, precision=precision) as w:
+        assert w.launch() == {"batch": 4, "block_size": 256, "xcd_remap": True}
+        oa = gpu_object(w, "poly", a, device=True)
+        ob = gpu_object(w, "poly", b, device=True)
+        for batch in (1, 2, 4):
+            for block in (64, 256):
+                for xcd in (False, True):
+                    w.set_launch(batch=batch, block_size=block, xcd_remap=xcd)
+                    assert w.launch() == {"batch": batch, "block_size": block, "xcd_remap": xcd}
+                    oa.broadphase(ob)
+                    assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want), (batch, block, xcd)
+                    ob.self_collision()
+                    assert np.array_equal(sort_pairs(ob.pairs()), want_self), (batch, block, xcd)
+        with pytest.raises(BvhError):
+            w.set_launch(batch=3)
+        with pytest.raises(BvhError):
+            w.set_launch(block_size=96)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_launch_shapes_full_size(torch_cuda, precision):
+    """2 x 2,097,152 triangles: the batches give one pair set (digest and
+    count), and the same node visits (the visited set does not depend on the
+    batch)."""
+    import torch
+
+    a, b = W.wavy_sheets(1024)
+    ta = torch.from_numpy(a).cuda()
+    tb = torch.from_numpy(b).cuda()
+    with CollisionWorld(256, leaf_size=4, precision=precision) as w:
+        oa = w.create_collision_object()
+        ob = w.create_collision_object()
+        oa.set_entity_data_poly(ta)
+        ob.set_entity_data_poly(tb)
+        oa.init_broadphase()
+        ob.init_broadphase()
+        seen = set()
+        for batch in (1, 2, 4):
+            w.set_launch(batch=batch)
+            oa.broadphase(ob)
+            st = oa.result_stats()
+            seen.add((oa.result_count(), oa.digest(), st["node_visits"]))
+        assert len(seen) == 1, seen
