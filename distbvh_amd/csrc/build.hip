@@ -594,29 +594,25 @@ __device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32
   L.endslot[endslot] = slot;
 }
 
-// Phase-1 climb inside the chunk [c0, c1): the sibling's slabs, id and range
-// come from LDS and arrivals are LDS atomics whose fences order LDS only, so a
-// level costs LDS latency; record halves go to global memory with stores that
-// nothing in this kernel waits for.
+// Phase-1 climb inside the chunk [c0, c1) from node [l, r] of patch
+// `mypatch`, whose slabs are in LDS slot l - c0 and in its half of its
+// parent's record.  Arrivals are LDS atomics whose fences order LDS only, so
+// a level costs LDS latency.  The second arriver forms the parent's slabs
+// from the two children's LDS slots, so no slabs are carried from one level
+// to the next (which keeps the register count down); record halves go to
+// global memory with stores that nothing in this kernel waits for.
 template <bool FP32>
-__device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint32_t node, uint32_t l, uint32_t r,
-                                            uint32_t mypatch, SlabT<FP32>* kd, uint32_t elem, uint32_t c0,
-                                            uint32_t c1, ChunkLds<FP32>& L) {
+__device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint32_t l, uint32_t r, uint32_t mypatch,
+                                            uint32_t c0, uint32_t c1, ChunkLds<FP32>& L) {
   RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
-    if (l == 0 && r == n - 1) {
-      *b.root = node;
-      store_rootbox<FP32>(b, kd);
-      return;
-    }
-    // delta[r] = L.delta[r - c0 + 1], delta[l - 1] = L.delta[l - c0]
-    const uint32_t dl = L.delta[l - c0], dr = L.delta[r - c0 + 1];  // delta[l - 1], delta[r]
+    // delta[l - 1] = L.delta[l - c0], delta[r] = L.delta[r - c0 + 1] (0 at the ends)
+    const uint32_t dl = L.delta[l - c0], dr = L.delta[r - c0 + 1];
     const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
     const uint32_t parent = is_left ? r : l - 1;
-    rec_store_half<FP32>(b, parent, is_left ? 0 : 1, kd, node, elem);
     if (parent < c0 || parent + 1 >= c1) {
       const uint32_t k = atomicAdd(&L.nfront, 1u);
-      if (k < 2) L.front[k] = FrontNode{l, r, mypatch, (l ? dl : 0u) | ((r != n - 1 ? dr : 0u) << 8)};
+      if (k < 2) L.front[k] = FrontNode{l, r, mypatch, dl | (dr << 8)};
       else atomicOr(b.status, kErrTree);
       return;
     }
@@ -640,24 +636,35 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
       atomicOr(b.status, kErrTree);
       return;
     }
-    SlabT<FP32> ks[kSlabs];
-#pragma unroll
-    for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][ss];
+    const uint32_t ms = l - c0;
     const uint32_t slast = L.last[ss], spatch = L.patch[ss];
-    slab_union(kd, ks);
+    SlabT<FP32> kd[kSlabs];
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      const SlabT<FP32> lo0 = L.slab[2 * a][ms], lo1 = L.slab[2 * a][ss];
+      const SlabT<FP32> hi0 = L.slab[2 * a + 1][ms], hi1 = L.slab[2 * a + 1][ss];
+      kd[2 * a] = (lo0 < lo1) ? lo0 : lo1;
+      kd[2 * a + 1] = (hi0 > hi1) ? hi0 : hi1;
+    }
     if (is_left) r = slast;
     else l = c0 + ss;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
     recs[parent].first = l;
     recs[parent].last = r;
     recs[parent].patch = mypatch;
-    if (is_patch_root(mypatch, l, r, n, l ? L.delta[l - c0] : 0u, r != n - 1 ? L.delta[r - c0 + 1] : 0u)) {
+    const uint32_t ndl = L.delta[l - c0], ndr = L.delta[r - c0 + 1];
+    if (is_patch_root(mypatch, l, r, n, ndl, ndr)) {
       b.patch_root[mypatch] = parent;
       store_patch_box<FP32>(b, mypatch, kd);
     }
+    if (l == 0 && r == n - 1) {
+      *b.root = parent;
+      store_rootbox<FP32>(b, kd);
+      return;
+    }
+    const bool up_left = (l == 0) || (r != n - 1 && ndr < ndl);
+    rec_store_half<FP32>(b, up_left ? r : l - 1, up_left ? 0 : 1, kd, parent, kEnd);
     publish<FP32>(L, l - c0, r - c0, kd, r, mypatch);
-    node = parent;
-    elem = kEnd;
   }
 }
 
@@ -706,14 +713,17 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       proj_all(b.origin[0], b.origin[1], b.origin[2], po);
       float kf[kSlabs];
       to_fp32_outward(kd, po, kf);
-      // largest distance of an fp32 slab from its fp64 value, in fp32 ulps
+      // largest distance of an fp32 slab from its fp64 value, in fp32 ulps;
+      // an ulp is a power of two 2^k (frexp exponent k + 1), so the division
+      // is the exponent shift ldexp(d, 1 - (k + 1)), exact
       float err = 0.0f;
 #pragma unroll
       for (int i = 0; i < kSlabs; ++i) {
         const double exact = kd[i] - po[i / 2];
         const float mag = __builtin_fabsf(kf[i]);
-        const double ulp = (double)(next_up(mag) - mag);
-        const float d = (float)(__builtin_fabs((double)kf[i] - exact) / ulp);
+        const float ulp = next_up(mag) - mag;
+        const int e = __builtin_amdgcn_frexp_expf(ulp);
+        const float d = (float)__builtin_ldexp(__builtin_fabs((double)kf[i] - exact), 1 - e);
         err = d > err ? d : err;
       }
       atomicMax(&L.err, __float_as_uint(err));
@@ -724,16 +734,18 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       if (single) store_patch_box<false>(b, pk, kd);
     }
     publish<FP32>(L, t, t, ks, p, pk);
-  }
-  __syncthreads();  // counters, deltas and leaf slots ready
-  if (act) {
     if (n == 1) {
       *b.root = leaf;
       store_rootbox<FP32>(b, ks);
     } else {
-      climb_chunk<FP32>(b, n, leaf, p, p, pk, ks, e, c0, c1, L);
+      // the leaf's half of its parent's record
+      const uint32_t dl = p ? b.delta[p - 1] : 0u, dr = (p != n - 1) ? b.delta[p] : 0u;
+      const bool is_left = (p == 0) || (p != n - 1 && dr < dl);
+      rec_store_half<FP32>(b, is_left ? p : p - 1, is_left ? 0 : 1, ks, leaf, e);
     }
   }
+  __syncthreads();  // counters, deltas and leaf slots ready
+  if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L);
   __syncthreads();
   if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);
   // a first arrival whose sibling lies outside the chunk leaves its arrival word

@@ -150,13 +150,14 @@ struct PairOut {
 
 // launch shape of the narrowphase (tuned by the autotuner): threads per
 // block (64, 128 or 256), whether blocks are remapped so each XCD walks a
-// contiguous range of query tiles, and the records a wave of the fp64 walk
-// loads per memory round trip (1, 2 or 4; the fp32 walk takes one, in scalar
-// registers)
+// contiguous range of query tiles, and the fp64 walk's batch: 1 = one
+// record per visit in scalar registers (k_narrow64s), 2 or 4 = that many
+// records per memory round trip staged in LDS (k_narrow64).  The fp32 walk
+// always takes one record in scalar registers.
 struct LaunchShape {
   unsigned block = 256;
   bool xcd_remap = true;
-  unsigned batch = 4;
+  unsigned batch = 1;
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)

@@ -1,11 +1,13 @@
 // query.hip -- broadphase and narrowphase traversal on gfx950.
 //
-//  * k_narrow64 / k_narrow32: one lane per query element (the query object's
-//    leaves in its sorted order, so a wave holds 64 spatially adjacent
-//    elements).  The wave walks the other object's tree with a wave-uniform
-//    node: each internal node's record holds both children's slabs, so one
-//    memory round trip tests two nodes; leaf children are tested at once and
-//    internal children that some lane overlaps go on a per-wave LDS stack.
+//  * k_narrow64s / k_narrow64 / k_narrow32: one lane per query element (the
+//    query object's leaves in its sorted order, so a wave holds 64 spatially
+//    adjacent elements).  The wave walks the other object's tree with a
+//    wave-uniform node: each internal node's record holds both children's
+//    slabs, so one memory round trip tests two nodes; leaf children are
+//    tested at once and internal children that some lane overlaps go on a
+//    per-wave LDS stack.  Records arrive in scalar registers (k_narrow32,
+//    k_narrow64s) or, K at a time, staged in LDS (k_narrow64).
 //    Hits are staged per wave in LDS and appended with one atomic per flush.
 //    Replaces query_tree / query_node_impl (collision_query.hpp:166-213) as
 //    used by a narrowphase functor per patch pair, and self_collision_set
@@ -286,6 +288,58 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   }
 }
 
+// Strict overlap of the lane's query slabs q with one box b whose 26 slabs
+// are wave-uniform (scalar registers), as the AND of 26 fp64 compares chained
+// with v_cmpx: each compare clears the EXEC bit of the lanes it fails (a
+// lane already cleared stays cleared), so the AND accumulates in EXEC with no
+// scalar instruction per compare.  EXEC is saved first and restored last;
+// the caller runs it with every lane of the wave active.  Returns the wave's
+// hit mask; *hit is the lane's bit.  `touch` are loads whose only purpose is
+// to bring cache lines in: they are consumed here so that they are kept.
+__device__ __forceinline__ uint64_t box_hits(const double* q, const double* b, uint32_t touch0, uint32_t touch1,
+                                             bool* hit) {
+  unsigned long long res, save, m;
+  int v;
+  asm volatile(
+      "s_mov_b64 %[save], exec\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q0], %[b1]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b0], %[q1]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q2], %[b3]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b2], %[q3]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q4], %[b5]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b4], %[q5]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q6], %[b7]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b6], %[q7]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q8], %[b9]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b8], %[q9]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q10], %[b11]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b10], %[q11]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q12], %[b13]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b12], %[q13]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q14], %[b15]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b14], %[q15]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q16], %[b17]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b16], %[q17]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q18], %[b19]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b18], %[q19]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q20], %[b21]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b20], %[q21]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q22], %[b23]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b22], %[q23]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[q24], %[b25]\n\t"
+      "v_cmpx_lt_f64_e64 %[m], %[b24], %[q25]\n\t"
+      "s_nop 4\n\t"
+      "s_mov_b64 %[res], exec\n\t"
+      "s_mov_b64 exec, %[save]\n\t"
+      "v_cndmask_b32_e64 %[v], 0, 1, %[res]\n\t"
+      : [res] "=&s"(res), [save] "=&s"(save), [m] "=&s"(m), [v] "=&v"(v)
+      : [q0] "v"(q[0]), [q1] "v"(q[1]), [q2] "v"(q[2]), [q3] "v"(q[3]), [q4] "v"(q[4]), [q5] "v"(q[5]), [q6] "v"(q[6]), [q7] "v"(q[7]), [q8] "v"(q[8]), [q9] "v"(q[9]), [q10] "v"(q[10]), [q11] "v"(q[11]), [q12] "v"(q[12]), [q13] "v"(q[13]), [q14] "v"(q[14]), [q15] "v"(q[15]), [q16] "v"(q[16]), [q17] "v"(q[17]), [q18] "v"(q[18]), [q19] "v"(q[19]), [q20] "v"(q[20]), [q21] "v"(q[21]), [q22] "v"(q[22]), [q23] "v"(q[23]), [q24] "v"(q[24]), [q25] "v"(q[25]),
+        [b0] "s"(b[0]), [b1] "s"(b[1]), [b2] "s"(b[2]), [b3] "s"(b[3]), [b4] "s"(b[4]), [b5] "s"(b[5]), [b6] "s"(b[6]), [b7] "s"(b[7]), [b8] "s"(b[8]), [b9] "s"(b[9]), [b10] "s"(b[10]), [b11] "s"(b[11]), [b12] "s"(b[12]), [b13] "s"(b[13]), [b14] "s"(b[14]), [b15] "s"(b[15]), [b16] "s"(b[16]), [b17] "s"(b[17]), [b18] "s"(b[18]), [b19] "s"(b[19]), [b20] "s"(b[20]), [b21] "s"(b[21]), [b22] "s"(b[22]), [b23] "s"(b[23]), [b24] "s"(b[24]), [b25] "s"(b[25]),
+        [t0] "s"(touch0), [t1] "s"(touch1));
+  *hit = v != 0;
+  return res;
+}
+
 // Both boxes of an LDS-staged fp64 record against the lane's query slabs,
 // one axis at a time: the scheduling barrier after each axis keeps four of
 // the record's doubles in registers, which holds the walk at 88-94 VGPRs
@@ -425,6 +479,117 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
     if (bad) {
       if (lane == 0) atomicOr(t.status, kErrTraverse);
       break;
+    }
+  }
+  wave_flush(buf, fill, out);
+  if (lane == 0 && out.visits) atomicAdd(out.visits, (unsigned long long)steps);
+}
+
+// fp64 form, one record per visit in scalar registers (batch 1): box 0
+// with the child ids and the cache lines of box 1 in one round of scalar
+// loads, then box 1 from the scalar cache; each box is tested with one
+// v_cmpx chain (box_hits), so a visit issues no LDS instruction and no
+// scalar instruction per compare.  The next node is kept in a register, the
+// right child of a two-child visit goes on the per-wave LDS stack.
+typedef __attribute__((address_space(4))) const double CDbl;
+
+template <bool SELF>
+__global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int xcd, PairOut out) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  const uint32_t wv = threadIdx.x / kWave;
+  uint64_t* buf = s_buf[wv];
+  uint32_t* stk = s_stack[wv];
+  uint32_t fill = 0;
+  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
+  const bool valid = p < q.n;
+  double qk[kSlabs];
+  uint32_t qa = 0;
+  if (valid) {
+    const double* src = q.rootbox64;
+    if (q.n > 1) {
+      uint32_t parent;
+      int side;
+      leaf_slot(q.delta, q.n, p, parent, side);
+      src = q.rec64[parent].box[side];
+    }
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      const double2 v = ((const double2*)src)[a];
+      qk[2 * a] = v.x;
+      qk[2 * a + 1] = v.y;
+    }
+    qa = q.perm[p];
+  } else {
+    empty_slabs(qk);
+  }
+  const uint32_t n = (uint32_t)t.n;
+  const uint32_t tleaf0 = n - 1;
+  const uint32_t lane = lane_id();
+  const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
+  uint32_t steps = 0;
+  uint32_t sp = 0;
+  uint32_t x = kEnd;  // the node to visit next (kEnd: pop the stack)
+  {
+    const CDbl* rb = (const CDbl*)t.rootbox64;
+    bool h = true;
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) h = h & (qk[2 * a] < rb[2 * a + 1]) & (rb[2 * a] < qk[2 * a + 1]);
+    if (SELF) h = h & ((uint32_t)(n - 1) > p32);  // the root's last position
+    if (n == 1) {
+      if (!SELF) wave_emit(buf, fill, out, h, pair_key(qa, uniform(t.perm[0]), SELF));
+    } else if (__ballot(h) != 0) {
+      x = uniform(*t.root);
+    }
+  }
+  for (;;) {
+    if (x == kEnd) {
+      if (sp == 0) break;
+      x = uniform(stk[--sp]);
+    }
+    if (x >= tleaf0 || ++steps > n) {
+      if (lane == 0) atomicOr(t.status, kErrTraverse);
+      break;
+    }
+    const CDbl* rp = (const CDbl*)(t.rec64 + x);
+    const CInt* ip = (const CInt*)(t.rec64 + x);
+    double bx[kSlabs];
+#pragma unroll
+    for (int k = 0; k < kSlabs; ++k) bx[k] = rp[k];
+    // child ids, leaf elements (and `last` in self mode): dwords 104..109;
+    // dwords 64 and 80 only bring box 1's cache lines into the scalar cache
+    const uint32_t cl = (uint32_t)ip[104], cr = (uint32_t)ip[105];
+    const uint32_t el = (uint32_t)ip[106], er = (uint32_t)ip[107];
+    const uint32_t rlast = SELF ? (uint32_t)ip[109] : 0u;
+    const uint32_t t0 = (uint32_t)ip[64], t1 = (uint32_t)ip[80];
+    __builtin_amdgcn_sched_barrier(0);
+    bool hl, hr;
+    box_hits(qk, bx, t0, t1, &hl);
+#pragma unroll
+    for (int k = 0; k < kSlabs; ++k) bx[k] = rp[kSlabs + k];
+    box_hits(qk, bx, 0u, 0u, &hr);
+    if (SELF) {
+      hl = hl & (x > p32);  // the left child ends at position x
+      hr = hr & (rlast > p32);
+    }
+    const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
+    const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
+    if (any_l && leaf_l)
+      wave_emit(buf, fill, out, SELF ? (hl & (cl - tleaf0 > p32)) : hl, pair_key(qa, el, SELF));
+    if (any_r && leaf_r)
+      wave_emit(buf, fill, out, SELF ? (hr & (cr - tleaf0 > p32)) : hr, pair_key(qa, er, SELF));
+    const bool go_l = any_l && !leaf_l, go_r = any_r && !leaf_r;
+    // depth first: continue with the left child, keep the right one
+    if (go_l && go_r) {
+      if (sp >= kStack) {
+        if (lane == 0) atomicOr(t.status, kErrTraverse);
+        break;
+      }
+      stk[sp++] = cr;
+      x = cl;
+    } else {
+      x = go_l ? cl : (go_r ? cr : kEnd);
     }
   }
   wave_flush(buf, fill, out);
@@ -617,7 +782,10 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   }
   if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
   switch (shape.batch) {
-    case 1: narrow64_dispatch<1>(q, t, self, xcd, blocks, bs, out, s); break;
+    case 1:
+      if (self) hipLaunchKernelGGL(k_narrow64s<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+      else hipLaunchKernelGGL(k_narrow64s<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+      break;
     case 2: narrow64_dispatch<2>(q, t, self, xcd, blocks, bs, out, s); break;
     case 4: narrow64_dispatch<4>(q, t, self, xcd, blocks, bs, out, s); break;
     default: return hipErrorInvalidValue;

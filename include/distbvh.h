@@ -231,7 +231,8 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
-  uint32_t batch;       /* records a wave of the fp64 walk loads per memory round trip: 1, 2 or 4 */
+  uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 or 4 = that
+                           many records per memory round trip staged in LDS */
   uint32_t block_size;  /* narrowphase threads per block: 64, 128 or 256 */
   uint32_t xcd_remap;   /* 1: each XCD walks a contiguous range of query tiles */
   uint32_t tried;       /* configurations timed (entries of tried_ms) */
@@ -248,7 +249,7 @@ typedef struct bvh_tuning {
 bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning *best);
 /* Sets / reads the world's narrowphase launch shape (batch, block_size,
  * xcd_remap of a bvh_tuning; the other fields are ignored / zero).  Default:
- * batch 4, block 256, remap on.  No reference counterpart. */
+ * batch 1, block 256, remap on.  No reference counterpart. */
 bvh_status bvh_set_launch(bvh_world w, const bvh_tuning *shape);
 bvh_status bvh_get_launch(bvh_world w, bvh_tuning *shape);
 
