@@ -515,16 +515,17 @@ __device__ __forceinline__ void empty_kdop(double* kd) {
 }
 
 // Exact (fp64) patch slabs, the union of the patch's element k-DOPs, for the
-// patch broadphase and local_patches (in both forms).  Called by every
-// lane of the wave; lanes without an element pass pk = kEnd and the empty
-// k-DOP.  A wave whose elements all lie in one patch reduces in registers and
-// lane 0 issues the 26 atomics; otherwise each lane issues its own.
-__device__ __forceinline__ void patch_union64(unsigned long long* pk64, uint32_t pk, const double* kd) {
+// patch broadphase and local_patches (fp32 form; the fp64 form takes them
+// from its patch roots).  Called by every lane of the wave; lanes without an
+// element pass pk = kEnd and the empty k-DOP.  A wave whose elements all lie
+// in one patch reduces in registers and lane 0 leaves the partial in its LDS
+// slot (part, ppatch = the patch) for patch_flush; otherwise each lane issues
+// its own atomics and the slot stays empty (ppatch = kEnd).
+__device__ __forceinline__ void patch_partial(unsigned long long* pk64, uint32_t pk, const double* kd, double* part,
+                                              uint32_t* ppatch) {
   const uint64_t m = __ballot(pk != kEnd);
-  if (m == 0) return;
-  const uint32_t pk0 = __builtin_amdgcn_readlane(pk, __ffsll((long long)m) - 1);
-  if (__ballot(pk != kEnd && pk != pk0) == 0) {
-    unsigned long long* d = pk64 + (uint64_t)pk0 * kSlabs;
+  const uint32_t pk0 = m ? __builtin_amdgcn_readlane(pk, __ffsll((long long)m) - 1) : kEnd;
+  if (m != 0 && __ballot(pk != kEnd && pk != pk0) == 0) {
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {
       // one slab pair at a time: the reductions are not interleaved, which
@@ -532,18 +533,53 @@ __device__ __forceinline__ void patch_union64(unsigned long long* pk64, uint32_t
       const double lo = wave_min(kd[2 * a]);
       const double hi = wave_max(kd[2 * a + 1]);
       if (lane_id() == 0) {
-        atomicMin(d + 2 * a, (unsigned long long)ordered_u64(lo));
-        atomicMax(d + 2 * a + 1, (unsigned long long)ordered_u64(hi));
+        part[2 * a] = lo;
+        part[2 * a + 1] = hi;
       }
       __builtin_amdgcn_sched_barrier(0);
     }
-  } else if (pk != kEnd) {
-    unsigned long long* d = pk64 + (uint64_t)pk * kSlabs;
+    if (lane_id() == 0) *ppatch = pk0;
+  } else {
+    if (lane_id() == 0) *ppatch = kEnd;
+    if (pk != kEnd) {
+      unsigned long long* d = pk64 + (uint64_t)pk * kSlabs;
 #pragma unroll
-    for (int a = 0; a < kAxes; ++a) {
-      atomicMin(d + 2 * a, (unsigned long long)ordered_u64(kd[2 * a]));
-      atomicMax(d + 2 * a + 1, (unsigned long long)ordered_u64(kd[2 * a + 1]));
+      for (int a = 0; a < kAxes; ++a) {
+        atomicMin(d + 2 * a, (unsigned long long)ordered_u64(kd[2 * a]));
+        atomicMax(d + 2 * a + 1, (unsigned long long)ordered_u64(kd[2 * a + 1]));
+      }
     }
+  }
+}
+
+// After a barrier: lane i < 26 of the first wave folds slab i of the waves'
+// partials, patch by patch, and issues one atomic per patch and slab (a
+// quarter of the atomics of one set per wave, issued 26 lanes at a time).
+template <int NW>
+__device__ __forceinline__ void patch_flush(unsigned long long* pk64, const double (*part)[kSlabs],
+                                            const uint32_t* ppatch) {
+  const uint32_t i = threadIdx.x;
+  if (i >= kSlabs) return;
+  const bool hi = i & 1;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t P = ppatch[w];
+    bool first = P != kEnd;
+#pragma unroll
+    for (int v = 0; v < w; ++v) first = first && ppatch[v] != P;
+    if (!first) continue;
+    // folded in the ordered-integer domain of the atomics, so the result is
+    // the one that one atomic per wave gives
+    unsigned long long x = ordered_u64(part[w][i]);
+#pragma unroll
+    for (int v = w + 1; v < NW; ++v) {
+      if (ppatch[v] != P) continue;
+      const unsigned long long y = ordered_u64(part[v][i]);
+      x = hi ? (y > x ? y : x) : (y < x ? y : x);
+    }
+    unsigned long long* d = pk64 + (uint64_t)P * kSlabs + i;
+    if (hi) atomicMax(d, x);
+    else atomicMin(d, x);
   }
 }
 
@@ -701,7 +737,9 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   }
   // exact patch slabs for the patch broadphase (fp32 form; the fp64 form
   // takes them from the patch roots)
-  if constexpr (FP32) patch_union64(b.patch_k64, pk, kd);
+  __shared__ double s_part[FP32 ? kTreeChunk / kWave : 1][kSlabs];
+  __shared__ uint32_t s_ppatch[FP32 ? kTreeChunk / kWave : 1];
+  if constexpr (FP32) patch_partial(b.patch_k64, pk, kd, s_part[t / kWave], &s_ppatch[t / kWave]);
   SlabT<FP32> ks[kSlabs];
   const uint32_t leaf = (uint32_t)(n - 1 + p);
   if (act) {
@@ -744,7 +782,8 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       rec_store_half<FP32>(b, is_left ? p : p - 1, is_left ? 0 : 1, ks, leaf, e);
     }
   }
-  __syncthreads();  // counters, deltas and leaf slots ready
+  __syncthreads();  // counters, deltas, leaf slots and patch partials ready
+  if constexpr (FP32) patch_flush<kTreeChunk / kWave>(b.patch_k64, s_part, s_ppatch);
   if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L);
   __syncthreads();
   if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);

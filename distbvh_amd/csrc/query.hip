@@ -139,6 +139,50 @@ __device__ __forceinline__ int32_t min_gap(const int32_t* q, P b) {
   return m;
 }
 
+// The same v_cmpx chain on ordered fp32 images (int32 compares): the strict
+// overlap of the lane's query slabs with one wave-uniform box, for internal
+// children of the fp32 walk (a leaf child needs the gap, min_gap).
+__device__ __forceinline__ bool box_hits32(const int32_t* q, const int32_t* b) {
+  unsigned long long res, save, m;
+  int v;
+  asm volatile(
+      "s_mov_b64 %[save], exec\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q0], %[b1]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b0], %[q1]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q2], %[b3]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b2], %[q3]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q4], %[b5]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b4], %[q5]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q6], %[b7]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b6], %[q7]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q8], %[b9]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b8], %[q9]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q10], %[b11]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b10], %[q11]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q12], %[b13]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b12], %[q13]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q14], %[b15]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b14], %[q15]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q16], %[b17]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b16], %[q17]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q18], %[b19]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b18], %[q19]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q20], %[b21]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b20], %[q21]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q22], %[b23]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b22], %[q23]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[q24], %[b25]\n\t"
+      "v_cmpx_lt_i32_e64 %[m], %[b24], %[q25]\n\t"
+      "s_nop 4\n\t"
+      "s_mov_b64 %[res], exec\n\t"
+      "s_mov_b64 exec, %[save]\n\t"
+      "v_cndmask_b32_e64 %[v], 0, 1, %[res]\n\t"
+      : [res] "=&s"(res), [save] "=&s"(save), [m] "=&s"(m), [v] "=&v"(v)
+      : [q0] "v"(q[0]), [q1] "v"(q[1]), [q2] "v"(q[2]), [q3] "v"(q[3]), [q4] "v"(q[4]), [q5] "v"(q[5]), [q6] "v"(q[6]), [q7] "v"(q[7]), [q8] "v"(q[8]), [q9] "v"(q[9]), [q10] "v"(q[10]), [q11] "v"(q[11]), [q12] "v"(q[12]), [q13] "v"(q[13]), [q14] "v"(q[14]), [q15] "v"(q[15]), [q16] "v"(q[16]), [q17] "v"(q[17]), [q18] "v"(q[18]), [q19] "v"(q[19]), [q20] "v"(q[20]), [q21] "v"(q[21]), [q22] "v"(q[22]), [q23] "v"(q[23]), [q24] "v"(q[24]), [q25] "v"(q[25]),
+        [b0] "s"(b[0]), [b1] "s"(b[1]), [b2] "s"(b[2]), [b3] "s"(b[3]), [b4] "s"(b[4]), [b5] "s"(b[5]), [b6] "s"(b[6]), [b7] "s"(b[7]), [b8] "s"(b[8]), [b9] "s"(b[9]), [b10] "s"(b[10]), [b11] "s"(b[11]), [b12] "s"(b[12]), [b13] "s"(b[13]), [b14] "s"(b[14]), [b15] "s"(b[15]), [b16] "s"(b[16]), [b17] "s"(b[17]), [b18] "s"(b[18]), [b19] "s"(b[19]), [b20] "s"(b[20]), [b21] "s"(b[21]), [b22] "s"(b[22]), [b23] "s"(b[23]), [b24] "s"(b[24]), [b25] "s"(b[25]));
+  return v != 0;
+}
+
 // Per-wave LDS staging of the candidates queued for the fp64 re-check: one
 // atomic on the list's counter per flush (mesh self collision queues every
 // pair of elements that share a vertex, so these are many).
@@ -252,22 +296,31 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     __builtin_amdgcn_sched_barrier(0);
     const uint32_t cl = (uint32_t)rw[52], cr = (uint32_t)rw[53];
     const uint32_t el = (uint32_t)rw[54], er = (uint32_t)rw[55];
-    const int32_t gl = min_gap(qf, rw);
-    const int32_t gr = min_gap(qf, rw + kSlabs);
-    bool hl = gl > 0, hr = gr > 0;
+    // internal children: overlap only (v_cmpx chain); leaf children: the gap,
+    // which also decides whether fp32 proves the fp64 overlap
+    const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
+    int32_t gl = 0, gr = 0;
+    bool hl, hr;
+    if (leaf_l) {
+      gl = min_gap(qf, rw);
+      hl = gl > 0;
+    } else {
+      hl = box_hits32(qf, rw);
+    }
+    if (leaf_r) {
+      gr = min_gap(qf, rw + kSlabs);
+      hr = gr > 0;
+    } else {
+      hr = box_hits32(qf, rw + kSlabs);
+    }
     if (SELF) {
       hl = hl && (uint64_t)x > p;  // the left child ends at position x
       hr = hr && (uint64_t)rlast > p;
     }
-    bool go_l = false, go_r = false;
-    if (__ballot(hl) != 0) {
-      if (cl >= tleaf0) leaf_check(gl, el, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
-      else go_l = true;
-    }
-    if (__ballot(hr) != 0) {
-      if (cr >= tleaf0) leaf_check(gr, er, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
-      else go_r = true;
-    }
+    const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
+    if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
+    if (any_r && leaf_r) leaf_check(gr, er, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
+    const bool go_l = any_l && !leaf_l, go_r = any_r && !leaf_r;
     // depth first: continue with the left child, keep the right one
     if (go_l && go_r) {
       if (sp >= kStack) {
