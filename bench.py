@@ -60,15 +60,25 @@ def kernel_bytes(stage, n, npairs, n_other, fp32=False, elem_bytes=72):
 
 
 # stage -> the kernel whose measured HBM traffic the roofline reports
-STAGE_KERNEL = {"narrowphase": ("k_narrow64<false>", "k_narrow32<false>"),
-                "tree": ("k_tree_leaves<0, 3, false>", "k_tree_leaves<0, 3, true>"),
+STAGE_KERNEL = {"tree": ("k_tree_leaves<0, 3, false>", "k_tree_leaves<0, 3, true>"),
                 "sort": ("k_sort_pass<unsigned long>", "k_sort_pass<unsigned long>")}
 
 
-def measured_traffic(stage, workload, fp32):
+def stage_kernel(stage, fp32, launch):
+    """The stage's main kernel as rocprofv3 names it (the fp64 walk depends
+    on the launch batch: one record in scalar registers, or K staged in LDS)."""
+    if stage == "narrowphase":
+        if fp32:
+            return "k_narrow32<false>"
+        b = launch.get("batch", 1) if launch else 1
+        return "k_narrow64s<false>" if b == 1 else "k_narrow64<false, %d>" % b
+    return STAGE_KERNEL.get(stage, (None, None))[1 if fp32 else 0]
+
+
+def measured_traffic(stage, workload, fp32, launch=None):
     """HBM bytes per launch of the stage's main kernel from the rocprofv3
     FETCH_SIZE / WRITE_SIZE table under profiles/ (None if absent)."""
-    name = STAGE_KERNEL.get(stage, (None, None))[1 if fp32 else 0]
+    name = stage_kernel(stage, fp32, launch)
     path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (workload, "fp32" if fp32 else "fp64"))
     if name is None or not os.path.exists(path):
         return None
@@ -178,14 +188,20 @@ def bench_sheets(args, torch, stream):
             oa.broadphase(ob)
             return oa.result_count()
 
+        step()
+        # each form timed at its fastest launch shape
+        tune = world.launch() if args.no_tune else oa.autotune(ob)
+        world.stage_times()
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = oa.result_stats()
         wl = "wavy_sheets_2M" if args.nq == 1024 and args.od == 256 else None
         roof, step_frac = roofline_of(stages, per_launch, counts,
                                       lambda k: kernel_bytes(k, n, npairs, n, fp32), ms,
-                                      (lambda k: measured_traffic(k, wl, fp32)) if wl else None)
+                                      (lambda k: measured_traffic(k, wl, fp32, tune)) if wl else None)
+        roof["kernel_name"] = stage_kernel(roof["kernel"], fp32, tune)
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
-                     "roofline": roof, "step_hbm_frac": step_frac, "digest": oa.digest()}
+                     "roofline": roof, "step_hbm_frac": step_frac, "digest": oa.digest(),
+                     "launch": {k: tune[k] for k in ("batch", "block_size", "xcd_remap")}}
         world.close()
     return a, b, n, out
 
@@ -213,6 +229,10 @@ def bench_spheres_self(args, torch, stream):
             o.self_collision()
             return o.result_count()
 
+        step()
+        # each form timed at its fastest launch shape
+        tune = world.launch() if args.no_tune else o.autotune()
+        world.stage_times()
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = o.result_stats()
 
@@ -223,7 +243,8 @@ def bench_spheres_self(args, torch, stream):
 
         roof, step_frac = roofline_of(stages, per_launch, counts, nb, ms)
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
-                     "roofline": roof, "step_hbm_frac": step_frac, "digest": o.digest()}
+                     "roofline": roof, "step_hbm_frac": step_frac, "digest": o.digest(),
+                     "launch": {k: tune[k] for k in ("batch", "block_size", "xcd_remap")}}
         world.close()
     return n, out
 
@@ -308,6 +329,9 @@ def main():
                     help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
                          "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape instead of autotuning it before timing "
+                         "(profiling runs: no tuner launches in the trace)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized device buffer once (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -385,6 +409,7 @@ def main():
                    "candidate_pairs": int(st["candidate_pairs"]),
                    "rechecked_pairs": int(st["rechecked_pairs"]),
                    "node_visits_per_wave": round(st["node_visits"] / waves, 1),
+                   "launch_autotuned": first["launch"],
                    "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in first["stages"].items()},
         "step_hbm_frac": first["step_hbm_frac"],
@@ -397,6 +422,7 @@ def main():
             "fp32_speedup": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
             "pairs_identical": bool(r64["npairs"] == r32["npairs"] and r64["digest"] == r32["digest"]),
             "fp32_stages_ms": {k: round(v, 4) for k, v in r32["stages"].items()},
+            "fp32_launch_autotuned": r32["launch"],
             "fp32_candidate_pairs": int(r32["stats"]["candidate_pairs"]),
             "fp32_rechecked_pairs": int(r32["stats"]["rechecked_pairs"]),
             "fp32_max_slab_error_ulps": r32["stats"]["max_slab_error_ulps"],

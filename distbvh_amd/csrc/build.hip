@@ -446,15 +446,29 @@ __device__ __forceinline__ bool is_patch_root(uint32_t patch, uint32_t l, uint32
   return patch != kMixed && (l == 0 || dl >= 128) && (r == n - 1 || dr >= 128);
 }
 
-// Global-memory climb (phases >= 2).  Node [l, r] of patch `mypatch` has
-// written its slabs, id and element into its half of its parent's record; dl
-// = delta[l-1] and dr = delta[r] (0 at the ends).  It climbs until it is the
-// first arrival at a parent, finishes the root, or is a child whose parent
-// splits on the chunk boundary [c0, c1).
+constexpr uint32_t kTreeChunk = 256;
+
+// Upper-phase climb (phases >= 2), one thread per handed-over node of the
+// workgroup's chunk [c0, c1).  Node [l, r] of patch `mypatch` with slabs kd
+// (its half of its parent's record is written); dl = delta[l-1] and dr =
+// delta[r] (0 at the ends).  Every arrival deposits the node's slabs, far end
+// and patch in the thread's LDS slot, then exchanges the parent's arrival
+// word for bit 63 | slot: the first arriver stops; the second finds its
+// sibling in that LDS slot, or, for a word left by phase 1 (far end + 1 in
+// the high half, patch in the low half; below 2^62 since n < 2^30), in the
+// sibling's half of the parent's record, written by an earlier launch.  So a
+// level costs one workgroup-scope atomic and LDS latency; record halves are
+// stored for the queries and the next phase, and nothing in this launch
+// waits for those stores.  The climb stops at the root or at a child whose
+// parent splits on the chunk boundary (handed to the next phase).  A first
+// arrival returns its parent in *pend (else kEnd): if its sibling arrives
+// only in a later launch, k_tree_up rewrites the word in the phase-1 format.
 template <bool FP32>
 __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t l, uint32_t r, uint32_t mypatch,
-                                      uint32_t dl, uint32_t dr, uint32_t c0, uint32_t c1, FrontNode* s_front,
-                                      uint32_t* s_nfront) {
+                                      uint32_t dl, uint32_t dr, SlabT<FP32>* kd, uint32_t c0, uint32_t c1,
+                                      uint32_t slot, SlabT<FP32> (*s_slab)[kSlabs], uint32_t* s_far,
+                                      uint32_t* s_patch, FrontNode* s_front, uint32_t* s_nfront, uint32_t* pend) {
+  *pend = kEnd;
   RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
     const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
@@ -465,19 +479,40 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t l
       else atomicOr(b.status, kErrTree);
       return;
     }
-    const unsigned long long mine = ((unsigned long long)((is_left ? l : r) + 1u) << 32) | mypatch;
-    const unsigned long long old =
-        __hip_atomic_exchange(&b.counters[parent], mine, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0) return;  // first arriver: the sibling finishes the parent
-    const uint32_t far = (uint32_t)(old >> 32) - 1u, spatch = (uint32_t)old;
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) s_slab[slot][i] = kd[i];
+    s_far[slot] = is_left ? l : r;
+    s_patch[slot] = mypatch;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const unsigned long long old = __hip_atomic_exchange(&b.counters[parent], (1ull << 63) | slot,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (old == 0) {  // first arriver: the sibling finishes the parent
+      *pend = parent;
+      return;
+    }
+    uint32_t far, spatch;
+    SlabT<FP32> ks[kSlabs];
+    if (old >> 63) {
+      const uint32_t ss = (uint32_t)old & 0x7fffffffu;
+      if (ss >= 2 * kTreeChunk) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      far = s_far[ss];
+      spatch = s_patch[ss];
+#pragma unroll
+      for (int i = 0; i < kSlabs; ++i) ks[i] = s_slab[ss][i];
+    } else {
+      far = (uint32_t)(old >> 32) - 1u;
+      spatch = (uint32_t)old;
+      rec_load_half<FP32>(b, parent, is_left ? 1 : 0, ks);
+    }
     // a left child's sibling ends past r, a right child's starts before l
     if (is_left ? (far <= r || far >= n) : (far >= l)) {
       atomicOr(b.status, kErrTree);
       return;
     }
-    SlabT<FP32> kd[kSlabs], ks[kSlabs];
-    rec_load_half<FP32>(b, parent, 0, kd);
-    rec_load_half<FP32>(b, parent, 1, ks);
     if (is_left) {
       r = far;
       dr = (r != n - 1) ? b.delta[r] : 0u;
@@ -504,7 +539,6 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t l
   }
 }
 
-constexpr uint32_t kTreeChunk = 256;
 
 __device__ __forceinline__ void empty_kdop(double* kd) {
 #pragma unroll
@@ -795,11 +829,14 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
 
 // Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one
 // thread per handed-over node (two per sub-chunk).  A handed-over node's
-// slabs are already in its half of its parent's record.
+// slabs are in its half of its parent's record; thread t's LDS slot is t.
 template <bool FP32>
 __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_t n, uint64_t chunk, uint64_t nsub,
                                                             const FrontNode* __restrict__ front_in,
                                                             FrontNode* __restrict__ front_out) {
+  __shared__ SlabT<FP32> s_slab[2 * kTreeChunk][kSlabs];
+  __shared__ uint32_t s_far[2 * kTreeChunk];
+  __shared__ uint32_t s_patch[2 * kTreeChunk];
   __shared__ FrontNode s_front[2];
   __shared__ uint32_t s_nfront;
   const uint32_t t = threadIdx.x;
@@ -809,14 +846,31 @@ __global__ __launch_bounds__(2 * kTreeChunk) void k_tree_up(BuildBufs b, uint64_
   if (t == 0) s_nfront = 0;
   __syncthreads();
   const uint64_t slot = (uint64_t)blockIdx.x * (2 * kTreeChunk) + t;  // = 2 * sub + i
+  uint32_t pend = kEnd;
   if (slot < 2 * nsub) {
     const FrontNode f = front_in[slot];
     if (f.l != kEnd) {
-      if (f.l > f.r || f.r >= n || (f.l == 0 && f.r == n - 1)) atomicOr(b.status, kErrTree);  // never the root
-      else climb<FP32>(b, n, f.l, f.r, f.patch, f.d & 0xffu, f.d >> 8, c0, c1, s_front, &s_nfront);
+      if (f.l > f.r || f.r >= n || (f.l == 0 && f.r == n - 1)) {
+        atomicOr(b.status, kErrTree);  // never the root
+      } else {
+        const uint32_t dl = f.d & 0xffu, dr = f.d >> 8;
+        const bool is_left = (f.l == 0) || (f.r != n - 1 && dr < dl);
+        SlabT<FP32> kd[kSlabs];
+        rec_load_half<FP32>(b, is_left ? f.r : f.l - 1, is_left ? 0 : 1, kd);
+        climb<FP32>(b, n, f.l, f.r, f.patch, dl, dr, kd, c0, c1, t, s_slab, s_far, s_patch, s_front, &s_nfront,
+                    &pend);
+      }
     }
   }
   __syncthreads();
+  // a first arrival whose deposit is still in the word: its sibling arrives in
+  // a later launch, which needs the far end and patch from the word itself
+  // (the slabs are in the record half)
+  if (pend != kEnd) {
+    const unsigned long long w =
+        __hip_atomic_load(&b.counters[pend], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (w == ((1ull << 63) | t)) b.counters[pend] = ((unsigned long long)(s_far[t] + 1u) << 32) | s_patch[t];
+  }
   if (t < 2) front_out[2 * blockIdx.x + t] = (t < s_nfront) ? s_front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
 }
 

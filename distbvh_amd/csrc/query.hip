@@ -242,6 +242,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   const uint32_t n = (uint32_t)t.n;
   const uint32_t tleaf0 = n - 1;
   const uint32_t lane = lane_id();
+  const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
   CRec32* recs = (CRec32*)t.rec32;
   uint32_t steps = 0;
   uint32_t ncand = 0;
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   {
     const int32_t g = min_gap(qf, (CInt*)t.rootbox32);
     bool h = g > 0;
-    if (SELF) h = h && (uint64_t)(n - 1) > p;  // the root's last position
+    if (SELF) h = h & ((n - 1) > p32);  // the root's last position
     if (n == 1) {
       if (!SELF) leaf_check(g, uniform(t.perm[0]), h);
     } else if (__ballot(h) != 0) {
@@ -314,12 +315,13 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
       hr = box_hits32(qf, rw + kSlabs);
     }
     if (SELF) {
-      hl = hl && (uint64_t)x > p;  // the left child ends at position x
-      hr = hr && (uint64_t)rlast > p;
+      // position bounds (non-short-circuit &: no divergent branch)
+      hl = hl & (x > p32);  // the left child ends at position x
+      hr = hr & (rlast > p32);
     }
     const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
-    if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl && (uint64_t)(cl - tleaf0) > p) : hl);
-    if (any_r && leaf_r) leaf_check(gr, er, SELF ? (hr && (uint64_t)(cr - tleaf0) > p) : hr);
+    if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl & (cl - tleaf0 > p32)) : hl);
+    if (any_r && leaf_r) leaf_check(gr, er, SELF ? (hr & (cr - tleaf0 > p32)) : hr);
     const bool go_l = any_l && !leaf_l, go_r = any_r && !leaf_r;
     // depth first: continue with the left child, keep the right one
     if (go_l && go_r) {
