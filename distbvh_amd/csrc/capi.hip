@@ -164,6 +164,14 @@ struct bvh_object_s {
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;
+  // geom_axis / ml_geom_axis: the splitter's buffers and the reference order
+  // (position -> input index) that patch centroids and patch elements follow;
+  // the tree's own order (perm) is patch-major, Morton order inside a patch
+  uint32_t split = BVH_SPLIT_CLUSTERING;
+  DevBuf sp_rp, sp_rp_b, sp_scan, sp_tmpf, sp_tmpt, sp_seg, sp_seg_next, sp_k64, sp_acc, sp_axis, sp_flag, sp_lb,
+      sp_ctr, sp_key, key0, key_a, key_b, sp_hist;
+  const uint32_t* ref_rp = nullptr;
+  const uint32_t* ref_perm() const { return split == BVH_SPLIT_CLUSTERING ? perm.as<uint32_t>() : ref_rp; }
   // results
   DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count, recheck, digest;
   uint64_t npairs = 0;
@@ -187,7 +195,9 @@ struct bvh_object_s {
                            &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
                            &root, &rec64, &rootbox64, &rec32, &rootbox32, &status,
                            &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
-                           &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest};
+                           &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
+                           &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
+                           &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist};
     size_t t = 0;
     for (const DevBuf* d : all) t += d->bytes;
     return t;
@@ -297,6 +307,77 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.front_a = o->front_a.as<FrontNode>();
   b.front_b = o->front_b.as<FrontNode>();
   b.patch_k64 = o->patch_k64.as<unsigned long long>();
+  if (o->split != BVH_SPLIT_CLUSTERING) {
+    BVH_TRY(o->sp_rp.ensure(n1 * 4));
+    BVH_TRY(o->sp_rp_b.ensure(n1 * 4));
+    BVH_TRY(o->sp_scan.ensure((n1 + 1) * 4));
+    BVH_TRY(o->sp_tmpf.ensure(n1 * 4));
+    BVH_TRY(o->sp_tmpt.ensure(n1 * 4));
+    BVH_TRY(o->sp_seg.ensure(2 * od * 4));
+    BVH_TRY(o->sp_seg_next.ensure(2 * od * 4));
+    BVH_TRY(o->sp_k64.ensure(od * kSlabs * 8));
+    BVH_TRY(o->sp_acc.ensure(od * 5 * sizeof(double)));
+    BVH_TRY(o->sp_axis.ensure(od * 4));
+    BVH_TRY(o->sp_flag.ensure(od * 2 * 4));
+    BVH_TRY(o->sp_lb.ensure(sort_tiles(n1 + 1) * 8));
+    BVH_TRY(o->sp_ctr.ensure(4));
+    BVH_TRY(o->sp_key.ensure(n1 * 4));
+    BVH_TRY(o->key0.ensure(n1 * 4));
+    BVH_TRY(o->key_a.ensure(n1 * 4));
+    BVH_TRY(o->key_b.ensure(n1 * 4));
+    BVH_TRY(o->sp_hist.ensure(2 * 256 * 4));
+  }
+  return BVH_OK;
+}
+
+// geom_axis / ml_geom_axis: the reference order and split offsets from the
+// splitter, then the tree order: the Morton order (already in perm and
+// codes_a) stably sorted by patch, and the codes gathered into that order, so
+// that every patch is a contiguous, Morton-sorted range of the tree order.
+static bvh_status split_and_order(bvh_object_s* o) {
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  const uint64_t n = o->in.n, od = w->od;
+  SplitInput si{};
+  si.n = n;
+  if (o->in.kind == kPoly) si.cplanes = o->centroid.as<double>();
+  else if (o->in.kind == kSphere) si.cplanes = o->in.data;  // planes x, y, z, r
+  else si.caos = o->in.centroids;
+  SplitBufs sb{};
+  sb.rp = o->sp_rp.as<uint32_t>();
+  sb.rp_b = o->sp_rp_b.as<uint32_t>();
+  sb.scan = o->sp_scan.as<uint32_t>();
+  sb.tmpf = o->sp_tmpf.as<uint32_t>();
+  sb.tmpt = o->sp_tmpt.as<uint32_t>();
+  sb.seg = o->sp_seg.as<uint32_t>();
+  sb.seg_next = o->sp_seg_next.as<uint32_t>();
+  sb.seg_k64 = o->sp_k64.as<unsigned long long>();
+  sb.seg_acc = o->sp_acc.as<double>();
+  sb.seg_axis = o->sp_axis.as<int32_t>();
+  sb.seg_flag = o->sp_flag.as<uint32_t>();
+  sb.scan_lb = o->sp_lb.as<uint64_t>();
+  sb.scan_ctr = o->sp_ctr.as<uint32_t>();
+  uint32_t* rp = nullptr;
+  uint32_t* patch_key = o->sp_key.as<uint32_t>();
+  HIP_TRY(launch_split(o->in, si, sb, od, o->split == BVH_SPLIT_ML_GEOM_AXIS ? 1 : 0, o->bb.splits, patch_key, &rp, s));
+  o->ref_rp = rp;
+  // stable sort of the Morton order by patch
+  HIP_TRY(launch_gather_by<uint32_t>(patch_key, o->bb.perm, n, o->key0.as<uint32_t>(), s));
+  const int npass = od <= 256 ? 1 : (od <= 65536 ? 2 : 4);
+  HIP_TRY(launch_fill_u32(o->sp_hist.as<uint32_t>(), 2 * 256, 0u, s));
+  HIP_TRY(launch_fill_u32(o->bb.lookback, (uint64_t)(npass > 2 ? 2 : npass) * sort_tiles(n) * 256, 0u, s));
+  if (npass > 2) return fail(BVH_E_UNSUPPORTED, "geom_axis splitting supports at most 65536 patches");
+  HIP_TRY(launch_sort_histogram<uint32_t>(o->key0.as<uint32_t>(), n, npass, o->sp_hist.as<uint32_t>(), s));
+  SortArgs<uint32_t> a{};
+  a.k0 = o->key0.as<uint32_t>(); a.ka = o->key_a.as<uint32_t>(); a.kb = o->key_b.as<uint32_t>();
+  a.v0 = o->bb.perm; a.va = o->bb.perm_b; a.vb = sb.tmpf;  // tmpf is free once the splitter is done
+  a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl + 1; a.lookback = o->bb.lookback;
+  HIP_TRY(launch_sort_passes<uint32_t>(a, npass, o->sp_hist.as<uint32_t>(), s));
+  HIP_TRY(launch_copy<uint32_t>(o->bb.perm, o->bb.perm_b, n, s));
+  if (o->bits == 63)
+    HIP_TRY(launch_gather_by<uint64_t>((const uint64_t*)o->bb.codes0, o->bb.perm, n, (uint64_t*)o->bb.codes_a, s));
+  else
+    HIP_TRY(launch_gather_by<uint32_t>((const uint32_t*)o->bb.codes0, o->bb.perm, n, (uint32_t*)o->bb.codes_a, s));
   return BVH_OK;
 }
 
@@ -337,9 +418,15 @@ static bvh_status object_build(bvh_object_s* o) {
     HIP_TRY(launch_sort_passes<uint32_t>(a, npass, o->bb.hist, s));
   }
   w->stage_end();
-  w->stage_begin("cluster");
-  HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
-  w->stage_end();
+  if (o->split == BVH_SPLIT_CLUSTERING) {
+    w->stage_begin("cluster");
+    HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
+    w->stage_end();
+  } else {
+    w->stage_begin("split");
+    BVH_TRY(split_and_order(o));
+    w->stage_end();
+  }
   if (o->fp32) {
     // the world's fp32 frame: the Morton bounds minimum of its first non-empty build
     HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
@@ -380,8 +467,10 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (!o) return fail(BVH_E_INVALID, "null object");
   if (split != BVH_SPLIT_CLUSTERING && split != BVH_SPLIT_GEOM_AXIS && split != BVH_SPLIT_ML_GEOM_AXIS)
     return fail(BVH_E_INVALID, "unknown split algorithm");
-  if (split != BVH_SPLIT_CLUSTERING)
-    return fail(BVH_E_UNSUPPORTED, "only split_algorithm::clustering is built on the GPU path");
+  // geom_axis splits log2(od) times: od must be a power of two
+  // (collision_object.cpp:122-124 asserts splits + 1 == od)
+  if (split != BVH_SPLIT_CLUSTERING && !is_pow2(o ? o->w->od : 0))
+    return fail(BVH_E_INVALID, "geom_axis / ml_geom_axis need a power-of-two overdecomposition factor");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
@@ -394,6 +483,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   o->in.n = n;
   o->bits = (int)w->cfg.morton_bits;
   o->fp32 = w->cfg.precision == BVH_PREC_FP32_OUTWARD ? 1 : 0;
+  o->split = split;
   size_t planes = kind == kPoly ? (size_t)nv * 3 : (kind == kSphere ? 4 : kSlabs);
   size_t bytes = planes * n * sizeof(double);
   if (w->capturing && (mem == BVH_MEM_HOST || gids))
@@ -1046,7 +1136,7 @@ bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
     const double* cpl = o->in.kind == kPoly ? o->centroid.as<double>() : nullptr;
     const double* spl = o->in.kind == kSphere ? o->in.data : nullptr;
     const double* cao = o->in.kind == kSnapshot ? o->in.centroids : nullptr;
-    HIP_TRY(launch_patch_centroids(cpl, spl, cao, o->perm.as<uint32_t>(), o->splits.as<uint32_t>(), n, od,
+    HIP_TRY(launch_patch_centroids(cpl, spl, cao, o->ref_perm(), o->splits.as<uint32_t>(), n, od,
                                    dc.as<double>(), s));
     BVH_TRY(copy_out(cent.data(), dc.p, 3 * od * sizeof(double), BVH_MEM_HOST, s));
   }
@@ -1083,7 +1173,7 @@ bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t* indices, uin
   if (cap < e - b) return fail(BVH_E_OVERFLOW, "index buffer too small");
   if (e == b) return BVH_OK;
   std::vector<uint32_t> tmp(e - b);
-  BVH_TRY(copy_out(tmp.data(), o->perm.as<uint32_t>() + b, (e - b) * 4, BVH_MEM_HOST, o->w->stream));
+  BVH_TRY(copy_out(tmp.data(), o->ref_perm() + b, (e - b) * 4, BVH_MEM_HOST, o->w->stream));
   for (uint64_t i = 0; i < e - b; ++i) indices[i] = tmp[i];
   return BVH_OK;
 }

@@ -117,6 +117,36 @@ hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const ui
                                hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
 
+// ---- geom_axis / ml_geom_axis splitters (split.hip) -------------------------------
+// element centroids: 3 planes of n (cplanes), or n x 3 (caos)
+struct SplitInput {
+  const double* cplanes;
+  const double* caos;
+  uint64_t n;
+};
+struct SplitBufs {
+  uint32_t* rp;        // n: position -> input index (the reference order)
+  uint32_t* rp_b;      // n
+  uint32_t* scan;      // n + 1: test bits, then their exclusive scan
+  uint32_t* tmpf;      // n
+  uint32_t* tmpt;      // n
+  uint32_t* seg;       // 2 od: range begin at [s], end at [od + s]
+  uint32_t* seg_next;  // 2 od
+  unsigned long long* seg_k64;  // od x 26: union of the range's element k-DOPs (ordered u64)
+  double* seg_acc;     // od x 5: sum, sum of magnitudes, interval lo, hi, exact split point
+  int32_t* seg_axis;   // od
+  uint32_t* seg_flag;  // od x 2: undecided elements, partition point (kEnd: nothing moves)
+  uint64_t* scan_lb;   // sort_tiles(n + 1) words
+  uint32_t* scan_ctr;  // 1
+};
+// The permutation (*result, one of sb.rp / sb.rp_b), the od - 1 split
+// offsets in the reference's order and the patch of every input element;
+// od a power of two.
+hipError_t launch_split(const ElemInput& in, const SplitInput& si, SplitBufs sb, uint64_t od, int ml,
+                        uint32_t* splits, uint32_t* patch_key, uint32_t** result, hipStream_t s);
+template <typename T>
+hipError_t launch_gather_by(const T* src, const uint32_t* idx, uint64_t n, T* dst, hipStream_t s);
+
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {
   const Rec64* rec64;

@@ -766,7 +766,17 @@ void orc_object_build(orc_object *o, const orc_pipeline_cfg *cfg, orc_pipeline_t
   orc_compute_bounds(o->kdops, n, mn, inv);
   double t1 = orc_now();
   for (size_t i = 0; i < n; ++i) o->perm[i] = (uint32_t)i;
-  if (cfg->morton_bits == 30) {
+  if (cfg->split == ORC_SPLIT_GEOM_AXIS || cfg->split == ORC_SPLIT_ML_GEOM_AXIS) {
+    /* set_entity_data_geom_axis / _ml_geom_axis (collision_object.hpp:133-162):
+     * depth = bit_log2(od); od must be a power of two (collision_object.cpp:
+     * 122-124 asserts splits + 1 == od) */
+    int depth = 0;
+    while (((size_t)1 << (depth + 1)) <= od) ++depth;
+    if (cfg->split == ORC_SPLIT_GEOM_AXIS) orc_split_geom_axis(o->kdops, o->centroids, n, depth, o->perm, o->splits);
+    else orc_split_ml_geom_axis(o->kdops, o->centroids, n, depth, o->perm, o->splits);
+    for (size_t i = 0; i < n; ++i) o->codes[i] = 0;
+    if (tm) tm->t_cluster += orc_now() - t1;
+  } else if (cfg->morton_bits == 30) {
     uint32_t *c32 = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
     orc_morton_codes32(o->centroids, n, mn, inv, c32);
     double t2 = orc_now();
@@ -798,6 +808,122 @@ void orc_object_build(orc_object *o, const orc_pipeline_cfg *cfg, orc_pipeline_t
   free(sc);
   double t6 = orc_now();
   if (tm) { tm->t_bounds += t1 - t0; tm->t_patches += t6 - t5; }
+}
+
+/* ---- geom_axis / ml_geom_axis element splitters ---------------------------------- */
+
+/* from_kdops over idx[b..e) (kdop.hpp:174-214: slab-wise m::min / m::max in
+ * order) and axis::longest (split/axis.hpp:42-49, kdop.hpp:354-362). */
+static int split_axis(const double *kdops, const uint32_t *idx, size_t b, size_t e) {
+  double k[ORC_KDOP];
+  orc_kdop_empty(k);
+  for (size_t i = b; i < e; ++i) {
+    const double *s = kdops + (size_t)idx[i] * ORC_KDOP;
+    for (int a = 0; a < ORC_NAXIS; ++a) {
+      k[2 * a] = m_min(k[2 * a], s[2 * a]);
+      k[2 * a + 1] = m_max(k[2 * a + 1], s[2 * a + 1]);
+    }
+  }
+  return orc_longest_axis(k);
+}
+
+/* libstdc++ std::partition (bidirectional iterators) of idx[b..e) by
+ * project(centroid, axis) < split; returns the partition point. */
+static size_t partition_idx(uint32_t *idx, const double *centroids, size_t first, size_t last, int axis,
+                            double split) {
+#define PRED(i) (orc_project(centroids + 3 * (size_t)idx[i], axis) < split)
+  for (;;) {
+    for (;;) {
+      if (first == last) return first;
+      if (PRED(first)) ++first;
+      else break;
+    }
+    --last;
+    for (;;) {
+      if (first == last) return first;
+      if (!PRED(last)) --last;
+      else break;
+    }
+    uint32_t t = idx[first];
+    idx[first] = idx[last];
+    idx[last] = t;
+    ++first;
+  }
+#undef PRED
+}
+
+typedef struct {
+  const double *kdops, *centroids;
+  size_t n;
+  uint32_t *idx;
+  uint64_t *splits;
+  size_t nsplits;
+  int axis;        /* geom_axis: one axis and split point for every call */
+  double split;
+} split_state;
+
+/* split_permutations_recursive_impl (split/split.hpp:220-243) with
+ * split_permutation (split/split.hpp:177-195): the axis and the split point
+ * are taken over all elements at every level (the element span is never
+ * narrowed there). */
+static void geom_rec(split_state *st, int depth, size_t b, size_t e) {
+  size_t sp = partition_idx(st->idx, st->centroids, b, e, st->axis, st->split);
+  if (sp == b || sp == e) sp = b + (e - b) / 2;
+  if (depth > 0) geom_rec(st, depth - 1, b, sp);
+  st->splits[st->nsplits++] = sp;
+  if (depth > 0) geom_rec(st, depth - 1, sp, e);
+}
+
+/* split_permutations<split::mean, axis::longest> (split/split.hpp:309-323):
+ * idx = iota, then 2^depth - 1 split offsets in recursion order.  The split
+ * point is split::mean over the elements in input order (split/mean.hpp:
+ * 49-65: std::accumulate from 0, then / size). */
+void orc_split_geom_axis(const double *kdops, const double *centroids, size_t n, int depth, uint32_t *idx,
+                         uint64_t *splits) {
+  for (size_t i = 0; i < n; ++i) idx[i] = (uint32_t)i;
+  if (depth <= 0) return;
+  split_state st = {kdops, centroids, n, idx, splits, 0, 0, 0.0};
+  st.axis = split_axis(kdops, idx, 0, n);
+  double sum = 0.0;
+  for (size_t i = 0; i < n; ++i) sum = sum + orc_project(centroids + 3 * i, st.axis);
+  st.split = sum / (double)n;
+  geom_rec(&st, depth - 1, 0, n);
+}
+
+/* split_permutation_ml (split/split.hpp:121-158): fewer than 2 elements give
+ * offset 0; the mean is split::mean's span overload (split/mean.hpp:67-79),
+ * a parallel_reduce restated in element order (Kokkos::Serial); the
+ * partition of the (element, index) pairs moves elements and indices
+ * together; an empty side gives size / 2. */
+static size_t split_ml(split_state *st, size_t b, size_t e, int axis) {
+  if (e - b < 2) return 0;
+  double sum = 0.0;
+  for (size_t i = b; i < e; ++i) sum += orc_project(st->centroids + 3 * (size_t)st->idx[i], axis);
+  const double split = sum / (double)(e - b);
+  size_t sp = partition_idx(st->idx, st->centroids, b, e, axis, split);
+  size_t delta = sp - b;
+  if (sp == b || sp == e) delta = (e - b) / 2;
+  return delta;
+}
+
+/* split_permutations_recursive_impl_ml (split/split.hpp:186-218): the axis
+ * from the current range (axis 0 for an empty one). */
+static void ml_rec(split_state *st, int depth, size_t b, size_t e) {
+  int axis = 0;
+  if (e > b) axis = split_axis(st->kdops, st->idx, b, e);
+  const size_t sp = b + split_ml(st, b, e, axis);
+  if (depth > 0) ml_rec(st, depth - 1, b, sp);
+  st->splits[st->nsplits++] = sp;
+  if (depth > 0) ml_rec(st, depth - 1, sp, e);
+}
+
+/* split_permutations_ml<split::mean, axis::longest> (split/split.hpp:285-307). */
+void orc_split_ml_geom_axis(const double *kdops, const double *centroids, size_t n, int depth, uint32_t *idx,
+                            uint64_t *splits) {
+  for (size_t i = 0; i < n; ++i) idx[i] = (uint32_t)i;
+  if (depth <= 0) return;
+  split_state st = {kdops, centroids, n, idx, splits, 0, 0, 0.0};
+  ml_rec(&st, depth - 1, 0, n);
 }
 
 typedef struct {

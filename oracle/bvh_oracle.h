@@ -137,7 +137,20 @@ typedef struct {
   size_t leaf_size;     /* leaf size of the per-patch element trees */
   int nthreads;         /* OpenMP threads for the narrowphase (1 = serial) */
   double sample;        /* fraction of broadphase patch pairs narrowphased (1 = all) */
+  int split;            /* ORC_SPLIT_*: split_algorithm of set_entity_data */
 } orc_pipeline_cfg;
+
+/* split_algorithm, types.hpp:52-57 */
+#define ORC_SPLIT_GEOM_AXIS 0
+#define ORC_SPLIT_ML_GEOM_AXIS 1
+#define ORC_SPLIT_CLUSTERING 2
+
+/* split_permutations / split_permutations_ml with split::mean and
+ * axis::longest: idx (n) and 2^depth - 1 split offsets */
+void orc_split_geom_axis(const double *kdops, const double *centroids, size_t n, int depth, uint32_t *idx,
+                         uint64_t *splits);
+void orc_split_ml_geom_axis(const double *kdops, const double *centroids, size_t n, int depth, uint32_t *idx,
+                            uint64_t *splits);
 
 typedef struct {
   /* per object */

@@ -35,7 +35,11 @@ class OrcPairs(C.Structure):
 
 class OrcCfg(C.Structure):
     _fields_ = [("od", C.c_size_t), ("morton_bits", C.c_int), ("leaf_size", C.c_size_t),
-                ("nthreads", C.c_int), ("sample", C.c_double)]
+                ("nthreads", C.c_int), ("sample", C.c_double), ("split", C.c_int)]
+
+
+# split_algorithm, types.hpp:52-57
+SPLIT_GEOM_AXIS, SPLIT_ML_GEOM_AXIS, SPLIT_CLUSTERING = 0, 1, 2
 
 
 class OrcTimes(C.Structure):
@@ -102,6 +106,8 @@ def lib():
             "orc_collide": (C.POINTER(OrcPairs), [C.POINTER(OrcObject), C.POINTER(OrcObject),
                                                   C.POINTER(OrcCfg), C.POINTER(OrcTimes)]),
             "orc_now": (C.c_double, []),
+            "orc_split_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
+            "orc_split_ml_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -231,6 +237,21 @@ def cluster_splits(sorted_codes, nsplits):
     return out[:nsplits]
 
 
+def split_permutations(kdops, centroids, depth, ml=False):
+    """split_permutations (ml=False) or split_permutations_ml (ml=True) with
+    split::mean and axis::longest: (indices, splits) as element_permutations
+    holds them (split/split.hpp:285-323)."""
+    kd = _c(kdops, np.float64)
+    ce = _c(centroids, np.float64)
+    n = kd.shape[0]
+    idx = np.empty(max(n, 1), np.uint32)
+    ns = (1 << depth) - 1 if depth > 0 else 0
+    sp = np.zeros(max(ns, 1), np.uint64)
+    f = lib().orc_split_ml_geom_axis if ml else lib().orc_split_geom_axis
+    f(_p(kd, dp), _p(ce, dp), n, depth, _p(idx, u32p), _p(sp, u64p))
+    return idx[:n], sp[:ns]
+
+
 def patches(splits, od, sorted_kdops, sorted_centroids):
     n = sorted_kdops.shape[0]
     sp = _c(splits, np.uint64) if len(splits) else np.zeros(1, np.uint64)
@@ -314,10 +335,11 @@ def brute_self(k, nthreads=8):
 # ---- pipeline -------------------------------------------------------------
 class Object:
     """One object built the reference way (bounds, Morton, 1-bit radix sort,
-    cluster splits, patches)."""
+    cluster splits, patches; or the geom_axis / ml_geom_axis splitters with
+    split=SPLIT_GEOM_AXIS / SPLIT_ML_GEOM_AXIS, od a power of two)."""
 
     def __init__(self, kdops, centroids, gids=None, od=1, morton_bits=63, leaf_size=1,
-                 nthreads=1, sample=1.0, times=None):
+                 nthreads=1, sample=1.0, times=None, split=SPLIT_CLUSTERING):
         self.kdops = _c(kdops, np.float64)
         self.centroids = _c(centroids, np.float64)
         n = self.kdops.shape[0]
@@ -329,7 +351,7 @@ class Object:
         self.pend = np.empty(od, np.uint64)
         self.patch_kdops = np.empty((od, 26), np.float64)
         self.patch_centroids = np.empty((od, 3), np.float64)
-        self.cfg = OrcCfg(od, morton_bits, leaf_size, nthreads, sample)
+        self.cfg = OrcCfg(od, morton_bits, leaf_size, nthreads, sample, split)
         self.s = OrcObject(n, _p(self.kdops, dp), _p(self.centroids, dp), _p(self.gids, u64p),
                            _p(self.codes, u64p), _p(self.perm, u32p), _p(self.splits, u64p),
                            _p(self.pbegin, u64p), _p(self.pend, u64p), _p(self.patch_kdops, dp),
