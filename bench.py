@@ -166,6 +166,7 @@ def roofline_of(stages, per_launch, counts, nbytes_fn, ms_per_step, traffic_fn=N
 def bench_sheets(args, torch, stream):
     """Two wavy sheets, both rebuilt per step, A collided with B."""
     from distbvh_amd import CollisionWorld
+    from distbvh_amd import capi
     from distbvh_amd import workloads as W
 
     a, b = W.wavy_sheets(args.nq)
@@ -180,9 +181,12 @@ def bench_sheets(args, torch, stream):
         oa = world.create_collision_object()
         ob = world.create_collision_object()
 
+        split = {"clustering": capi.BVH_SPLIT_CLUSTERING, "geom_axis": capi.BVH_SPLIT_GEOM_AXIS,
+                 "ml_geom_axis": capi.BVH_SPLIT_ML_GEOM_AXIS}[args.split]
+
         def step():
-            oa.set_entity_data_poly(ta)
-            ob.set_entity_data_poly(tb)
+            oa.set_entity_data_poly(ta, split_algorithm=split)
+            ob.set_entity_data_poly(tb, split_algorithm=split)
             oa.init_broadphase()
             ob.init_broadphase()
             oa.broadphase(ob)
@@ -328,6 +332,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="fp64",
                     help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
                          "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
+    ap.add_argument("--split", choices=["clustering", "geom_axis", "ml_geom_axis"], default="clustering",
+                    help="sheets: the split_algorithm of set_entity_data (patches)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of autotuning it before timing "
@@ -377,6 +383,8 @@ def main():
         a, b, n, res = bench_sheets(args, torch, stream)
         workload = ("wavy contact sheets, 2 objects x %d triangles (%dx%d quads), fp64 vertices, 26-DOP, "
                     "63-bit Morton, %d patches, leaf size %d" % (n, args.nq, args.nq, args.od, args.leaf))
+        if args.split != "clustering":
+            workload += ", %s patches" % args.split
         metric = "ms/step build+broad+narrow @2M tris/object, %HBM peak; batched p99 latency"
     else:
         n, res = bench_spheres_self(args, torch, stream)
@@ -427,7 +435,7 @@ def main():
             "fp32_rechecked_pairs": int(r32["stats"]["rechecked_pairs"]),
             "fp32_max_slab_error_ulps": r32["stats"]["max_slab_error_ulps"],
         }
-    if not args.no_cpu and args.workload == "sheets":
+    if not args.no_cpu and args.workload == "sheets" and args.split == "clustering":
         out["cpu_baseline"] = cpu_baseline(a, b, args.od, args.leaf)
         cpu_pairs = out["cpu_baseline"].pop("pairs")
         out["config"]["cpu_pairs_equal_count"] = bool(cpu_pairs == first["npairs"])
