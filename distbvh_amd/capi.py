@@ -106,6 +106,7 @@ SIGNATURES = {
     "bvh_sort_u64": [_vp, _vp, _vp, _u64],
     "bvh_sort_u32": [_vp, _vp, _vp, _u64],
     "bvh_exclusive_scan_u32": [_vp, _vp, _u64],
+    "bvh_closest_point": [_vp, _vp, _u64, _vp, _vp, _u32],
 }
 _RESTYPES = {"bvh_status_string": C.c_char_p, "bvh_last_error": C.c_char_p,
              "bvh_abi_version": C.c_uint32, "bvh_world_config_init": None}

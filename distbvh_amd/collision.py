@@ -127,6 +127,23 @@ class CollisionWorld:
     def exclusive_scan(self, vals):
         call("bvh_exclusive_scan_u32", self.handle, C.c_void_p(vals.data_ptr()), vals.numel())
 
+    def closest_point(self, points, tri):
+        """bvh_closest_point: the closest point of triangle tri ((3, 3): a, b,
+        c) to each of the (n, 3) points (numpy: host memory; CUDA tensor:
+        device memory, the result a new tensor on the same device)."""
+        t = np.ascontiguousarray(np.asarray(tri, np.float64).reshape(9))
+        if hasattr(points, "data_ptr"):
+            import torch
+            out = torch.empty_like(points)
+            call("bvh_closest_point", self.handle, C.c_void_p(points.data_ptr()), points.shape[0],
+                 C.c_void_p(t.ctypes.data), C.c_void_p(out.data_ptr()), capi.BVH_MEM_DEVICE)
+            return out
+        p = np.ascontiguousarray(points, np.float64)
+        out = np.empty_like(p)
+        call("bvh_closest_point", self.handle, C.c_void_p(p.ctypes.data), p.shape[0], C.c_void_p(t.ctypes.data),
+             C.c_void_p(out.ctypes.data), capi.BVH_MEM_HOST)
+        return out
+
 
 class StepGraph:
     """A step captured into a HIP graph (bvh_capture_begin / _end)."""

@@ -110,6 +110,7 @@ struct bvh_world_s {
   size_t events_used = 0;
   // primitive-call scratch
   DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
+  DevBuf prim_in, prim_out;  // host-memory staging of bvh_closest_point
   // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
   DevBuf frame;
   LaunchShape shape;  // narrowphase launch shape (autotuned)
@@ -169,7 +170,7 @@ struct bvh_object_s {
   // the tree's own order (perm) is patch-major, Morton order inside a patch
   uint32_t split = BVH_SPLIT_CLUSTERING;
   DevBuf sp_rp, sp_rp_b, sp_scan, sp_tmpf, sp_tmpt, sp_seg, sp_seg_next, sp_k64, sp_acc, sp_axis, sp_flag, sp_lb,
-      sp_ctr, sp_key, key0, key_a, key_b, sp_hist;
+      sp_ctr, sp_key, key0, key_a, key_b, sp_hist, sp_x;
   const uint32_t* ref_rp = nullptr;
   const uint32_t* ref_perm() const { return split == BVH_SPLIT_CLUSTERING ? perm.as<uint32_t>() : ref_rp; }
   // results
@@ -197,7 +198,7 @@ struct bvh_object_s {
                            &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
                            &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
                            &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
-                           &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist};
+                           &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x};
     size_t t = 0;
     for (const DevBuf* d : all) t += d->bytes;
     return t;
@@ -326,6 +327,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
     BVH_TRY(o->key_a.ensure(n1 * 4));
     BVH_TRY(o->key_b.ensure(n1 * 4));
     BVH_TRY(o->sp_hist.ensure(2 * 256 * 4));
+    BVH_TRY(o->sp_x.ensure(n1 * sizeof(double)));
   }
   return BVH_OK;
 }
@@ -357,6 +359,7 @@ static bvh_status split_and_order(bvh_object_s* o) {
   sb.seg_flag = o->sp_flag.as<uint32_t>();
   sb.scan_lb = o->sp_lb.as<uint64_t>();
   sb.scan_ctr = o->sp_ctr.as<uint32_t>();
+  sb.xbuf = o->sp_x.as<double>();
   uint32_t* rp = nullptr;
   uint32_t* patch_key = o->sp_key.as<uint32_t>();
   HIP_TRY(launch_split(o->in, si, sb, od, o->split == BVH_SPLIT_ML_GEOM_AXIS ? 1 : 0, o->bb.splits, patch_key, &rp, s));
@@ -1233,6 +1236,30 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
   HIP_TRY(launch_scan_u32(vals, n, w->s_lb.as<uint64_t>(), w->s_ctl.as<uint32_t>(), s));
   HIP_TRY(hipStreamSynchronize(s));
   return BVH_OK;
+}
+
+bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, const double* tri, double* closest,
+                             uint32_t mem) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  if (w->capturing) return fail(BVH_E_STATE, "closest_point inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (!tri) return fail(BVH_E_INVALID, "null triangle");
+  if (n == 0) return BVH_OK;
+  if (!points || !closest) return fail(BVH_E_INVALID, "null points or output");
+  hipStream_t s = w->stream;
+  Tri t;
+  std::memcpy(&t, tri, sizeof(t));
+  const size_t bytes = n * 3 * sizeof(double);
+  if (mem == BVH_MEM_DEVICE) {
+    HIP_TRY(launch_closest_point(points, n, t, closest, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BVH_OK;
+  }
+  BVH_TRY(w->prim_in.ensure(bytes));
+  BVH_TRY(w->prim_out.ensure(bytes));
+  HIP_TRY(hipMemcpyAsync(w->prim_in.p, points, bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_closest_point(w->prim_in.as<double>(), n, t, w->prim_out.as<double>(), s));
+  return copy_out(closest, w->prim_out.p, bytes, BVH_MEM_HOST, s);
 }
 
 // ---- autotuner ---------------------------------------------------------------------------

@@ -138,6 +138,7 @@ struct SplitBufs {
   uint32_t* seg_flag;  // od x 2: undecided elements, partition point (kEnd: nothing moves)
   uint64_t* scan_lb;   // sort_tiles(n + 1) words
   uint32_t* scan_ctr;  // 1
+  double* xbuf;        // n: projections in range order (ranges that need the exact split point)
 };
 // The permutation (*result, one of sb.rp / sb.rp_b), the od - 1 split
 // offsets in the reference's order and the patch of every input element;
@@ -146,6 +147,13 @@ hipError_t launch_split(const ElemInput& in, const SplitInput& si, SplitBufs sb,
                         uint32_t* splits, uint32_t* patch_key, uint32_t** result, hipStream_t s);
 template <typename T>
 hipError_t launch_gather_by(const T* src, const uint32_t* idx, uint64_t n, T* dst, hipStream_t s);
+
+// ---- primitives (primitive.hip) ----------------------------------------------------
+struct Tri {
+  double a[3], b[3], c[3];
+};
+// closest point on the triangle for each of n points (n x 3 in, n x 3 out)
+hipError_t launch_closest_point(const double* pts, uint64_t n, const Tri& tri, double* out, hipStream_t s);
 
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {

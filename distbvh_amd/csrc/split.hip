@@ -25,8 +25,9 @@
 // magnitudes are accumulated in any order, which bounds both that sum and the
 // reference's (recursive summation: |error| <= gamma(m) * sum |x|).  Every
 // element whose projection lies outside the resulting interval is on a known
-// side; only if some projection lies inside it does one thread per such
-// range recompute the sum in the reference's order (exact, O(range) serial).
+// side; only if some projection lies inside it is the sum recomputed in the
+// reference's order (exact; the projections are staged in range order and one
+// lane per such range adds them, so the serial part is the fp64 add chain).
 //
 // The two-pointer partition pairs, in order, the misplaced elements of the
 // left part (those that fail the test, from the left) with the misplaced
@@ -249,21 +250,45 @@ __global__ __launch_bounds__(256) void k_split_pred(SplitInput si, SplitBufs sb,
   sb.scan[i] = p;
 }
 
+// the projections in range order, for the exact split points of flagged
+// ranges (contiguous, so the serial sum reads them coalesced)
+__global__ __launch_bounds__(256) void k_split_xbuf(SplitInput si, SplitBufs sb, uint32_t nseg) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= si.n) return;
+  const uint32_t s = find_seg(sb.seg, nseg, (uint32_t)i);
+  if (sb.seg_flag[2 * s] == 0) return;
+  double c[3];
+  split_centroid(si, sb.rp[i], c);
+  sb.xbuf[i] = project_rt(c, sb.seg_axis[s]);
+}
+
 // the reference's split point, for flagged ranges: the projections summed in
 // the range's current order from 0 (std::accumulate / Kokkos::Serial
-// parallel_reduce), then divided by the size
-__global__ void k_split_exact(SplitInput si, SplitBufs sb, uint32_t nseg, uint64_t od) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// parallel_reduce), then divided by the size.  One wave per range: the wave
+// stages 512 projections in LDS with coalesced loads and lane 0 adds them
+// in order (the add chain is the serial part).
+constexpr uint32_t kExactChunk = 512;
+
+__global__ __launch_bounds__(64) void k_split_exact(SplitBufs sb, uint32_t nseg, uint64_t od) {
+  __shared__ double buf[kExactChunk];
+  const uint32_t s = blockIdx.x;
   if (s >= nseg || sb.seg_flag[2 * s] == 0) return;
   const uint32_t b = sb.seg[s], e = sb.seg[od + s];
-  const int axis = sb.seg_axis[s];
+  const uint32_t lane = threadIdx.x;
   double sum = 0.0;
-  for (uint32_t i = b; i < e; ++i) {
-    double c[3];
-    split_centroid(si, sb.rp[i], c);
-    sum = sum + project_rt(c, axis);
+  for (uint32_t base = b; base < e; base += kExactChunk) {
+    const uint32_t cnt = (e - base) < kExactChunk ? (e - base) : kExactChunk;
+#pragma unroll
+    for (uint32_t k = 0; k < kExactChunk / 64; ++k) {
+      const uint32_t j = k * 64 + lane;
+      if (j < cnt) buf[j] = sb.xbuf[base + j];
+    }
+    __syncthreads();
+    if (lane == 0)
+      for (uint32_t j = 0; j < cnt; ++j) sum = sum + buf[j];
+    __syncthreads();
   }
-  sb.seg_acc[5 * s + 4] = sum / (double)(e - b);
+  if (lane == 0) sb.seg_acc[5 * s + 4] = sum / (double)(e - b);
 }
 
 __global__ __launch_bounds__(256) void k_split_fix(SplitInput si, SplitBufs sb, uint32_t nseg) {
@@ -417,7 +442,8 @@ hipError_t launch_split(const ElemInput& in, const SplitInput& si, SplitBufs sb,
     hipLaunchKernelGGL(k_split_sum, dim3(g), dim3(256), 0, s, si, sb, nseg);
     hipLaunchKernelGGL(k_split_interval, dim3(gs), dim3(64), 0, s, sb, nseg, od);
     hipLaunchKernelGGL(k_split_pred, dim3(g1), dim3(256), 0, s, si, sb, nseg);
-    hipLaunchKernelGGL(k_split_exact, dim3(gs), dim3(64), 0, s, si, sb, nseg, od);
+    hipLaunchKernelGGL(k_split_xbuf, dim3(g), dim3(256), 0, s, si, sb, nseg);
+    hipLaunchKernelGGL(k_split_exact, dim3(nseg), dim3(64), 0, s, sb, nseg, od);
     hipLaunchKernelGGL(k_split_fix, dim3(g), dim3(256), 0, s, si, sb, nseg);
     e = launch_fill_u32((uint32_t*)sb.scan_lb, 2 * sort_tiles(n + 1), 0u, s);
     if (e != hipSuccess) return e;

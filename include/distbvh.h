@@ -229,6 +229,14 @@ bvh_status bvh_sort_u32(bvh_world w, uint32_t *keys, uint32_t *vals, uint64_t n)
 /* prefix_sum, util/prefix_sum.hpp:41-54 (exclusive), in place */
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 
+/* bvh::kokkos::closest_point(points, triangle, closest)
+ * (primitive/kokkos_closest_point.hpp:55-133): the closest point of the
+ * triangle tri = {a, b, c} (9 doubles, host memory) to each of n points;
+ * points and closest are n x 3 doubles in `mem` (BVH_MEM_HOST or
+ * BVH_MEM_DEVICE).  Bit for bit the reference's expressions. */
+bvh_status bvh_closest_point(bvh_world w, const double *points, uint64_t n, const double *tri,
+                             double *closest, uint32_t mem);
+
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
   uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 or 4 = that

@@ -926,6 +926,67 @@ void orc_split_ml_geom_axis(const double *kdops, const double *centroids, size_t
   ml_rec(&st, depth - 1, 0, n);
 }
 
+/* ---- closest point on a triangle ----------------------------------------------------- */
+
+/* m::dot (math/vec.hpp:247-250, haddv_op math/ops/vec_base_ops.hpp:248-263):
+ * the products summed left to right from 0 */
+static double dot3(const double *u, const double *v) {
+  double s = 0.0;
+  s += u[0] * v[0];
+  s += u[1] * v[1];
+  s += u[2] * v[2];
+  return s;
+}
+
+/* bvh::kokkos::closest_point (primitive/kokkos_closest_point.hpp:55-133), one
+ * point after the other; points and out are n x 3, tri = a, b, c */
+void orc_closest_point(const double *points, size_t n, const double *tri, double *out) {
+  const double *a = tri, *b = tri + 3, *c = tri + 6;
+  double ab[3], ac[3], bc[3];
+  for (int k = 0; k < 3; ++k) {
+    ab[k] = b[k] - a[k];
+    ac[k] = c[k] - a[k];
+    bc[k] = c[k] - b[k];
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const double *p = points + 3 * i;
+    double *o = out + 3 * i;
+    double ap[3], bp[3], cp[3];
+    for (int k = 0; k < 3; ++k) {
+      ap[k] = p[k] - a[k];
+      bp[k] = p[k] - b[k];
+      cp[k] = p[k] - c[k];
+    }
+    double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { for (int k = 0; k < 3; ++k) o[k] = a[k]; continue; }
+    double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { for (int k = 0; k < 3; ++k) o[k] = b[k]; continue; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+      double v = d1 / (d1 - d3);
+      for (int k = 0; k < 3; ++k) o[k] = a[k] + v * ab[k];
+      continue;
+    }
+    double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) { for (int k = 0; k < 3; ++k) o[k] = c[k]; continue; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+      double w = d2 / (d2 - d6);
+      for (int k = 0; k < 3; ++k) o[k] = a[k] + w * ac[k];
+      continue;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+      double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      for (int k = 0; k < 3; ++k) o[k] = b[k] + w * bc[k];
+      continue;
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    for (int k = 0; k < 3; ++k) o[k] = a[k] + ab[k] * v + ac[k] * w;
+  }
+}
+
 typedef struct {
   uint64_t pp, ep, pq, eq;
 } rec4;

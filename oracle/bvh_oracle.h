@@ -145,6 +145,9 @@ typedef struct {
 #define ORC_SPLIT_ML_GEOM_AXIS 1
 #define ORC_SPLIT_CLUSTERING 2
 
+/* bvh::kokkos::closest_point: n x 3 points, tri = a, b, c (9 doubles) */
+void orc_closest_point(const double *points, size_t n, const double *tri, double *out);
+
 /* split_permutations / split_permutations_ml with split::mean and
  * axis::longest: idx (n) and 2^depth - 1 split offsets */
 void orc_split_geom_axis(const double *kdops, const double *centroids, size_t n, int depth, uint32_t *idx,

@@ -107,6 +107,7 @@ def lib():
                                                   C.POINTER(OrcCfg), C.POINTER(OrcTimes)]),
             "orc_now": (C.c_double, []),
             "orc_split_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
+            "orc_closest_point": (None, [dp, C.c_size_t, dp, dp]),
             "orc_split_ml_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
         }
         for name, (res, args) in sig.items():
@@ -250,6 +251,16 @@ def split_permutations(kdops, centroids, depth, ml=False):
     f = lib().orc_split_ml_geom_axis if ml else lib().orc_split_geom_axis
     f(_p(kd, dp), _p(ce, dp), n, depth, _p(idx, u32p), _p(sp, u64p))
     return idx[:n], sp[:ns]
+
+
+def closest_point(points, tri):
+    """bvh::kokkos::closest_point restated: (n, 3) points, tri (3, 3)."""
+    p = _c(np.asarray(points, np.float64).reshape(-1, 3), np.float64)
+    t = _c(np.asarray(tri, np.float64).reshape(9), np.float64)
+    out = np.empty_like(p)
+    if len(p):
+        lib().orc_closest_point(_p(p, dp), len(p), _p(t, dp), _p(out, dp))
+    return out
 
 
 def patches(splits, od, sorted_kdops, sorted_centroids):
