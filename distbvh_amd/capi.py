@@ -36,6 +36,11 @@ class Record(C.Structure):
                 ("element_q", C.c_uint64)]
 
 
+class TreeNode(C.Structure):
+    _fields_ = [("parent_offset", C.c_int64), ("child_offsets", C.c_int64 * 2), ("kdop", C.c_double * 26),
+                ("entity_range", C.c_uint64 * 2)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("block_size", C.c_uint32), ("xcd_remap", C.c_uint32),
                 ("tried", C.c_uint32), ("ms", C.c_double), ("tried_ms", C.c_double * 24)]
@@ -107,6 +112,7 @@ SIGNATURES = {
     "bvh_sort_u32": [_vp, _vp, _vp, _u64],
     "bvh_exclusive_scan_u32": [_vp, _vp, _u64],
     "bvh_closest_point": [_vp, _vp, _u64, _vp, _vp, _u32],
+    "bvh_object_patch_tree": [_vp, _vp, C.c_size_t, _vp, _vp, C.c_size_t, _vp],
 }
 _RESTYPES = {"bvh_status_string": C.c_char_p, "bvh_last_error": C.c_char_p,
              "bvh_abi_version": C.c_uint32, "bvh_world_config_init": None}

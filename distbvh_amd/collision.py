@@ -327,6 +327,21 @@ class CollisionObject:
         return [{"global_id": p.global_id, "size": p.size, "kdop": np.array(p.kdop[:]),
                  "centroid": np.array(p.centroid[:])} for p in arr]
 
+    def patch_tree(self):
+        """bvh_object_patch_tree: the patch-level tree for_each_tree hands out,
+        pre-order: (nodes, leaf_patches) with nodes a list of dicts
+        (parent_offset, child_offsets, kdop, entity_range)."""
+        nn = C.c_size_t()
+        nl = C.c_size_t()
+        call("bvh_object_patch_tree", self.handle, None, 0, C.byref(nn), None, 0, C.byref(nl))
+        nodes = (capi.TreeNode * max(nn.value, 1))()
+        leaves = np.zeros(max(nl.value, 1), np.uint64)
+        call("bvh_object_patch_tree", self.handle, nodes, nn.value, C.byref(nn), C.c_void_p(leaves.ctypes.data),
+             nl.value, C.byref(nl))
+        out = [{"parent_offset": t.parent_offset, "child_offsets": tuple(t.child_offsets),
+                "kdop": np.array(t.kdop[:]), "entity_range": tuple(t.entity_range)} for t in nodes[:nn.value]]
+        return out, leaves[:nl.value]
+
     def patch_elements(self, k):
         cnt = C.c_uint64()
         out = np.zeros(max(self.n, 1), np.uint64)

@@ -892,6 +892,66 @@ __global__ __launch_bounds__(256) void k_sorted_kdops(const Rec64* __restrict__ 
   }
 }
 
+// The patch-level tree (the levels of the tree above the patch roots) in
+// pre-order, for for_each_tree: per output node its parent and children
+// (output indices, -1 for none) and its patch (a leaf) or -1.  One thread;
+// `stack` holds 2 x 2 od words.  A node is a patch root when its range lies
+// in one patch (its record's patch, or the leaf's position's patch).
+template <bool FP32>
+__global__ void k_patch_tree(BuildBufs b, uint64_t n, uint64_t od, int64_t* __restrict__ out,
+                             uint32_t* __restrict__ count, uint64_t* __restrict__ stack) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *count = 0;
+  if (n == 0) return;
+  const RecT<FP32>* recs = recs_of<FP32>(b);
+  const uint64_t cap = 2 * od;
+  uint64_t* snode = stack;
+  uint64_t* spar = stack + cap;
+  uint64_t sp = 0, cnt = 0;
+  snode[sp] = *b.root;
+  spar[sp] = ~0ull;
+  ++sp;
+  while (sp > 0) {
+    --sp;
+    const uint32_t x = (uint32_t)snode[sp];
+    const uint64_t par = spar[sp];
+    if (cnt >= cap || x >= 2 * n - 1) {
+      atomicOr(b.status, kErrTree);
+      return;
+    }
+    const uint64_t k = cnt++;
+    out[4 * k] = par == ~0ull ? -1 : (int64_t)(par >> 1);
+    out[4 * k + 1] = -1;
+    out[4 * k + 2] = -1;
+    if (par != ~0ull) out[4 * (par >> 1) + 1 + (par & 1)] = (int64_t)k;
+    const uint32_t pv = x >= n - 1 ? b.patch_of_pos[x - (n - 1)] : recs[x].patch;
+    if (pv != kMixed) {
+      out[4 * k + 3] = pv;
+    } else {
+      out[4 * k + 3] = -1;
+      if (sp + 2 > cap) {
+        atomicOr(b.status, kErrTree);
+        return;
+      }
+      snode[sp] = recs[x].child[1];  // right below left: left comes out first
+      spar[sp] = (k << 1) | 1;
+      ++sp;
+      snode[sp] = recs[x].child[0];
+      spar[sp] = k << 1;
+      ++sp;
+    }
+  }
+  *count = (uint32_t)cnt;
+}
+
+hipError_t launch_patch_tree(const BuildBufs& b, uint64_t n, uint64_t od, int fp32, int64_t* out, uint32_t* count,
+                             uint64_t* stack, hipStream_t s) {
+  if (!out || !count || !stack) return hipErrorInvalidValue;
+  if (fp32) hipLaunchKernelGGL(k_patch_tree<true>, dim3(1), dim3(64), 0, s, b, n, od, out, count, stack);
+  else hipLaunchKernelGGL(k_patch_tree<false>, dim3(1), dim3(64), 0, s, b, n, od, out, count, stack);
+  return hipGetLastError();
+}
+
 // ---- launchers ---------------------------------------------------------------------
 template <int KIND, int NV>
 static void prep_dispatch(const ElemInput& in, const BuildBufs& b, const ZeroList& zl, hipStream_t s) {

@@ -1181,6 +1181,66 @@ bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t* indices, uin
   return BVH_OK;
 }
 
+bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node* nodes, size_t cap, size_t* nnodes,
+                                 uint64_t* leaf_patches, size_t leaf_cap, size_t* nleaves) {
+  if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  bvh_world w = o->w;
+  hipStream_t s = w->stream;
+  const uint64_t n = o->in.n, od = w->od;
+  DevBuf dout, dcnt, dstack;
+  BVH_TRY(dout.ensure(4 * 2 * od * sizeof(int64_t)));
+  BVH_TRY(dcnt.ensure(sizeof(uint32_t)));
+  BVH_TRY(dstack.ensure(4 * od * sizeof(uint64_t)));
+  HIP_TRY(launch_patch_tree(o->bb, n, od, o->fp32, dout.as<int64_t>(), dcnt.as<uint32_t>(), dstack.as<uint64_t>(), s));
+  uint32_t cnt = 0, st = 0;
+  BVH_TRY(copy_out(&cnt, dcnt.p, sizeof(cnt), BVH_MEM_HOST, s));
+  BVH_TRY(copy_out(&st, o->status.p, sizeof(st), BVH_MEM_HOST, s));
+  if (st & kErrTree) return fail(BVH_E_HIP, "inconsistent patch tree");
+  std::vector<int64_t> topo(4 * (size_t)cnt);
+  if (cnt) BVH_TRY(copy_out(topo.data(), dout.p, topo.size() * sizeof(int64_t), BVH_MEM_HOST, s));
+  std::vector<uint64_t> enc(od * kSlabs);
+  BVH_TRY(copy_out(enc.data(), o->patch_k64.p, enc.size() * sizeof(uint64_t), BVH_MEM_HOST, s));
+  // leaves are met left to right in pre-order; ranges and k-DOPs bottom-up
+  std::vector<bvh_tree_node> t(cnt);
+  std::vector<uint64_t> leaves;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    bvh_tree_node& nd = t[k];
+    const int64_t par = topo[4 * k], l = topo[4 * k + 1], r = topo[4 * k + 2], pv = topo[4 * k + 3];
+    nd.parent_offset = par < 0 ? 0 : par - (int64_t)k;
+    nd.child_offsets[0] = l < 0 ? 0 : l - (int64_t)k;
+    nd.child_offsets[1] = r < 0 ? 0 : r - (int64_t)k;
+    if (pv >= 0) {
+      nd.entity_range[0] = leaves.size();
+      nd.entity_range[1] = leaves.size() + 1;
+      for (int i = 0; i < kSlabs; ++i) nd.kdop[i] = from_ordered_u64(enc[(size_t)pv * kSlabs + i]);
+      leaves.push_back((uint64_t)pv);
+    }
+  }
+  for (uint32_t k = cnt; k-- > 0;) {
+    bvh_tree_node& nd = t[k];
+    if (topo[4 * k + 3] >= 0) continue;
+    const bvh_tree_node& a = t[k + nd.child_offsets[0]];
+    const bvh_tree_node& b = t[k + nd.child_offsets[1]];
+    nd.entity_range[0] = a.entity_range[0];
+    nd.entity_range[1] = b.entity_range[1];
+    for (int i = 0; i < kAxes; ++i) {  // union_with: m::min / m::max (kdop.hpp:317-325)
+      nd.kdop[2 * i] = (a.kdop[2 * i] < b.kdop[2 * i]) ? a.kdop[2 * i] : b.kdop[2 * i];
+      nd.kdop[2 * i + 1] = (a.kdop[2 * i + 1] > b.kdop[2 * i + 1]) ? a.kdop[2 * i + 1] : b.kdop[2 * i + 1];
+    }
+  }
+  if (nnodes) *nnodes = cnt;
+  if (nleaves) *nleaves = leaves.size();
+  if (nodes) {
+    if (cap < cnt) return fail(BVH_E_OVERFLOW, "node buffer too small");
+    std::copy(t.begin(), t.end(), nodes);
+  }
+  if (leaf_patches) {
+    if (leaf_cap < leaves.size()) return fail(BVH_E_OVERFLOW, "leaf buffer too small");
+    std::copy(leaves.begin(), leaves.end(), leaf_patches);
+  }
+  return BVH_OK;
+}
+
 bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint64_t* ranges, size_t cap,
                            size_t* nnodes, size_t* root) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");

@@ -116,6 +116,10 @@ hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hip
 hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
                                hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
+// the patch-level tree in pre-order: 4 words per node (parent, left, right:
+// output indices or -1; patch or -1), *count nodes; stack: 4 od words
+hipError_t launch_patch_tree(const BuildBufs& b, uint64_t n, uint64_t od, int fp32, int64_t* out, uint32_t* count,
+                             uint64_t* stack, hipStream_t s);
 
 // ---- geom_axis / ml_geom_axis splitters (split.hip) -------------------------------
 // element centroids: 3 planes of n (cplanes), or n x 3 (caos)

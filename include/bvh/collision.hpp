@@ -256,8 +256,28 @@ class collision_object {
     return out;
   }
 
-  // for_each_tree(f), collision_object.hpp:164-168: f receives the node count
-  // and the root; the nodes themselves via bvh_object_tree.
+  // The patch-level tree for_each_tree hands out (snapshot_tree, tree.hpp:
+  // 75-452): pre-order nodes (node.hpp:318-322) over the non-empty patches.
+  struct patch_tree {
+    std::vector<bvh_tree_node> nodes;
+    std::vector<std::uint64_t> leafs;  // patch ids, left to right
+    std::size_t count() const noexcept { return leafs.size(); }
+    bool empty() const noexcept { return leafs.empty(); }
+  };
+
+  // for_each_tree(f), collision_object.hpp:164-168: f(const patch_tree &)
+  template <typename F>
+  void for_each_tree(F&& f) const {
+    std::size_t nn = 0, nl = 0;
+    BVH_FACADE_CALL(bvh_object_patch_tree(h_, nullptr, 0, &nn, nullptr, 0, &nl));
+    patch_tree t;
+    t.nodes.resize(nn);
+    t.leafs.resize(nl);
+    BVH_FACADE_CALL(bvh_object_patch_tree(h_, t.nodes.data(), nn, &nn, t.leafs.data(), nl, &nl));
+    std::forward<F>(f)(static_cast<const patch_tree&>(t));
+  }
+
+  // the element-level tree's node count and root (bvh_object_tree)
   std::pair<std::size_t, std::size_t> tree_shape() const {
     std::size_t nn = 0, root = 0;
     BVH_FACADE_CALL(bvh_object_tree(h_, nullptr, nullptr, nullptr, 0, &nn, &root));

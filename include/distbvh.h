@@ -216,7 +216,24 @@ bvh_status bvh_object_sorted(bvh_object o, uint64_t *codes, uint32_t *perm, uint
 bvh_status bvh_object_splits(bvh_object o, uint64_t *splits, size_t cap);
 /* element 26-DOPs in sorted order (n x 26), the snapshots' kdops */
 bvh_status bvh_object_sorted_kdops(bvh_object o, double *kdops, uint32_t mem);
-/* tree export for for_each_tree (collision_object.hpp:164-168): node count,
+/* for_each_tree (collision_object.hpp:164-168, collision_object.cpp:217-229)
+ * hands the patch-level tree (a snapshot_tree over the non-empty patches,
+ * tree.hpp:75-452, count() = its leaves).  Here: the levels of the tree above
+ * the patch roots, in pre-order, as bvh_node (node.hpp:318-322): offsets
+ * relative to the node (0: none / no parent), the exact fp64 k-DOP (a leaf:
+ * its patch's; an inner node: the union of its children) and the range
+ * [first, last) of its leaves in leaf_patches (patch ids, left to right).
+ * Either output may be null to query the sizes. */
+typedef struct bvh_tree_node {
+  int64_t parent_offset;
+  int64_t child_offsets[2];
+  double kdop[26];
+  uint64_t entity_range[2];
+} bvh_tree_node;
+bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node *nodes, size_t cap, size_t *nnodes,
+                                 uint64_t *leaf_patches, size_t leaf_cap, size_t *nleaves);
+/* the element-level tree (no reference counterpart: the reference's element
+ * trees are per patch pair): node count,
  * then per node kdop (26), children (-1 for none) and leaf range [first, last]. */
 bvh_status bvh_object_tree(bvh_object o, double *kdops, int64_t *children, uint64_t *ranges,
                            size_t cap, size_t *nnodes, size_t *root);

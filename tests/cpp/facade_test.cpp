@@ -136,6 +136,17 @@ static void patches_case() {
   CHECK(total == 512);
   CHECK(mn == 0.0 && mx == 1.0);
   CHECK(o.tree_shape().first == 2 * 512 - 1);
+  // collision_object_test.cpp:49-81: for_each_tree's tree has one leaf per
+  // (non-empty) patch, none for an object without elements
+  o.for_each_tree([](const auto& t) {
+    CHECK(t.count() == 4);
+    CHECK(t.nodes.size() == 2 * 4 - 1);
+    CHECK(t.nodes[0].parent_offset == 0 && t.nodes[0].entity_range[0] == 0 && t.nodes[0].entity_range[1] == 4);
+  });
+  o.set_entity_data(bvh::span<const Element>(e.data(), 0), bvh::split_algorithm::clustering);
+  o.for_each_tree([](const auto& t) { CHECK(t.count() == 0 && t.nodes.empty()); });
+  o.set_entity_data(bvh::span<const Element>(e.data(), e.size()), bvh::split_algorithm::geom_axis);
+  o.for_each_tree([](const auto& t) { CHECK(t.count() == 4); });
 }
 
 int main() {
