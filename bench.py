@@ -120,6 +120,17 @@ def cpu_baseline(a, b, od, leaf):
     }
 
 
+def pick_launch(args, world, autotune):
+    """The narrowphase launch shape a form is timed at: --launch pins one
+    (profiling runs of an autotuned shape without the tuner's launches),
+    --no-tune keeps the default, else the autotuner picks it."""
+    if args.launch:
+        batch, block, xcd = (int(x) for x in args.launch.split(","))
+        world.set_launch(batch=batch, block_size=block, xcd_remap=bool(xcd))
+        return world.launch()
+    return world.launch() if args.no_tune else autotune()
+
+
 def run_steps(world, step, steps, warmup, stream, torch):
     """Warmup, then `steps` timed steps bracketed by HIP events on the
     library's stream; returns (ms per step, per-stage ms per step, per-stage
@@ -194,7 +205,7 @@ def bench_sheets(args, torch, stream):
 
         step()
         # each form timed at its fastest launch shape
-        tune = world.launch() if args.no_tune else oa.autotune(ob)
+        tune = pick_launch(args, world, lambda: oa.autotune(ob))
         world.stage_times()
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = oa.result_stats()
@@ -235,7 +246,7 @@ def bench_spheres_self(args, torch, stream):
 
         step()
         # each form timed at its fastest launch shape
-        tune = world.launch() if args.no_tune else o.autotune()
+        tune = pick_launch(args, world, o.autotune)
         world.stage_times()
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = o.result_stats()
@@ -338,6 +349,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of autotuning it before timing "
                          "(profiling runs: no tuner launches in the trace)")
+    ap.add_argument("--launch", default="", metavar="BATCH,BLOCK,XCD",
+                    help="pin the narrowphase launch shape of every form (e.g. 4,256,1) instead of "
+                         "autotuning it (profiling runs of a shape the autotuner picked)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-sized device buffer once (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
