@@ -144,6 +144,55 @@ class CollisionWorld:
              C.c_void_p(out.ctypes.data), capi.BVH_MEM_HOST)
         return out
 
+    def max_variant_axis(self, ca, cb):
+        """bvh_max_variant_axis: the axis of largest variance of the (na, 3)
+        and (nb, 3) centroids together (numpy or CUDA tensors, both alike)."""
+        ax = C.c_int32(-1)
+        if hasattr(ca, "data_ptr"):
+            call("bvh_max_variant_axis", self.handle, C.c_void_p(ca.data_ptr()), ca.shape[0],
+                 C.c_void_p(cb.data_ptr()), cb.shape[0], capi.BVH_MEM_DEVICE, C.byref(ax))
+            return ax.value
+        a = np.ascontiguousarray(ca, np.float64)
+        b = np.ascontiguousarray(cb, np.float64)
+        call("bvh_max_variant_axis", self.handle, C.c_void_p(a.ctypes.data), a.shape[0], C.c_void_p(b.ctypes.data),
+             b.shape[0], capi.BVH_MEM_HOST, C.byref(ax))
+        return ax.value
+
+    def sort_and_sweep(self, pa, ka, ga, pb, kb, gb, axis):
+        """bvh_sort_and_sweep: the (k, 2) uint64 pairs (id in A, id in B) of
+        overlapping elements of patches A and B, in the reference's callback
+        order per element of A.  pa / pb: the patches' 26-DOPs; ka / kb:
+        (n, 26) element k-DOPs; ga / gb: global ids or None (local
+        indices).  numpy arrays (host memory) or CUDA tensors (device
+        memory; the pairs are then a CUDA tensor)."""
+        pa = np.ascontiguousarray(np.asarray(pa, np.float64).reshape(26))
+        pb = np.ascontiguousarray(np.asarray(pb, np.float64).reshape(26))
+        n = C.c_uint64(0)
+        dev = hasattr(ka, "data_ptr")
+        if dev:
+            import torch
+            ptr = lambda t: C.c_void_p(t.data_ptr() if t is not None else None)
+            args = (ptr(ka), ka.shape[0], C.c_void_p(pb.ctypes.data), ptr(kb), ptr(gb), kb.shape[0])
+            call("bvh_sort_and_sweep", self.handle, C.c_void_p(pa.ctypes.data), args[0], ptr(ga), args[1], *args[2:],
+                 int(axis), capi.BVH_MEM_DEVICE, None, 0, C.byref(n))
+            out = torch.empty((n.value, 2), dtype=torch.int64, device=ka.device)
+            if n.value:
+                call("bvh_sort_and_sweep", self.handle, C.c_void_p(pa.ctypes.data), args[0], ptr(ga), args[1],
+                     *args[2:], int(axis), capi.BVH_MEM_DEVICE, C.c_void_p(out.data_ptr()), n.value, C.byref(n))
+            return out
+        ka = np.ascontiguousarray(ka, np.float64)
+        kb = np.ascontiguousarray(kb, np.float64)
+        ga = None if ga is None else np.ascontiguousarray(ga, np.uint64)
+        gb = None if gb is None else np.ascontiguousarray(gb, np.uint64)
+        ptr = lambda a: C.c_void_p(a.ctypes.data if a is not None else None)
+        args = (C.c_void_p(pa.ctypes.data), ptr(ka), ptr(ga), ka.shape[0], C.c_void_p(pb.ctypes.data), ptr(kb),
+                ptr(gb), kb.shape[0], int(axis), capi.BVH_MEM_HOST)
+        call("bvh_sort_and_sweep", self.handle, *args, None, 0, C.byref(n))
+        out = np.empty((n.value, 2), np.uint64)
+        if n.value:
+            call("bvh_sort_and_sweep", self.handle, *args, C.c_void_p(out.ctypes.data), n.value, C.byref(n))
+        return out
+
 
 class StepGraph:
     """A step captured into a HIP graph (bvh_capture_begin / _end)."""

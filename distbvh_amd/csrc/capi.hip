@@ -111,6 +111,8 @@ struct bvh_world_s {
   // primitive-call scratch
   DevBuf s_kb, s_ka, s_va, s_vb, s_hist, s_ctl, s_lb;
   DevBuf prim_in, prim_out;  // host-memory staging of bvh_closest_point
+  // bvh_max_variant_axis / bvh_sort_and_sweep: staging, sorted B, counts
+  DevBuf sw_ka, sw_kb, sw_ga, sw_gb, sw_bq, sw_bidx, sw_kbs, sw_bmax, sw_count, sw_total, sw_out, sw_part;
   // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
   DevBuf frame;
   LaunchShape shape;  // narrowphase launch shape (autotuned)
@@ -695,6 +697,20 @@ static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   HIP_TRY(hipMemcpyAsync(keys, a.ka, n * sizeof(K), hipMemcpyDeviceToDevice, s));
   if (vals) HIP_TRY(hipMemcpyAsync(vals, a.va, n * 4, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+// device copy of a caller array: the array itself (device memory) or the
+// staging buffer it was uploaded into (host memory)
+template <typename T>
+static bvh_status stage_in(DevBuf& buf, const T* src, uint64_t count, uint32_t mem, hipStream_t s, const T** dev) {
+  if (mem == BVH_MEM_DEVICE || !src || count == 0) {
+    *dev = src;
+    return BVH_OK;
+  }
+  BVH_TRY(buf.ensure(count * sizeof(T)));
+  HIP_TRY(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  *dev = buf.as<T>();
   return BVH_OK;
 }
 
@@ -1320,6 +1336,91 @@ bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, cons
   HIP_TRY(hipMemcpyAsync(w->prim_in.p, points, bytes, hipMemcpyHostToDevice, s));
   HIP_TRY(launch_closest_point(w->prim_in.as<double>(), n, t, w->prim_out.as<double>(), s));
   return copy_out(closest, w->prim_out.p, bytes, BVH_MEM_HOST, s);
+}
+
+// ---- sort and sweep between two patches ---------------------------------------------
+bvh_status bvh_max_variant_axis(bvh_world w, const double* ca, uint64_t na, const double* cb, uint64_t nb,
+                                uint32_t mem, int32_t* axis) {
+  if (!w || !axis) return fail(BVH_E_INVALID, "null argument");
+  if (w->capturing) return fail(BVH_E_STATE, "max_variant_axis inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if ((na && !ca) || (nb && !cb)) return fail(BVH_E_INVALID, "null centroids");
+  hipStream_t s = w->stream;
+  const double *da = nullptr, *db = nullptr;
+  BVH_TRY(stage_in(w->sw_ka, ca, na * 3, mem, s, &da));
+  BVH_TRY(stage_in(w->sw_kb, cb, nb * 3, mem, s, &db));
+  BVH_TRY(w->sw_part.ensure((size_t)variant_axis_partials() * 6 * sizeof(double) + 256));
+  int32_t* dax = reinterpret_cast<int32_t*>(w->sw_part.as<char>() + (size_t)variant_axis_partials() * 6 * sizeof(double));
+  HIP_TRY(launch_variant_axis(da, na, db, nb, w->sw_part.as<double>(), dax, s));
+  return copy_out(axis, dax, sizeof(int32_t), BVH_MEM_HOST, s);
+}
+
+bvh_status bvh_sort_and_sweep(bvh_world w, const double* pa, const double* ka, const uint64_t* ga, uint64_t na,
+                              const double* pb, const double* kb, const uint64_t* gb, uint64_t nb, int32_t axis,
+                              uint32_t mem, uint64_t* pairs, uint64_t cap, uint64_t* npairs) {
+  if (!w || !npairs) return fail(BVH_E_INVALID, "null argument");
+  *npairs = 0;
+  if (w->capturing) return fail(BVH_E_STATE, "sort_and_sweep inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (!pa || !pb) return fail(BVH_E_INVALID, "null patch k-DOP");
+  if (axis < 0 || axis >= 13) return fail(BVH_E_INVALID, "axis must be a k-DOP direction, 0..12");
+  if ((na && !ka) || (nb && !kb)) return fail(BVH_E_INVALID, "null element k-DOPs");
+  if (na == 0 || nb == 0) return BVH_OK;
+  if (nb >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements of B");
+  hipStream_t s = w->stream;
+  // the sweep range: the union of the two patches' slabs on the axis, in the
+  // reference's std::min / std::max and conversion factor (kokkos.hpp:175-178)
+  const double lo_a = pa[2 * axis], lo_b = pb[2 * axis], hi_a = pa[2 * axis + 1], hi_b = pb[2 * axis + 1];
+  const double gmin = (lo_b < lo_a) ? lo_b : lo_a;
+  const double gmax = (hi_a < hi_b) ? hi_b : hi_a;
+  const double conv = (double)UINT32_MAX / (gmax - gmin);
+  SweepArgs a{};
+  BVH_TRY(stage_in(w->sw_ka, ka, na * 26, mem, s, &a.ka));
+  BVH_TRY(stage_in(w->sw_ga, ga, na, mem, s, &a.ga));
+  const double* dkb = nullptr;
+  BVH_TRY(stage_in(w->sw_kb, kb, nb * 26, mem, s, &dkb));
+  BVH_TRY(stage_in(w->sw_gb, gb, nb, mem, s, &a.gb));
+  BVH_TRY(w->sw_bq.ensure(nb * 4));
+  BVH_TRY(w->sw_bidx.ensure(nb * 4));
+  BVH_TRY(w->sw_kbs.ensure(nb * 26 * sizeof(double)));
+  BVH_TRY(w->sw_bmax.ensure(nb * sizeof(double)));
+  BVH_TRY(w->sw_count.ensure(na * 4));
+  BVH_TRY(w->sw_total.ensure(sizeof(unsigned long long)));
+  HIP_TRY(launch_sweep_prepare(dkb, nb, axis, gmin, conv, w->sw_bq.as<uint32_t>(), w->sw_bidx.as<uint32_t>(), s));
+  BVH_TRY(prim_sort<uint32_t>(w, w->sw_bq.as<uint32_t>(), w->sw_bidx.as<uint32_t>(), nb));
+  HIP_TRY(launch_sweep_gather(dkb, w->sw_bidx.as<uint32_t>(), nb, axis, gmin, conv, w->sw_kbs.as<double>(),
+                              w->sw_bmax.as<double>(), s));
+  a.na = na;
+  a.nb = nb;
+  a.kbs = w->sw_kbs.as<double>();
+  a.bmax = w->sw_bmax.as<double>();
+  a.bq = w->sw_bq.as<uint32_t>();
+  a.bidx = w->sw_bidx.as<uint32_t>();
+  a.axis = axis;
+  a.gmin = gmin;
+  a.conv = conv;
+  a.count = w->sw_count.as<uint32_t>();
+  a.total = w->sw_total.as<unsigned long long>();
+  HIP_TRY(launch_fill_u32(w->sw_total.as<uint32_t>(), 2, 0u, s));
+  HIP_TRY(launch_sweep(a, false, s));
+  unsigned long long total = 0;
+  BVH_TRY(copy_out(&total, a.total, sizeof(total), BVH_MEM_HOST, s));
+  if (total >= (1ull << 32)) return fail(BVH_E_INVALID, "more than 2^32 - 1 pairs between two patches");
+  *npairs = total;
+  if (!pairs || total == 0) return BVH_OK;
+  if (cap < total) return fail(BVH_E_OVERFLOW, "pair buffer holds " + std::to_string(cap) + " pairs, " +
+                                                   std::to_string(total) + " needed");
+  BVH_TRY(bvh_exclusive_scan_u32(w, a.count, na));
+  if (mem == BVH_MEM_DEVICE) {
+    a.out = pairs;
+    HIP_TRY(launch_sweep(a, true, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BVH_OK;
+  }
+  BVH_TRY(w->sw_out.ensure(total * 2 * sizeof(uint64_t)));
+  a.out = w->sw_out.as<uint64_t>();
+  HIP_TRY(launch_sweep(a, true, s));
+  return copy_out(pairs, a.out, total * 2 * sizeof(uint64_t), BVH_MEM_HOST, s);
 }
 
 // ---- autotuner ---------------------------------------------------------------------------

@@ -159,6 +159,33 @@ struct Tri {
 // closest point on the triangle for each of n points (n x 3 in, n x 3 out)
 hipError_t launch_closest_point(const double* pts, uint64_t n, const Tri& tri, double* out, hipStream_t s);
 
+// ---- sort and sweep between two patches (sweep.hip) --------------------------------
+struct SweepArgs {
+  const double* ka;            // A's k-DOPs, na x 26
+  const uint64_t* ga;          // A's global ids (null: local indices)
+  uint64_t na;
+  const double* kbs;           // B's k-DOPs in sorted order, nb x 26
+  const double* bmax;          // B's scaled maxima on the axis, sorted order
+  const uint32_t* bq;          // B's truncated minima, sorted
+  const uint32_t* bidx;        // sorted position -> B index
+  const uint64_t* gb;          // B's global ids (null: local indices)
+  uint64_t nb;
+  int axis;
+  double gmin, conv;
+  uint32_t* count;             // per a: count (count pass), exclusive offset (write pass)
+  unsigned long long* total;   // zeroed before the count pass
+  uint64_t* out;               // 2 words per pair
+};
+// max_variant_axis: partials (variant_axis_partials() x 6 doubles), axis out
+hipError_t launch_variant_axis(const double* ca, uint64_t na, const double* cb, uint64_t nb, double* part,
+                               int32_t* axis, hipStream_t s);
+uint32_t variant_axis_partials();
+hipError_t launch_sweep_prepare(const double* kb, uint64_t nb, int axis, double gmin, double conv, uint32_t* bq,
+                                uint32_t* bidx, hipStream_t s);
+hipError_t launch_sweep_gather(const double* kb, const uint32_t* bidx, uint64_t nb, int axis, double gmin,
+                               double conv, double* kbs, double* bmax, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, bool write, hipStream_t s);
+
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {
   const Rec64* rec64;

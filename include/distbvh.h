@@ -254,6 +254,28 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 bvh_status bvh_closest_point(bvh_world w, const double *points, uint64_t n, const double *tri,
                              double *closest, uint32_t mem);
 
+/* max_variant_axis(a, b) (narrowphase/kokkos.hpp:119-161): the axis (0, 1, 2)
+ * of largest variance of the centroids of A and B together (na x 3 and
+ * nb x 3 doubles in `mem`), var = sum(c^2) - sum(c)^2 / (na + nb), the first
+ * axis on ties.  The sums are a fixed-order parallel reduction; the
+ * reference's parallel_reduce has no fixed order either.  *axis: host. */
+bvh_status bvh_max_variant_axis(bvh_world w, const double *ca, uint64_t na, const double *cb, uint64_t nb,
+                                uint32_t mem, int32_t *axis);
+/* sort_and_sweep_local(pa, a, pb, b, axis, f) (narrowphase/kokkos.hpp:164-248):
+ * the pairs of elements of patch A and patch B whose 26-DOPs overlap, found
+ * by sweeping B sorted by its slab minima on direction `axis` (0..12), scaled
+ * to uint32 over the union of the patches' slabs pa, pb (26 doubles each,
+ * host memory) and truncated.  ka / kb are na x 26 / nb x 26 element k-DOPs
+ * (min, max per direction), ga / gb their global ids (NULL: the local
+ * indices), all in `mem`; pairs (2 x uint64 per pair: id in A, id in B, in
+ * `mem`) come in the order the reference's callback sees them in for each
+ * element of A: A ascending, then B in sorted order.  *npairs is the count;
+ * pairs may be NULL to ask for it; BVH_E_OVERFLOW if cap is below it.
+ * nb < 2^30, at most 2^32 - 1 pairs. */
+bvh_status bvh_sort_and_sweep(bvh_world w, const double *pa, const double *ka, const uint64_t *ga, uint64_t na,
+                              const double *pb, const double *kb, const uint64_t *gb, uint64_t nb, int32_t axis,
+                              uint32_t mem, uint64_t *pairs, uint64_t cap, uint64_t *npairs);
+
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
   uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 or 4 = that

@@ -121,6 +121,10 @@ KNOWN = {
         "source": "tests/KokkosTest.cpp:87-139 (disabled in the reference build; arithmetic valid)",
         "grid_12x1x1_vs_unit_cube_hits": 12,
     },
+    "max_variant_axis": {
+        "source": "tests/KokkosTest.cpp:37-85 (disabled in the reference build; arithmetic valid)",
+        "grids_vs_unit_cube": [[[12, 1, 1], 0], [[1, 12, 1], 1], [[1, 1, 12], 2]],
+    },
     "narrowphase_functor": {
         "source": "tests/collision_object_test.cpp:376-443 (one rank)",
         "od": 2, "object1": [1, 1, 1], "object2": [2, 3, 2], "results": 12,

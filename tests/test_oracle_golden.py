@@ -278,6 +278,61 @@ def test_tree_invariants():
     assert got == exp
 
 
+def test_max_variant_axis_known():
+    """KokkosTest.cpp's grids of 12 elements along x, y, z against a unit cube."""
+    _, cb, _ = _grid(1, 1, 1)
+    for dims, axis in KNOWN["max_variant_axis"]["grids_vs_unit_cube"]:
+        _, ca, _ = _grid(*dims)
+        assert O.max_variant_axis(ca, cb) == axis
+
+
+def test_max_variant_axis_closed_form():
+    """Against numpy's two-pass variance (mean subtracted first): the axis of
+    largest variance, the first on ties (exact ties from integer grids)."""
+    rng = np.random.default_rng(5)
+    for scale in ([3.0, 1.0, 0.5], [0.2, 0.1, 2.0], [1.0, 4.0, 1.0]):
+        ca = rng.normal(size=(500, 3)) * scale + 7.0
+        cb = rng.normal(size=(300, 3)) * scale - 2.0
+        assert O.max_variant_axis(ca, cb) == int(np.argmax(np.concatenate([ca, cb]).var(axis=0)))
+    # integer grids: the sums are exact, so the ties are exact
+    g = np.array([(i, j, k) for i in range(4) for j in range(4) for k in range(1)], np.float64)
+    assert O.max_variant_axis(g, g[:1]) == 0 and O.max_variant_axis(g[:, [2, 0, 1]], g[:1, [2, 0, 1]]) == 1
+    g = np.array([(0, j, k) for j in range(3) for k in range(3)], np.float64)
+    assert O.max_variant_axis(g, g[:0]) == 1
+
+
+def sweep_order_closed_form(pa, ka, ga, pb, kb, gb, axis):
+    """The sweep's pairs as a definition: for each a in order, every b whose
+    26-DOP overlaps a's (brute force), listed in the stable order of B's
+    truncated scaled minima on the axis."""
+    gmin = min(pa[2 * axis], pb[2 * axis])
+    gmax = max(pa[2 * axis + 1], pb[2 * axis + 1])
+    conv = float(np.iinfo(np.uint32).max) / (gmax - gmin)
+    bq = np.trunc((kb[:, 2 * axis] - gmin) * conv).astype(np.uint64)
+    order = np.argsort(bq, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    hits = O.brute_pairs(ka, kb)
+    hits = hits[np.lexsort((rank[hits[:, 1].astype(np.int64)], hits[:, 0]))]
+    return np.stack([ga[hits[:, 0].astype(np.int64)], gb[hits[:, 1].astype(np.int64)]], axis=1)
+
+
+def test_sort_and_sweep_order():
+    """The oracle's pair order (a ascending, then B's sorted order) is the
+    closed form's, on the wavy sheets and on a degenerate sweep axis."""
+    a, b = W.wavy_sheets(12)
+    ka, ca = O.elements_poly(a)
+    kb, cb = O.elements_poly(b)
+    ga = np.arange(ka.shape[0], dtype=np.uint64) * 3 + 5
+    gb = np.arange(kb.shape[0], dtype=np.uint64) * 7 + 1
+    pka = np.empty(26); pkb = np.empty(26)
+    pka[0::2], pka[1::2] = ka[:, 0::2].min(0), ka[:, 1::2].max(0)
+    pkb[0::2], pkb[1::2] = kb[:, 0::2].min(0), kb[:, 1::2].max(0)
+    for axis in (0, 1, 2, 7):
+        got = O.sort_and_sweep(pka, ka, ga, pkb, kb, gb, axis)
+        assert np.array_equal(got, sweep_order_closed_form(pka, ka, ga, pkb, kb, gb, axis)), axis
+
+
 def test_sort_and_sweep_known():
     k = KNOWN["sort_and_sweep"]
     ka, ca, ga = _grid(12, 1, 1)
