@@ -17,6 +17,7 @@ BVH_MEM_HOST, BVH_MEM_DEVICE = 0, 1
 BVH_PREC_FP64, BVH_PREC_FP32_OUTWARD = 0, 1
 BVH_FLAG_SORT_RESULTS = 0x1
 BVH_FLAG_TIMING = 0x2
+BVH_FLAG_SERIAL_BUILDS = 0x4
 
 
 class BvhError(RuntimeError):

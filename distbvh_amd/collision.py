@@ -31,14 +31,15 @@ class CollisionWorld:
     """collision_world(od, config)."""
 
     def __init__(self, overdecomposition=1, morton_bits=63, leaf_size=1, precision="fp64",
-                 sort_results=False, timing=False, stream=None, device=0):
+                 sort_results=False, timing=False, stream=None, device=0, serial_builds=False):
         cfg = capi.WorldConfig()
         capi.load().bvh_world_config_init(C.byref(cfg))
         cfg.morton_bits = morton_bits
         cfg.leaf_size = leaf_size
         cfg.precision = capi.BVH_PREC_FP32_OUTWARD if precision == "fp32" else capi.BVH_PREC_FP64
         cfg.flags = (capi.BVH_FLAG_SORT_RESULTS if sort_results else 0) | \
-                    (capi.BVH_FLAG_TIMING if timing else 0)
+                    (capi.BVH_FLAG_TIMING if timing else 0) | \
+                    (capi.BVH_FLAG_SERIAL_BUILDS if serial_builds else 0)
         cfg.stream = stream
         cfg.device = device
         h = C.c_void_p()
@@ -253,18 +254,20 @@ class CollisionObject:
         g, gmem, gkeep = _ptr_mem(global_ids, np.uint64)
         if global_ids is not None and gmem != mem:
             raise ValueError("global ids and elements must live in the same memory")
+        # the previous arrays stay referenced until the library has finished
+        # the build that reads them
+        call("bvh_set_entity_data_poly", self.handle, p, nv, n, g, float(epsilon), mem, split_algorithm)
         self._keep = (keep, gkeep)
         self.n = n
-        call("bvh_set_entity_data_poly", self.handle, p, nv, n, g, float(epsilon), mem, split_algorithm)
 
     def set_entity_data_sphere(self, xyzr, global_ids=None, split_algorithm=CLUSTERING):
         """xyzr: (4, n) float64 planes x, y, z, r."""
         n = tuple(xyzr.shape)[1]
         p, mem, keep = _ptr_mem(xyzr, np.float64)
         g, _, gkeep = _ptr_mem(global_ids, np.uint64)
+        call("bvh_set_entity_data_sphere", self.handle, p, n, g, mem, split_algorithm)
         self._keep = (keep, gkeep)
         self.n = n
-        call("bvh_set_entity_data_sphere", self.handle, p, n, g, mem, split_algorithm)
 
     def set_entity_data_snapshot(self, kdops, centroids, global_ids=None, split_algorithm=CLUSTERING):
         """kdops: (n, 26) float64, centroids: (n, 3) float64."""
@@ -272,17 +275,17 @@ class CollisionObject:
         p, mem, keep = _ptr_mem(kdops, np.float64)
         c, _, ckeep = _ptr_mem(centroids, np.float64)
         g, _, gkeep = _ptr_mem(global_ids, np.uint64)
+        call("bvh_set_entity_data_snapshot", self.handle, p, c, n, g, mem, split_algorithm)
         self._keep = (keep, ckeep, gkeep)
         self.n = n
-        call("bvh_set_entity_data_snapshot", self.handle, p, c, n, g, mem, split_algorithm)
 
     def refit(self, data):
         """New element data of the last set_entity_data's kind and count
         (same array shape); the tree keeps its topology and every k-DOP is
         recomputed (bvh_refit)."""
         p, mem, keep = _ptr_mem(data, np.float64)
-        self._keep = (keep,) + tuple(self._keep[1:])
         call("bvh_refit", self.handle, p, mem)
+        self._keep = (keep,) + tuple(self._keep[1:])
 
     def device_bytes(self):
         v = C.c_uint64()

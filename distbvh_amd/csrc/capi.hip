@@ -115,6 +115,7 @@ struct bvh_world_s {
   DevBuf sw_ka, sw_kb, sw_ga, sw_gb, sw_bq, sw_bidx, sw_kbs, sw_bmax, sw_count, sw_total, sw_out, sw_part;
   // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
   DevBuf frame;
+  bool frame_fixed = false;  // a build that sets the frame has been enqueued in stream order
   LaunchShape shape;  // narrowphase launch shape (autotuned)
   // stream capture (batching form): objects whose queries were captured
   bool capturing = false;
@@ -191,6 +192,24 @@ struct bvh_object_s {
   // the last launched query: kind and the list capacities it ran with
   bool q_self = false, q_narrow = false;
   uint64_t q_pcap = 0, q_cap = 0, q_rcap = 0;
+  // Build stream: a build forks from the world stream onto it (so the builds
+  // of several objects run concurrently) and the world stream waits for it
+  // (ev_done) at the next call that is not a set_entity_data.
+  hipStream_t bs = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_done = nullptr;
+  bool pending = false;
+
+  bvh_object_s() = default;
+  bvh_object_s(const bvh_object_s&) = delete;
+  bvh_object_s& operator=(const bvh_object_s&) = delete;
+  ~bvh_object_s() {
+    if (bs) {
+      (void)hipStreamSynchronize(bs);
+      (void)hipStreamDestroy(bs);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+  }
 
   size_t device_bytes() const {
     const DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
@@ -208,6 +227,28 @@ struct bvh_object_s {
 };
 
 // ---- helpers ---------------------------------------------------------------------------
+// The world stream waits for o's build (if one is still pending).
+static bvh_status join(bvh_object_s* o) {
+  if (!o->pending) return BVH_OK;
+  o->pending = false;
+  HIP_TRY(hipStreamWaitEvent(o->w->stream, o->ev_done, 0));
+  return BVH_OK;
+}
+// ... for every object of the world: every call that reads objects' device
+// data or may be followed by the caller's use of it starts with this.
+static bvh_status join_all(bvh_world w) {
+  for (auto& o : w->objects) BVH_TRY(join(o.get()));
+  return BVH_OK;
+}
+// Points the world's stream at another stream for the scope (a build enqueued
+// on an object's build stream); restores it on every exit path.
+struct StreamSwap {
+  bvh_world w;
+  hipStream_t keep;
+  StreamSwap(bvh_world w_, hipStream_t s) : w(w_), keep(w_->stream) { w->stream = s; }
+  ~StreamSwap() { w->stream = keep; }
+};
+
 static TreeView view_of(const bvh_object_s* o) {
   TreeView v{};
   v.rec64 = o->rec64.as<Rec64>();
@@ -435,6 +476,7 @@ static bvh_status object_build(bvh_object_s* o) {
   if (o->fp32) {
     // the world's fp32 frame: the Morton bounds minimum of its first non-empty build
     HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
+    w->frame_fixed = true;
   }
   HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   w->stage_begin("delta");
@@ -466,6 +508,10 @@ static bvh_status object_refit(bvh_object_s* o) {
   return BVH_OK;
 }
 
+static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
+                                   const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
+                                   uint32_t split, hipStream_t bstream);
+
 static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                     const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
                                     uint32_t split) {
@@ -480,6 +526,41 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
   bvh_world w = o->w;
+  if (o->pending) {
+    // the previous build may still use the buffers this one may reallocate
+    HIP_TRY(hipStreamSynchronize(o->bs));
+    BVH_TRY(join(o));
+  }
+  // The build runs on the object's own stream, after everything enqueued on
+  // the world stream so far, unless builds are serial, a capture is open, or
+  // this fp32 build would be the one that fixes the world's frame (later
+  // builds read it, so it must precede them in stream order).
+  const bool fork = !(w->cfg.flags & BVH_FLAG_SERIAL_BUILDS) && !w->capturing &&
+                    !(w->cfg.precision == BVH_PREC_FP32_OUTWARD && !w->frame_fixed);
+  if (fork) {
+    if (!o->bs) {
+      HIP_TRY(hipStreamCreateWithFlags(&o->bs, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&o->ev_done, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(o->ev_fork, w->stream));
+    HIP_TRY(hipStreamWaitEvent(o->bs, o->ev_fork, 0));
+  }
+  const bvh_status st = set_entity_build(o, kind, nv, eps, data, cent, n, gids, mem, split, fork ? o->bs : w->stream);
+  if (fork) {
+    // the world stream waits for this at the next call that needs it (also
+    // after a failed build, so that nothing it enqueued is left unordered)
+    HIP_TRY(hipEventRecord(o->ev_done, o->bs));
+    o->pending = true;
+  }
+  return st;
+}
+
+static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
+                                   const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
+                                   uint32_t split, hipStream_t bstream) {
+  bvh_world w = o->w;
+  StreamSwap swap(w, bstream);
   hipStream_t s = w->stream;
   o->have_results = false;
   o->in.kind = kind;
@@ -673,6 +754,7 @@ static bvh_status results_ready(bvh_object_s* o) {
 template <typename K>
 static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   if (n == 0) return BVH_OK;
   if (!keys) return fail(BVH_E_INVALID, "null keys");
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 keys");
@@ -779,6 +861,7 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
 
 bvh_status bvh_world_destroy(bvh_world w) {
   if (!w) return BVH_OK;
+  (void)join_all(w);
   (void)hipStreamSynchronize(w->stream);
   w->objects.clear();
   for (hipEvent_t e : w->event_pool) (void)hipEventDestroy(e);
@@ -818,6 +901,7 @@ bvh_status bvh_start_iteration(bvh_world w) {
 
 bvh_status bvh_finish_iteration(bvh_world w) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   for (auto& o : w->objects) BVH_TRY(bvh_end_phase(o.get()));
   w->iteration_open = false;
   HIP_TRY(hipStreamSynchronize(w->stream));
@@ -826,12 +910,14 @@ bvh_status bvh_finish_iteration(bvh_world w) {
 
 bvh_status bvh_synchronize(bvh_world w) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   HIP_TRY(hipStreamSynchronize(w->stream));
   return BVH_OK;
 }
 
 bvh_status bvh_stage_times(bvh_world w, const char** names, double* ms, size_t cap, size_t* count) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   HIP_TRY(hipStreamSynchronize(w->stream));
   size_t k = 0;
   for (const Stage& st : w->stages) {
@@ -862,6 +948,7 @@ bvh_status bvh_object_device_bytes(bvh_object o, uint64_t* bytes) {
 
 bvh_status bvh_object_root_kdop(bvh_object o, double* kdop) {
   if (!o || !kdop) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   if (!o->built) return fail(BVH_E_STATE, "object not built");
   for (int i = 0; i < 26; ++i) kdop[i] = (i & 1) ? -1.7976931348623157e308 : 1.7976931348623157e308;
   if (o->in.n == 0) return BVH_OK;
@@ -919,6 +1006,7 @@ bvh_status bvh_set_entity_data_snapshot(bvh_object o, const double* kdops, const
 
 bvh_status bvh_refit(bvh_object o, const double* data, uint32_t mem) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   if (!o->built || !o->have_data) return fail(BVH_E_STATE, "refit before set_entity_data");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   const uint64_t n = o->in.n;
@@ -948,6 +1036,7 @@ bvh_status bvh_init_broadphase(bvh_object o) {
 
 bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
   if (!o || !other) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
   if (o == other) {
     // the reference skips self collisions in the narrowphase
@@ -965,6 +1054,7 @@ bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
 
 bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other) {
   if (!o || !other) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
   if (o == other) {
     o->npairs = 0;
@@ -980,6 +1070,7 @@ bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other) {
 
 bvh_status bvh_self_collision(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   return run_query(o, o, true);
 }
 
@@ -990,6 +1081,7 @@ bvh_status bvh_end_phase(bvh_object o) {
 
 bvh_status bvh_result_count(bvh_object o, uint64_t* count) {
   if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   *count = o->have_results ? o->npairs : 0;
   return BVH_OK;
@@ -997,6 +1089,7 @@ bvh_status bvh_result_count(bvh_object o, uint64_t* count) {
 
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_t* count) {
   if (!o || !pairs || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   *pairs = o->have_results ? o->result_ptr : nullptr;
   *count = o->have_results ? o->npairs : 0;
@@ -1005,6 +1098,7 @@ bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t** pairs, uint64_
 
 bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]) {
   if (!o || !digest) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   hipStream_t s = o->w->stream;
@@ -1017,6 +1111,7 @@ bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]) {
 
 bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t mem) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "result buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
@@ -1026,6 +1121,7 @@ bvh_status bvh_result_pairs(bvh_object o, uint64_t* out, uint64_t cap, uint32_t 
 
 bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_t cap, uint32_t mem) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "record buffer holds " + std::to_string(cap) + " of " + std::to_string(n));
@@ -1043,6 +1139,7 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_
 bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs, uint64_t* candidates,
                             uint64_t* rechecked, double* max_slab_error_ulps) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   if (node_visits) *node_visits = o->have_results ? o->nvisits : 0;
   if (patch_pairs) *patch_pairs = o->have_results ? o->npatch_pairs : 0;
@@ -1058,6 +1155,7 @@ bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch
 
 bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {
   if (!o || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   *count = o->have_results ? o->npatch_pairs : 0;
   return BVH_OK;
@@ -1065,6 +1163,7 @@ bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {
 
 bvh_status bvh_patch_pairs(bvh_object o, uint64_t* out, uint64_t cap) {
   if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   const uint64_t n = o->have_results ? o->npatch_pairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "patch pair buffer too small");
@@ -1081,6 +1180,7 @@ bvh_status bvh_patch_pairs(bvh_object o, uint64_t* out, uint64_t cap) {
 
 bvh_status bvh_object_bounds(bvh_object o, double* min3, double* inv3) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   Bounds b{};
   if (o->in.n > 0) BVH_TRY(copy_out(&b, o->bounds.p, sizeof(Bounds), BVH_MEM_HOST, o->w->stream));
   for (int c = 0; c < 3; ++c) {
@@ -1092,6 +1192,7 @@ bvh_status bvh_object_bounds(bvh_object o, double* min3, double* inv3) {
 
 bvh_status bvh_object_sorted(bvh_object o, uint64_t* codes, uint32_t* perm, uint32_t mem) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
@@ -1111,6 +1212,7 @@ bvh_status bvh_object_sorted(bvh_object o, uint64_t* codes, uint32_t* perm, uint
 
 bvh_status bvh_object_splits(bvh_object o, uint64_t* splits, size_t cap) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   const size_t ns = o->w->od - 1;
   if (cap < ns) return fail(BVH_E_OVERFLOW, "splits buffer too small");
   if (ns == 0) return BVH_OK;
@@ -1122,6 +1224,7 @@ bvh_status bvh_object_splits(bvh_object o, uint64_t* splits, size_t cap) {
 
 bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   if (o->fp32) return fail(BVH_E_UNSUPPORTED, "the fp32 form stores fp32 slabs only");
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
@@ -1141,6 +1244,7 @@ bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
 
 bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   const size_t od = o->w->od;
   if (cap < od) return fail(BVH_E_OVERFLOW, "patch buffer too small");
   hipStream_t s = o->w->stream;
@@ -1181,6 +1285,7 @@ bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
 
 bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t* indices, uint64_t cap, uint64_t* count) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   const size_t od = o->w->od;
   if (patch >= od) return fail(BVH_E_INVALID, "patch index out of range");
   const uint64_t n = o->in.n;
@@ -1200,6 +1305,7 @@ bvh_status bvh_patch_elements(bvh_object o, size_t patch, uint64_t* indices, uin
 bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node* nodes, size_t cap, size_t* nnodes,
                                  uint64_t* leaf_patches, size_t leaf_cap, size_t* nleaves) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n, od = w->od;
@@ -1260,6 +1366,7 @@ bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node* nodes, size_t cap,
 bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint64_t* ranges, size_t cap,
                            size_t* nnodes, size_t* root) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
   if (o->fp32) return fail(BVH_E_UNSUPPORTED, "tree export is fp64 only");
   const uint64_t n = o->in.n;
   const size_t nn = n ? 2 * n - 1 : 0;
@@ -1301,6 +1408,7 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
 
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   if (n == 0) return BVH_OK;
   if (!vals) return fail(BVH_E_INVALID, "null values");
   hipStream_t s = w->stream;
@@ -1317,6 +1425,7 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
 bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, const double* tri, double* closest,
                              uint32_t mem) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   if (w->capturing) return fail(BVH_E_STATE, "closest_point inside a capture");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   if (!tri) return fail(BVH_E_INVALID, "null triangle");
@@ -1342,6 +1451,7 @@ bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, cons
 bvh_status bvh_max_variant_axis(bvh_world w, const double* ca, uint64_t na, const double* cb, uint64_t nb,
                                 uint32_t mem, int32_t* axis) {
   if (!w || !axis) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(w));
   if (w->capturing) return fail(BVH_E_STATE, "max_variant_axis inside a capture");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   if ((na && !ca) || (nb && !cb)) return fail(BVH_E_INVALID, "null centroids");
@@ -1359,6 +1469,7 @@ bvh_status bvh_sort_and_sweep(bvh_world w, const double* pa, const double* ka, c
                               const double* pb, const double* kb, const uint64_t* gb, uint64_t nb, int32_t axis,
                               uint32_t mem, uint64_t* pairs, uint64_t cap, uint64_t* npairs) {
   if (!w || !npairs) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(w));
   *npairs = 0;
   if (w->capturing) return fail(BVH_E_STATE, "sort_and_sweep inside a capture");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
@@ -1426,6 +1537,7 @@ bvh_status bvh_sort_and_sweep(bvh_world w, const double* pa, const double* ka, c
 // ---- autotuner ---------------------------------------------------------------------------
 bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   if (!o || !other || !best) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
   if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "autotune inside a capture");
@@ -1514,6 +1626,7 @@ struct bvh_graph_s {
 
 bvh_status bvh_capture_begin(bvh_world w) {
   if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
   if (w->capturing || t_capturing) return fail(BVH_E_STATE, "a capture is already open on this thread");
   HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
   w->capturing = true;
@@ -1551,6 +1664,7 @@ bvh_status bvh_capture_end(bvh_world w, bvh_graph* out) {
 
 bvh_status bvh_graph_launch(bvh_graph g) {
   if (!g) return fail(BVH_E_INVALID, "null graph");
+  BVH_TRY(join_all(g->w));
   if (g->w->capturing) return fail(BVH_E_STATE, "graph launch inside a capture");
   if (g->gen != g_alloc_gen.load())
     return fail(BVH_E_STATE, "device buffers were reallocated since the capture; capture again");

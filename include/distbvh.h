@@ -8,7 +8,13 @@
  *
  * Threading: one host thread drives a world.  All device work of a world is
  * ordered on one HIP stream (the caller's, or one the world creates with the
- * default priority).
+ * default priority), with one exception: the build that set_entity_data
+ * enqueues runs on the object's own stream (default priority), after all
+ * work enqueued on the world stream before the call, so that the builds of
+ * several objects run concurrently; the world stream waits for it at the
+ * next call on the world that is not a set_entity_data or init_broadphase
+ * (BVH_FLAG_SERIAL_BUILDS keeps every build on the world stream).  Until
+ * then the caller's own work on the world stream is not ordered after it.
  *
  * Element data: polytope vertices are SoA planes, plane (v*3 + c) of length
  * n holds coordinate c (x, y, z) of vertex v of every element; spheres are
@@ -57,6 +63,7 @@ typedef enum bvh_precision {
 /* bvh_world_config.flags */
 #define BVH_FLAG_SORT_RESULTS 0x1u /* sort element pairs by (element_p, element_q) */
 #define BVH_FLAG_TIMING 0x2u       /* record per-stage HIP event timings */
+#define BVH_FLAG_SERIAL_BUILDS 0x4u /* build on the world stream, one object after another */
 
 /* world_config, collision_world.hpp:48-52 (log levels replaced by device
  * options) */

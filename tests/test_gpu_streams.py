@@ -1,0 +1,117 @@
+"""Concurrent object builds (each set_entity_data builds on the object's own
+stream, joined into the world stream at the next call that is not a
+set_entity_data / init_broadphase) against serial builds
+(BVH_FLAG_SERIAL_BUILDS): the same trees, patches and pairs, in both
+precision forms, and the stream order the caller relies on."""
+import numpy as np
+import pytest
+
+from distbvh_amd import CollisionWorld
+from distbvh_amd import workloads as W
+from helpers import sort_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a GPU"
+    return torch
+
+
+def _step(w, objs, datas):
+    for o, d in zip(objs, datas):
+        o.set_entity_data_poly(d)
+    for o in objs:
+        o.init_broadphase()
+    out = []
+    for i in range(len(objs)):
+        for j in range(i + 1, len(objs)):
+            objs[i].broadphase(objs[j])
+            out.append((objs[i].result_count(), objs[i].digest()))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_concurrent_builds_equal_serial(torch_cuda, precision):
+    """Three objects rebuilt over several steps with moving data: every
+    step's pairs, each object's patches and (fp64) sorted k-DOPs equal the
+    serial builds'."""
+    torch = torch_cuda
+    a, b = W.wavy_sheets(96)
+    c = b.copy()
+    c[:, 2, :] += 0.01
+    ta, tb, tc = (torch.from_numpy(x).cuda() for x in (a, b, c))
+    res = {}
+    for serial in (True, False):
+        with CollisionWorld(16, precision=precision, serial_builds=serial, leaf_size=2) as w:
+            objs = [w.create_collision_object() for _ in range(3)]
+            steps = []
+            for k in range(3):
+                shift = torch.tensor([0.0, 0.0, 0.002 * k], dtype=torch.float64, device="cuda").view(1, 3, 1)
+                steps.append(_step(w, objs, [ta, tb + shift, tc - shift]))
+            patches = [[(p["size"], p["kdop"].tobytes()) for p in o.local_patches()] for o in objs]
+            kd = [o.sorted_kdops() for o in objs] if precision == "fp64" else None
+            res[serial] = (steps, patches, kd)
+    s, c_ = res[True], res[False]
+    assert s[0] == c_[0]
+    assert s[1] == c_[1]
+    if precision == "fp64":
+        for x, y in zip(s[2], c_[2]):
+            assert np.array_equal(x, y)
+
+
+def test_caller_work_between_builds(torch_cuda):
+    """Work the caller enqueues on the world stream after one object's
+    set_entity_data is seen by the next object's build (it forks after it),
+    and the pairs equal the serial builds'."""
+    torch = torch_cuda
+    a, b = W.wavy_sheets(64)
+    ta = torch.from_numpy(a).cuda()
+    tb0 = torch.from_numpy(b).cuda()
+    stream = torch.cuda.current_stream()
+    got = {}
+    for serial in (True, False):
+        with CollisionWorld(8, serial_builds=serial, stream=stream.cuda_stream) as w:
+            oa = w.create_collision_object()
+            ob = w.create_collision_object()
+            for k in range(3):
+                oa.set_entity_data_poly(ta)
+                # produced on the world stream while A's build may still run
+                tb = tb0 * 1.0
+                tb[:, 2, :] += 0.003 * k
+                ob.set_entity_data_poly(tb)
+                oa.init_broadphase()
+                ob.init_broadphase()
+                oa.broadphase(ob)
+                got.setdefault(serial, []).append((oa.result_count(), oa.digest(), sort_pairs(oa.pairs()).tobytes()))
+    assert got[True] == got[False]
+
+
+def test_concurrent_builds_stage_times(torch_cuda):
+    """Per-stage timings are still recorded for every build (on the build
+    streams), and a capture after concurrent builds replays the same pairs."""
+    torch = torch_cuda
+    a, b = W.wavy_sheets(64)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    with CollisionWorld(8, timing=True) as w:
+        oa = w.create_collision_object()
+        ob = w.create_collision_object()
+        _step(w, [oa, ob], [ta, tb])
+        st = w.stage_times()
+        names = [n for n, _ in st]
+        assert names.count("tree") == 2 and names.count("sort") == 2 and "narrowphase" in names
+        assert all(ms >= 0.0 for _, ms in st)
+        ref = (oa.result_count(), oa.digest())
+        with w.capture() as g:
+            oa.set_entity_data_poly(ta)
+            ob.set_entity_data_poly(tb)
+            oa.init_broadphase()
+            ob.init_broadphase()
+            oa.broadphase(ob)
+        for _ in range(3):
+            g.launch()
+            assert (oa.result_count(), oa.digest()) == ref
+        _step(w, [oa, ob], [ta, tb])
+        assert (oa.result_count(), oa.digest()) == ref
