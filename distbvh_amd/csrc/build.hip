@@ -743,7 +743,10 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   __shared__ ChunkLds<FP32> L;
   const uint64_t n = in.n;
   const uint32_t t = threadIdx.x;
-  const uint32_t c0 = blockIdx.x * kTreeChunk;
+  // chunks adjacent in the sorted order on one XCD: they gather elements from
+  // the same lines of the element planes
+  const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t c0 = chunk * kTreeChunk;
   const uint32_t c1 = (uint32_t)((uint64_t)c0 + kTreeChunk < n ? c0 + kTreeChunk : n);
   L.cnt[t] = 0;
   {
@@ -824,7 +827,7 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   // a first arrival whose sibling lies outside the chunk leaves its arrival word
   if (c0 + t + 1 < c1 && L.cnt[t] == 1)
     b.counters[c0 + t] = ((unsigned long long)(L.far[t] + 1u) << 32) | L.fpatch[t];
-  if (t < 2) b.front_a[2 * blockIdx.x + t] = (t < L.nfront) ? L.front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
+  if (t < 2) b.front_a[2 * chunk + t] = (t < L.nfront) ? L.front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
 }
 
 // Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one

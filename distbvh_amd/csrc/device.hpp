@@ -161,6 +161,16 @@ __device__ __forceinline__ uint32_t morton30(double x, double y, double z, const
   return (expand32(q[2]) << 2) + (expand32(q[1]) << 1) + expand32(q[0]);
 }
 
+// Work-group b runs on XCD b % 8 (round-robin dispatch).  Tile t = xcd_tile(b)
+// gives each XCD a contiguous range of tiles (a bijection of [0, nb)), so
+// tiles that are adjacent in the sorted order (and touch the same lines:
+// neighbouring parts of the tree, elements near each other in the input)
+// share one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, j = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return x * per + (x < rem ? x : rem) + j;
+}
+
 // ---- tree records ---------------------------------------------------------------
 // Internal node k (k in [0, n-2]) is the split between sorted positions k and
 // k+1; leaf p is node n-1+p.

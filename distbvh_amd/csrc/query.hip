@@ -54,14 +54,6 @@ __device__ __forceinline__ void empty_slabs(double* k) {
   }
 }
 
-// Work-group b runs on XCD b % 8 (round-robin dispatch).  Tile t = xcd_tile(b)
-// gives each XCD a contiguous range of tiles, so queries that are adjacent in
-// the sorted order (and walk the same part of the tree) share one L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
-  const uint32_t x = b & 7u, j = b >> 3, per = nb >> 3, rem = nb & 7u;
-  return x * per + (x < rem ? x : rem) + j;
-}
-
 // Per-wave pair staging in LDS: hits are appended with a ballot (all lanes of
 // the wave are converged in the uniform traversal loop) and written to
 // global memory in coalesced runs with one atomic per flush.
