@@ -5,7 +5,7 @@ Workload (one step): two deformed contact sheets, 1024 x 1024 quads =
 z = 0.01 sin(2 pi 8 x) cos(2 pi 8 y); sheet B = A + (0, 0, 0.005) + seeded
 N(0, 0.002^2) vertex noise.  Per step both objects are rebuilt
 (26-DOP bounds, 63-bit Morton codes, radix sort, cluster splits into 256
-patches, bottom-up tree with refit, leaf size 4) and A is collided with B
+patches, bottom-up tree with refit, one element per leaf) and A is collided with B
 (patch broadphase + element narrowphase); the step ends when the pair count is
 on the host.
 
@@ -347,7 +347,9 @@ def main():
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
     ap.add_argument("--spheres", type=int, default=16777216, help="spheres of the spheres-self workload")
     ap.add_argument("--od", type=int, default=256)
-    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--leaf", type=int, default=1,
+                    help="leaf size of the CPU oracle's element trees (the reference's is 1, "
+                         "tree_build.hpp:69); the GPU trees hold one element per leaf")
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="fp64",
                     help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
                          "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
@@ -370,8 +372,7 @@ def main():
     args.precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: this library has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -68,7 +68,7 @@ def _width(kd):
 class Server:
     """Resident objects per mesh pair, refit-or-rebuild per request."""
 
-    def __init__(self, od=64, precision="fp64", budget_bytes=200 << 30, growth_limit=1.2, leaf_size=4):
+    def __init__(self, od=64, precision="fp64", budget_bytes=200 << 30, growth_limit=1.2, leaf_size=1):
         self.world = CollisionWorld(od, leaf_size=leaf_size, precision=precision)
         self.budget = budget_bytes
         self.growth_limit = growth_limit

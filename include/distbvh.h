@@ -70,7 +70,10 @@ typedef enum bvh_precision {
 typedef struct bvh_world_config {
   uint32_t morton_bits; /* 63 (21 bits/axis, default) or 30 (10 bits/axis, the
                            reference clustering path, split/cluster.hpp:139) */
-  uint32_t leaf_size;   /* elements per tree leaf, 1..8 (default 1) */
+  uint32_t leaf_size;   /* 1..8, accepted for configuration compatibility: the trees
+                           hold one element per leaf, as the reference's do
+                           (tree_build.hpp:69); a leaf's slabs sit in its parent's
+                           record, so the walk tests leaves two at a time */
   uint32_t precision;   /* bvh_precision */
   uint32_t flags;
   void *stream;         /* hipStream_t, or NULL for a world-owned stream */
