@@ -274,9 +274,9 @@ def bench_spheres_self(args, torch, stream):
 
 def bench_batch(args, torch, stream):
     """Batching form: a batch of 4,096 height-field meshes (lognormal sizes,
-    ~19M triangles) in one object, fp32-outward, rebuilt and self-collided per
-    step; the step is captured once into a HIP graph and replayed, after the
-    autotuner picked the launch shape."""
+    ~19M triangles) in one object, rebuilt and self-collided per step; the
+    step is captured once into a HIP graph and replayed, after the autotuner
+    picked the launch shape."""
     from distbvh_amd import CollisionWorld
     from distbvh_amd import workloads as W
 
@@ -284,7 +284,7 @@ def bench_batch(args, torch, stream):
     n = a.shape[2]
     ta = torch.from_numpy(a).cuda()
     del a
-    prec = "fp32"  # the batching form runs the fp32-outward tree
+    prec = args.precisions[0]
     world = CollisionWorld(4096, morton_bits=63, leaf_size=args.leaf, timing=False, stream=stream.cuda_stream,
                            precision=prec)
     o = world.create_collision_object()
@@ -322,7 +322,8 @@ def bench_batch(args, torch, stream):
             "config": {"workload": "4,096 height-field meshes on a 16^3 lattice, lognormal sizes (median "
                                    "2,048, clipped to [64, 65,536]), 10%% neighbour overlap; %d triangles" % n,
                        "precision": prec, "od": 4096,
-                       "autotuned": {"block_size": tune["block_size"], "xcd_remap": tune["xcd_remap"],
+                       "autotuned": {"batch": tune["batch"], "block_size": tune["block_size"],
+                                     "xcd_remap": tune["xcd_remap"],
                                      "query_ms": round(tune["ms"], 4),
                                      "tried_ms": [round(t, 4) for t in tune["tried_ms"]]},
                        "pairs": int(npairs), "pairs_per_s": round(npairs / (ms_graph * 1e-3), 1),
