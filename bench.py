@@ -5,7 +5,7 @@ Workload (one step): two deformed contact sheets, 1024 x 1024 quads =
 z = 0.01 sin(2 pi 8 x) cos(2 pi 8 y); sheet B = A + (0, 0, 0.005) + seeded
 N(0, 0.002^2) vertex noise.  Per step both objects are rebuilt
 (26-DOP bounds, 63-bit Morton codes, radix sort, cluster splits into 256
-patches, bottom-up tree with refit, one element per leaf) and A is collided with B
+patches, bottom-up tree with refit, leaf size 4) and A is collided with B
 (patch broadphase + element narrowphase); the step ends when the pair count is
 on the host.
 
@@ -284,7 +284,8 @@ def bench_batch(args, torch, stream):
     n = a.shape[2]
     ta = torch.from_numpy(a).cuda()
     del a
-    prec = args.precisions[0]
+    # fp32-outward k-DOPs unless a precision is named (--precision fp64)
+    prec = args.precisions[0] if args.precision_given else "fp32"
     world = CollisionWorld(4096, morton_bits=63, leaf_size=args.leaf, timing=False, stream=stream.cuda_stream,
                            precision=prec)
     o = world.create_collision_object()
@@ -348,9 +349,10 @@ def main():
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
     ap.add_argument("--spheres", type=int, default=16777216, help="spheres of the spheres-self workload")
     ap.add_argument("--od", type=int, default=256)
-    ap.add_argument("--leaf", type=int, default=1,
-                    help="leaf size of the CPU oracle's element trees (the reference's is 1, "
-                         "tree_build.hpp:69); the GPU trees hold one element per leaf")
+    ap.add_argument("--leaf", type=int, default=4,
+                    help="leaf size of the production workload (4), used by the CPU oracle's element "
+                         "trees; the GPU trees hold one element per leaf and test leaves from their "
+                         "parents' records, so it does not change their work")
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="fp64",
                     help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
                          "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
@@ -370,6 +372,7 @@ def main():
                     help="first copy a MIB-sized device buffer once (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    args.precision_given = any(a == "--precision" or a.startswith("--precision=") for a in sys.argv[1:])
     args.precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
 
     if args.gpus != 1:
