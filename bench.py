@@ -346,6 +346,8 @@ def main():
                          "graph per step over a batch of 4,096 meshes; serving: Poisson request stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=4,
+                    help="serving: concurrent slots (host threads with a world and stream each)")
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
     ap.add_argument("--spheres", type=int, default=16777216, help="spheres of the spheres-self workload")
     ap.add_argument("--od", type=int, default=256)
@@ -392,7 +394,7 @@ def main():
     if args.workload == "serving":
         from distbvh_amd import serve as S
         prec = args.precisions[0]
-        r = S.serve(rate=args.rate, duration=args.duration, precision=prec)
+        r = S.serve(rate=args.rate, duration=args.duration, precision=prec, slots=args.slots)
         out = {"metric": "serving p99 latency (arrival -> pairs on host), Poisson %.0f req/s, mesh pairs "
                          "10k-2M triangles" % args.rate,
                "value": round(r["p99_ms"], 3), "unit": "ms", "n_gpus": 1, "steps": r["requests"], "warmup": 0,
@@ -400,8 +402,11 @@ def main():
                "vs_baseline": None, "dtype": "f32+f64" if prec == "fp32" else "f64", "data": "synthetic",
                "config": {"workload": "open-loop Poisson stream; each request a wavy-sheet pair (log-uniform "
                                       "10k-2M triangles per object, 8 pairs resident) with B rigidly moved + "
-                                      "0.1% jitter, uploaded from pinned host memory; refit B, rebuild when "
-                                      "the root 26-DOP grew > 1.2x; pairs copied to the host",
+                                      "0.1%% jitter, uploaded from pinned host memory; refit B, rebuild when "
+                                      "the root 26-DOP grew > 1.2x; pairs copied to the host; %d slots "
+                                      "(host threads with a world and stream each) serve requests at "
+                                      "once" % r["slots"],
+                          "slots": r["slots"],
                           "precision": prec, "parallelism": "single GPU"},
                "serving": r, "cpu_baseline": None}
         print(json.dumps(out), flush=True)

@@ -9,7 +9,9 @@ B's tree on the new vertices, and rebuilds B instead when the refitted root
 last build) or when the pair is not cached; then it runs broadphase +
 narrowphase of A against B and copies the pairs to the host.  Latency is
 measured from a request's arrival time to its pairs on the host, so queueing
-behind earlier requests counts.
+behind earlier requests counts.  Several slots (host threads, each with its
+own world and stream) serve requests at once, so requests that arrive while
+others are on the GPU overlap with them on the device.
 
 The workload's sizes and seeds are parameters of `serve()`; `bench.py
 --workload serving` prints its p50 / p99 latency and pair throughput.
@@ -121,10 +123,33 @@ class Server:
         self.world.close()
 
 
+def _pair_digest(pairs):
+    """Order-independent digest of a pair list: SHA-256 of the sorted rows."""
+    import hashlib
+
+    p = np.ascontiguousarray(pairs, dtype=np.uint64)
+    if len(p):
+        p = p[np.lexsort((p[:, 1], p[:, 0]))]
+    return hashlib.sha256(p.tobytes()).hexdigest()[:16]
+
+
 def serve(rate=200.0, duration=2.0, k=8, lo=1e4, hi=2e6, variants=4, seed=0xC5, precision="fp64", od=64,
-          budget_bytes=200 << 30, warm=True, pin=True):
+          budget_bytes=200 << 30, warm=True, pin=True, slots=4, record=False):
     """Runs the open-loop stream for `duration` seconds of arrivals and
-    returns latency percentiles (ms), pairs/s and the refit / rebuild counts."""
+    returns latency percentiles (ms), pairs/s and the refit / rebuild counts.
+
+    Requests are served by `slots` workers, each a host thread driving its
+    own world (its own HIP stream) and holding the resident objects of the
+    mesh pairs it owns (pair m belongs to slot m % slots; its share of the
+    device-memory budget bounds its cache).  A request that arrives while
+    others are on the GPU is admitted as soon as its slot is free, and the
+    requests of different slots run on the device at the same time.  With
+    `record`, the result also lists every request's (pair count, digest) in
+    arrival order, so a run in the mix can be compared with a run of the
+    same requests alone (slots=1)."""
+    import queue
+    import threading
+
     import torch
 
     pool = mesh_pool(k, lo, hi, seed)
@@ -142,30 +167,75 @@ def serve(rate=200.0, duration=2.0, k=8, lo=1e4, hi=2e6, variants=4, seed=0xC5, 
     gaps = -np.log(1.0 - W.uniform(seed ^ 0x1, nreq)) / rate
     arrivals = np.cumsum(gaps)
     picks = np.minimum((W.uniform(seed ^ 0x2, nreq) * k).astype(np.int64), k - 1)
-    server = Server(od=od, precision=precision, budget_bytes=budget_bytes)
-    if warm:  # first build of every pair (cold start is not part of the stream)
-        for i, (at, vs) in enumerate(payload):
-            server.handle(i, at.numpy(), vs[0].numpy())
+    slots = max(1, min(int(slots), k))
     lat = np.zeros(nreq)
-    npairs = 0
-    t0 = time.perf_counter()
+    counts = np.zeros(nreq, np.int64)
+    digests = [None] * nreq
+    queues = [queue.Queue() for _ in range(slots)]
+    warmed = threading.Barrier(slots + 1)
+    start = {}
+    stats = [None] * slots
+    errors = []
+
+    def worker(s):
+        srv = None
+        waited = False
+        try:
+            srv = Server(od=od, precision=precision, budget_bytes=budget_bytes // slots)
+            if warm:  # first build of every owned pair (cold start is not part of the stream)
+                for m in range(s, k, slots):
+                    at, vs = payload[m]
+                    srv.handle(m, at.numpy(), vs[0].numpy())
+            waited = True
+            warmed.wait()
+            while True:
+                item = queues[s].get()
+                if item is None:
+                    break
+                r, m = item
+                at, vs = payload[m]
+                pairs = srv.handle(m, at.numpy(), vs[r % variants].numpy())
+                lat[r] = (time.perf_counter() - start["t0"]) - arrivals[r]
+                counts[r] = len(pairs)
+                if record:
+                    digests[r] = _pair_digest(pairs)
+        except Exception as e:  # reported after the stream
+            errors.append(e)
+            if not waited:
+                warmed.wait()
+        finally:
+            if srv is not None:
+                stats[s] = (srv.refits, srv.rebuilds, srv.evictions)
+                srv.close()
+
+    threads = [threading.Thread(target=worker, args=(s,), daemon=True) for s in range(slots)]
+    for th in threads:
+        th.start()
+    warmed.wait()
+    start["t0"] = t0 = time.perf_counter()
     for r in range(nreq):
         now = time.perf_counter() - t0
         if now < arrivals[r]:
             time.sleep(arrivals[r] - now)
         m = int(picks[r])
-        at, vs = payload[m]
-        pairs = server.handle(m, at.numpy(), vs[r % variants].numpy())
-        npairs += len(pairs)
-        lat[r] = (time.perf_counter() - t0) - arrivals[r]
+        queues[m % slots].put((r, m))
+    for q in queues:
+        q.put(None)
+    for th in threads:
+        th.join()
     wall = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    npairs = int(counts.sum())
     out = {
-        "requests": nreq, "rate_req_s": rate, "wall_s": wall,
+        "requests": nreq, "rate_req_s": rate, "wall_s": wall, "slots": slots,
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
-        "pairs": int(npairs), "pairs_per_s": npairs / wall,
-        "refits": server.refits, "rebuilds": server.rebuilds, "evictions": server.evictions,
+        "pairs": npairs, "pairs_per_s": npairs / wall,
+        "refits": sum(x[0] for x in stats if x), "rebuilds": sum(x[1] for x in stats if x),
+        "evictions": sum(x[2] for x in stats if x),
         "mesh_triangles": [int(p[0].shape[2]) for p in pool],
     }
-    server.close()
+    if record:
+        out["per_request"] = [(int(c), d) for c, d in zip(counts, digests)]
     return out
