@@ -471,6 +471,9 @@ def main():
             "fp32_stages_ms": {k: round(v, 4) for k, v in r32["stages"].items()},
             "fp32_launch_autotuned": r32["launch"],
             "fp32_candidate_pairs": int(r32["stats"]["candidate_pairs"]),
+            # leaf candidates of the fp32 walk that are not fp64 pairs, per fp64 pair
+            "fp32_excess_false_positive_rate": round(
+                (int(r32["stats"]["candidate_pairs"]) - int(r64["npairs"])) / max(1, int(r64["npairs"])), 6),
             "fp32_rechecked_pairs": int(r32["stats"]["rechecked_pairs"]),
             "fp32_max_slab_error_ulps": r32["stats"]["max_slab_error_ulps"],
         }
