@@ -152,6 +152,9 @@ def serve(rate=200.0, duration=2.0, k=8, lo=1e4, hi=2e6, variants=4, seed=0xC5, 
 
     import torch
 
+    from . import capi
+
+    capi.load()  # before the workers start, so that they share one handle
     pool = mesh_pool(k, lo, hi, seed)
     # request payloads: `variants` moved copies of each B, in pinned host memory
     payload = []
