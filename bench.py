@@ -48,9 +48,10 @@ def kernel_bytes(stage, n, npairs, n_other, fp32=False, elem_bytes=72):
         return n * (8 + 4 + 16)
     if stage == "delta":       # read codes, write patch id + split height
         return n * (8 + 4 + 1 + 4)
-    if stage == "tree":        # gather elements + permutation + patch ids, write
-        # each node's slabs into its parent's record + the parent's links
-        return n * (elem_bytes + 4 + 4 + 1 + slab + 16)
+    if stage == "tree":        # gather elements + permutation + patch ids + split
+        # heights, write every record (each node's slabs and links sit in its
+        # parent's record)
+        return n * (elem_bytes + 4 + 4 + 1) + max(n - 1, 0) * rec
     if stage == "narrowphase":  # query slabs (from their parents' records), the
         # other object's records once, pair keys
         return n * (slab + 4 + 2) + (n_other - 1) * rec + 8 * npairs
