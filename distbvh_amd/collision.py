@@ -292,6 +292,13 @@ class CollisionObject:
         call("bvh_object_device_bytes", self.handle, C.byref(v))
         return v.value
 
+    def release(self):
+        """bvh_object_release: free the object's device memory; it is empty
+        until its next set_entity_data."""
+        call("bvh_object_release", self.handle)
+        self._keep = None
+        self.n = 0
+
     def root_kdop(self):
         k = np.zeros(26)
         call("bvh_object_root_kdop", self.handle, C.c_void_p(k.ctypes.data))

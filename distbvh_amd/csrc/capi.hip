@@ -63,6 +63,15 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  // frees the memory (a later ensure allocates again); graphs captured on it
+  // become stale
+  void release() {
+    if (!p) return;
+    g_alloc_gen.fetch_add(1);
+    (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
   bvh_status ensure(size_t want) {
     if (want <= bytes && p) return BVH_OK;
     if (t_capturing)
@@ -211,17 +220,22 @@ struct bvh_object_s {
     if (ev_done) (void)hipEventDestroy(ev_done);
   }
 
-  size_t device_bytes() const {
-    const DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
-                           &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
-                           &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
-                           &root, &rec64, &rootbox64, &rec32, &rootbox32, &status,
-                           &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
-                           &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
-                           &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
-                           &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x};
+  // every device buffer of the object
+  template <typename F>
+  void for_each_buf(F f) {
+    DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
+                     &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
+                     &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
+                     &root, &rec64, &rootbox64, &rec32, &rootbox32, &status,
+                     &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
+                     &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
+                     &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
+                     &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x};
+    for (DevBuf* d : all) f(*d);
+  }
+  size_t device_bytes() {
     size_t t = 0;
-    for (const DevBuf* d : all) t += d->bytes;
+    for_each_buf([&](DevBuf& d) { t += d.bytes; });
     return t;
   }
 };
@@ -943,6 +957,35 @@ bvh_status bvh_object_id(bvh_object o, size_t* id) {
 bvh_status bvh_object_device_bytes(bvh_object o, uint64_t* bytes) {
   if (!o || !bytes) return fail(BVH_E_INVALID, "null argument");
   *bytes = o->device_bytes();
+  return BVH_OK;
+}
+
+bvh_status bvh_object_release(bvh_object o) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  bvh_world w = o->w;
+  if (w->capturing) return fail(BVH_E_STATE, "release inside a capture");
+  BVH_TRY(join_all(w));
+  // queued work of the world and of the object's build stream may still use
+  // the buffers
+  if (o->bs) HIP_TRY(hipStreamSynchronize(o->bs));
+  HIP_TRY(hipStreamSynchronize(w->stream));
+  o->for_each_buf([](DevBuf& d) { d.release(); });
+  // results of other objects against this one read its buffers: gone too
+  for (auto& x : w->objects)
+    if (x->result_other == o) {
+      x->have_results = false;
+      x->results_pending = false;
+      x->result_other = nullptr;
+    }
+  o->built = false;
+  o->have_data = false;
+  o->have_results = false;
+  o->results_pending = false;
+  o->npairs = o->npatch_pairs = o->nvisits = o->ncandidates = o->nrecheck = 0;
+  o->result_ptr = nullptr;
+  o->result_other = nullptr;
+  o->ref_rp = nullptr;
+  o->in = ElemInput{};
   return BVH_OK;
 }
 

@@ -75,6 +75,7 @@ class Server:
         self.budget = budget_bytes
         self.growth_limit = growth_limit
         self.cache = {}   # mesh id -> dict(a, b, width0, last_use)
+        self.free = []    # released object pairs, reused for the next pair admitted
         self.refits = 0
         self.rebuilds = 0
         self.evictions = 0
@@ -86,15 +87,22 @@ class Server:
     def _evict(self):
         while self.cache and self._resident_bytes() > self.budget:
             victim = min(self.cache, key=lambda m: self.cache[m]["last_use"])
-            del self.cache[victim]
+            e = self.cache.pop(victim)
+            # the device memory goes back; the handles are kept for reuse
+            e["a"].release()
+            e["b"].release()
+            self.free.append((e["a"], e["b"]))
             self.evictions += 1
 
     def handle(self, mesh_id, a_verts, b_verts):
         self.clock += 1
         e = self.cache.get(mesh_id)
         if e is None:
-            oa = self.world.create_collision_object()
-            ob = self.world.create_collision_object()
+            if self.free:
+                oa, ob = self.free.pop()
+            else:
+                oa = self.world.create_collision_object()
+                ob = self.world.create_collision_object()
             oa.set_entity_data_poly(a_verts)
             oa.init_broadphase()
             e = {"a": oa, "b": ob, "width0": None}

@@ -208,6 +208,11 @@ bvh_status bvh_local_patches(bvh_object o, bvh_patch_info *out, size_t cap);
 bvh_status bvh_object_overdecomposition(bvh_object o, size_t *od);
 /* device memory the object holds, in bytes (its build and result buffers) */
 bvh_status bvh_object_device_bytes(bvh_object o, uint64_t *bytes);
+/* Frees the object's device memory (no reference counterpart: the reference's
+ * objects live as long as their world).  The handle stays valid; the object is
+ * empty until its next set_entity_data.  For serving loops that keep many
+ * objects resident under a device-memory budget. */
+bvh_status bvh_object_release(bvh_object o);
 /* the root node's 26-DOP (fp32 form: the outward fp32 slabs shifted back to
  * world coordinates); for refit-or-rebuild decisions */
 bvh_status bvh_object_root_kdop(bvh_object o, double *kdop);

@@ -49,3 +49,40 @@ def test_serving_latency_report(torch_cuda):
     assert out["requests"] == 20 and out["slots"] == 2
     assert 0.0 <= out["p50_ms"] <= out["p99_ms"] <= out["max_ms"]
     assert out["pairs"] > 0 and out["pairs_per_s"] > 0
+
+
+def test_serving_under_a_small_budget(torch_cuda):
+    """A budget that holds about one mesh pair per slot: pairs are evicted
+    (their device memory released, the objects reused) and rebuilt on their
+    next request, and every request's pairs stay those of the run alone."""
+    alone = S.serve(slots=1, **KW)
+    tight = S.serve(slots=2, budget_bytes=2 * 3 << 20, **KW)
+    assert tight["evictions"] > 0
+    assert tight["per_request"] == alone["per_request"]
+
+
+def test_object_release(torch_cuda):
+    """bvh_object_release frees the object's memory, invalidates results
+    against it, and the object builds again on its next set_entity_data."""
+    from distbvh_amd import CollisionWorld, capi
+
+    a, b = W.wavy_sheets(32)
+    with CollisionWorld(8) as w:
+        oa = w.create_collision_object()
+        ob = w.create_collision_object()
+        oa.set_entity_data_poly(a)
+        ob.set_entity_data_poly(b)
+        oa.init_broadphase()
+        ob.init_broadphase()
+        oa.broadphase(ob)
+        ref = (oa.result_count(), oa.digest())
+        assert ref[0] > 0 and ob.device_bytes() > 0
+        ob.release()
+        assert ob.device_bytes() == 0
+        assert oa.result_count() == 0
+        with pytest.raises(capi.BvhError):
+            oa.broadphase(ob)
+        ob.set_entity_data_poly(b)
+        ob.init_broadphase()
+        oa.broadphase(ob)
+        assert (oa.result_count(), oa.digest()) == ref
