@@ -129,6 +129,7 @@ struct bvh_world_s {
   // stream capture (batching form): objects whose queries were captured
   bool capturing = false;
   std::vector<bvh_object_s*> captured;
+  std::vector<bvh_graph> graphs;  // live graphs captured on this world
   unsigned long long* h_count = nullptr;  // pinned, 16 words: query counters and status words
   bool iteration_open = false;
 
@@ -542,15 +543,18 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   bvh_world w = o->w;
   if (o->pending) {
     // the previous build may still use the buffers this one may reallocate
-    HIP_TRY(hipStreamSynchronize(o->bs));
+    // (inside a capture nothing is reallocated, and nothing may synchronize)
+    if (!w->capturing) HIP_TRY(hipStreamSynchronize(o->bs));
     BVH_TRY(join(o));
   }
   // The build runs on the object's own stream, after everything enqueued on
-  // the world stream so far, unless builds are serial, a capture is open, or
-  // this fp32 build would be the one that fixes the world's frame (later
-  // builds read it, so it must precede them in stream order).
-  const bool fork = !(w->cfg.flags & BVH_FLAG_SERIAL_BUILDS) && !w->capturing &&
-                    !(w->cfg.precision == BVH_PREC_FP32_OUTWARD && !w->frame_fixed);
+  // the world stream so far (inside a capture the build stream joins the
+  // capture there and the graph gets a branch per build), unless builds are
+  // serial or this fp32 build would be the one that fixes the world's frame
+  // (later builds read it, so it must precede them in stream order).
+  const bool fork = !(w->cfg.flags & BVH_FLAG_SERIAL_BUILDS) &&
+                    !(w->cfg.precision == BVH_PREC_FP32_OUTWARD && !w->frame_fixed) &&
+                    !(w->capturing && !o->bs);  // no stream is created inside a capture
   if (fork) {
     if (!o->bs) {
       HIP_TRY(hipStreamCreateWithFlags(&o->bs, hipStreamNonBlocking));
@@ -873,10 +877,14 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
   return BVH_OK;
 }
 
+static void orphan_graphs(bvh_world w);
+
 bvh_status bvh_world_destroy(bvh_world w) {
   if (!w) return BVH_OK;
   (void)join_all(w);
   (void)hipStreamSynchronize(w->stream);
+  // graphs outlive their world only as handles to destroy
+  orphan_graphs(w);
   w->objects.clear();
   for (hipEvent_t e : w->event_pool) (void)hipEventDestroy(e);
   if (w->h_count) (void)hipHostFree(w->h_count);
@@ -1667,6 +1675,11 @@ struct bvh_graph_s {
   std::vector<bvh_object_s*> objects;
 };
 
+static void orphan_graphs(bvh_world w) {
+  for (bvh_graph g : w->graphs) g->w = nullptr;
+  w->graphs.clear();
+}
+
 bvh_status bvh_capture_begin(bvh_world w) {
   if (!w) return fail(BVH_E_INVALID, "null world");
   BVH_TRY(join_all(w));
@@ -1682,12 +1695,16 @@ bvh_status bvh_capture_end(bvh_world w, bvh_graph* out) {
   if (!w || !out) return fail(BVH_E_INVALID, "null argument");
   *out = nullptr;
   if (!w->capturing) return fail(BVH_E_STATE, "no capture open");
+  // builds forked inside the capture rejoin the world stream before it ends
+  const bvh_status js = join_all(w);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(w->stream, &graph);
   w->capturing = false;
   t_capturing = false;
-  if (e != hipSuccess) {
+  if (e != hipSuccess || js != BVH_OK) {
     w->captured.clear();
+    if (graph) (void)hipGraphDestroy(graph);
+    if (js != BVH_OK) return js;
     return fail(BVH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
   auto g = std::make_unique<bvh_graph_s>();
@@ -1701,12 +1718,14 @@ bvh_status bvh_capture_end(bvh_world w, bvh_graph* out) {
     (void)hipGraphDestroy(graph);
     return fail(BVH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
+  w->graphs.push_back(g.get());
   *out = g.release();
   return BVH_OK;
 }
 
 bvh_status bvh_graph_launch(bvh_graph g) {
   if (!g) return fail(BVH_E_INVALID, "null graph");
+  if (!g->w) return fail(BVH_E_STATE, "the graph's world was destroyed");
   BVH_TRY(join_all(g->w));
   if (g->w->capturing) return fail(BVH_E_STATE, "graph launch inside a capture");
   if (g->gen != g_alloc_gen.load())
@@ -1721,7 +1740,11 @@ bvh_status bvh_graph_launch(bvh_graph g) {
 
 bvh_status bvh_graph_destroy(bvh_graph g) {
   if (!g) return BVH_OK;
-  (void)hipStreamSynchronize(g->w->stream);
+  if (g->w) {
+    (void)hipStreamSynchronize(g->w->stream);
+    auto& v = g->w->graphs;
+    v.erase(std::remove(v.begin(), v.end(), g), v.end());
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;

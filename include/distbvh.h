@@ -327,7 +327,9 @@ bvh_status bvh_get_launch(bvh_world w, bvh_tuning *shape);
  * element data in place; its results are read with the usual calls, which
  * synchronize; a result list that outgrew its buffer reports
  * BVH_E_OVERFLOW (run the step once outside the graph and capture again).
- * sort_results cannot be captured. */
+ * sort_results cannot be captured.  Builds of objects built before the
+ * capture fork onto their build streams inside it, so the graph has a branch
+ * per build.  A graph may outlive its world only to be destroyed. */
 bvh_status bvh_capture_begin(bvh_world w);
 bvh_status bvh_capture_end(bvh_world w, bvh_graph *graph);
 bvh_status bvh_graph_launch(bvh_graph graph);

@@ -115,3 +115,69 @@ def test_concurrent_builds_stage_times(torch_cuda):
             assert (oa.result_count(), oa.digest()) == ref
         _step(w, [oa, ob], [ta, tb])
         assert (oa.result_count(), oa.digest()) == ref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_capture_of_concurrent_builds(torch_cuda, precision):
+    """A captured two-object step whose builds fork onto the objects' streams
+    (a graph with a branch per build), replayed on new contents of the
+    element buffer: every replay gives the pairs of serial eager builds of
+    the same data."""
+    torch = torch_cuda
+    a, b = W.wavy_sheets(64)
+    ta = torch.from_numpy(a).cuda()
+    tb = torch.from_numpy(b).cuda()
+    buf = tb.clone()
+    with CollisionWorld(8, precision=precision) as w, \
+            CollisionWorld(8, precision=precision, serial_builds=True) as ref:
+        oa = w.create_collision_object()
+        ob = w.create_collision_object()
+        ra = ref.create_collision_object()
+        rb = ref.create_collision_object()
+        _step(w, [oa, ob], [ta, buf])  # sizes the buffers, creates the build streams
+        # the expected pairs first: any device allocation in the process
+        # (here the serial world's) makes a captured graph stale
+        shifts, exps = [], []
+        for k in range(4):
+            shift = torch.tensor([0.0, 0.0, 0.0015 * k], dtype=torch.float64, device="cuda").view(1, 3, 1)
+            buf.copy_(tb + shift)
+            exps.append(_step(ref, [ra, rb], [ta, buf])[0])
+            shifts.append(shift)
+        torch.cuda.synchronize()
+        with w.capture() as g:
+            oa.set_entity_data_poly(ta)
+            ob.set_entity_data_poly(buf)
+            oa.init_broadphase()
+            ob.init_broadphase()
+            oa.broadphase(ob)
+        for k in range(4):
+            buf.copy_(tb + shifts[k])
+            torch.cuda.synchronize()
+            g.launch()
+            assert (oa.result_count(), oa.digest()) == exps[k], k
+        g.close()
+
+
+def test_graph_outlives_its_world(torch_cuda):
+    """A graph destroyed after its world: launching it is refused, and
+    destroying it is safe."""
+    from distbvh_amd import capi
+
+    torch = torch_cuda
+    a, b = W.wavy_sheets(16)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    w = CollisionWorld(4)
+    oa = w.create_collision_object()
+    ob = w.create_collision_object()
+    _step(w, [oa, ob], [ta, tb])
+    with w.capture() as g:
+        oa.set_entity_data_poly(ta)
+        ob.set_entity_data_poly(tb)
+        oa.init_broadphase()
+        ob.init_broadphase()
+        oa.broadphase(ob)
+    g.launch()
+    w.close()
+    with pytest.raises(capi.BvhError):
+        g.launch()
+    g.close()
