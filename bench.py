@@ -201,12 +201,15 @@ def bench_sheets(args, torch, stream):
         split = {"clustering": capi.BVH_SPLIT_CLUSTERING, "geom_axis": capi.BVH_SPLIT_GEOM_AXIS,
                  "ml_geom_axis": capi.BVH_SPLIT_ML_GEOM_AXIS}[args.split]
 
-        def step():
+        def step_launch():
             oa.set_entity_data_poly(ta, split_algorithm=split)
             ob.set_entity_data_poly(tb, split_algorithm=split)
             oa.init_broadphase()
             ob.init_broadphase()
             oa.broadphase(ob)
+
+        def step():
+            step_launch()
             return oa.result_count()
 
         step()
@@ -215,6 +218,18 @@ def bench_sheets(args, torch, stream):
         world.stage_times()
         ms, stages, per_launch, counts, npairs = run_steps(world, step, args.steps, args.warmup, stream, torch)
         qstats = oa.result_stats()
+        digest = oa.digest()
+        graph_ms = None
+        if args.split == "clustering":
+            # the batching form on the same step: one HIP graph per step (a
+            # branch per build), replayed on the resident element buffers
+            with world.capture() as g:
+                step_launch()
+            gms, _, _, _, gpairs = run_steps(world, lambda: (g.launch(), oa.result_count())[1], args.steps,
+                                             args.warmup, stream, torch)
+            graph_ok = int(gpairs) == int(npairs) and oa.digest() == digest
+            g.close()
+            graph_ms = {"ms_per_step": round(gms, 4), "pairs_equal_eager": bool(graph_ok)}
         wl = "wavy_sheets_2M" if args.nq == 1024 and args.od == 256 else None
         # the narrowphase: the kernel with the most device time per step (the
         # builds of A and B overlap, so their stage times are wall times)
@@ -224,7 +239,7 @@ def bench_sheets(args, torch, stream):
                                       dom="narrowphase")
         roof["kernel_name"] = stage_kernel(roof["kernel"], fp32, tune)
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
-                     "roofline": roof, "step_hbm_frac": step_frac, "digest": oa.digest(),
+                     "roofline": roof, "step_hbm_frac": step_frac, "digest": digest, "graph": graph_ms,
                      "launch": {k: tune[k] for k in ("batch", "block_size", "xcd_remap")}}
         world.close()
     return a, b, n, out
@@ -460,6 +475,7 @@ def main():
                               "are wall times under that overlap)"),
                    "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in first["stages"].items()},
+        "graph_form": first.get("graph"),
         "step_hbm_frac": first["step_hbm_frac"],
         "roofline": first["roofline"],
     }
@@ -470,6 +486,7 @@ def main():
             "fp32_speedup": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
             "pairs_identical": bool(r64["npairs"] == r32["npairs"] and r64["digest"] == r32["digest"]),
             "fp32_stages_ms": {k: round(v, 4) for k, v in r32["stages"].items()},
+            "fp32_graph_form": r32.get("graph"),
             "fp32_launch_autotuned": r32["launch"],
             "fp32_candidate_pairs": int(r32["stats"]["candidate_pairs"]),
             # leaf candidates of the fp32 walk that are not fp64 pairs, per fp64 pair
