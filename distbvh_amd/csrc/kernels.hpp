@@ -221,12 +221,13 @@ struct PairOut {
 // block (64, 128 or 256), whether blocks are remapped so each XCD walks a
 // contiguous range of query tiles, and the fp64 walk's batch: 1 = one
 // record per visit in scalar registers (k_narrow64s), 2 or 4 = that many
-// records per memory round trip staged in LDS (k_narrow64).  The fp32 walk
-// always takes one record in scalar registers.
+// records per memory round trip loaded into LDS (k_narrow64; 2 by default,
+// the fastest on dense contact).  The fp32 walk always takes one record in
+// scalar registers.
 struct LaunchShape {
   unsigned block = 256;
   bool xcd_remap = true;
-  unsigned batch = 1;
+  unsigned batch = 2;
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)

@@ -405,8 +405,9 @@ __device__ __forceinline__ void overlap_pair(const double* qk, const Rec64& r, b
 
 // fp64 form: the same record walk, K records per memory round trip.  Two
 // fp64 boxes (416 B) do not fit the scalar registers, so each 448-byte record
-// is staged in LDS with one 8-byte vector load per lane (56 lanes) and every
-// lane reads both boxes as LDS broadcasts.  Each iteration pops up to K nodes
+// is loaded into LDS by a direct-to-LDS load (28 lanes x 16 bytes, no VGPR
+// staging, which keeps the walk at 80 VGPRs and six waves per SIMD for K = 2)
+// and every lane reads both boxes as LDS broadcasts.  Each iteration pops up to K nodes
 // off the wave's stack and issues all their loads before it tests any: the
 // set of visited nodes does not depend on K (a node is visited when some lane
 // overlaps it and its parent was visited), only the number of dependent
@@ -480,14 +481,16 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
       break;
     }
     {
-      // every load in flight before the first LDS store
-      uint64_t v[K];
+      // every record straight from memory into its LDS slot (the loads
+      // write LDS directly, so no VGPRs hold them in between): lanes 0..27
+      // each bring 16 bytes, which land at the slot + 16 x lane
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if ((uint32_t)k < m && lane < sizeof(Rec64) / 8) v[k] = ((const uint64_t*)(t.rec64 + xs[k]))[lane];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        if ((uint32_t)k < m && lane < sizeof(Rec64) / 8) ((uint64_t*)&R[k])[lane] = v[k];
+        if ((uint32_t)k < m && lane < sizeof(Rec64) / 16)
+          __builtin_amdgcn_global_load_lds((const void*)((const char*)(t.rec64 + xs[k]) + lane * 16),
+                                           (void __attribute__((address_space(3)))*)&R[k], 16, 0, 0);
+      __builtin_amdgcn_s_waitcnt(0);  // the LDS writes of those loads are done
+      asm volatile("" ::: "memory");
       __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll

@@ -293,8 +293,8 @@ bvh_status bvh_sort_and_sweep(bvh_world w, const double *pa, const double *ka, c
 
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
-  uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 or 4 = that
-                           many records per memory round trip staged in LDS */
+  uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 (default)
+                           or 4 = that many records per memory round trip loaded into LDS */
   uint32_t block_size;  /* narrowphase threads per block: 64, 128 or 256 */
   uint32_t xcd_remap;   /* 1: each XCD walks a contiguous range of query tiles */
   uint32_t tried;       /* configurations timed (entries of tried_ms) */

@@ -172,7 +172,7 @@ def test_launch_shapes(torch_cuda, precision):
     want_self = sort_pairs(O.brute_self(kb))
     assert len(want) > 0 and len(want_self) > 0
     with CollisionWorld(16, leaf_size=1, precision=precision) as w:
-        assert w.launch() == {"batch": 1, "block_size": 256, "xcd_remap": True}
+        assert w.launch() == {"batch": 2, "block_size": 256, "xcd_remap": True}
         oa = gpu_object(w, "poly", a, device=True)
         ob = gpu_object(w, "poly", b, device=True)
         for batch in (1, 2, 4):
