@@ -121,6 +121,25 @@ def cpu_baseline(a, b, od, leaf):
     }
 
 
+def stream_copy_gbs(torch, mib=1024, reps=5):
+    """Measured device-to-device copy bandwidth (read + write bytes per
+    second), the practical HBM ceiling beside the 8 TB/s peak."""
+    src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return round(2 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
+
+
 def pick_launch(args, world, autotune):
     """The narrowphase launch shape a form is timed at: --launch pins one
     (profiling runs of an autotuned shape without the tuner's launches),
@@ -475,6 +494,7 @@ def main():
                               "are wall times under that overlap)"),
                    "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in first["stages"].items()},
+        "stream_copy_gbs": stream_copy_gbs(torch),
         "graph_form": first.get("graph"),
         "step_hbm_frac": first["step_hbm_frac"],
         "roofline": first["roofline"],
