@@ -30,6 +30,7 @@ constexpr int kSortTile = kSortBlock * kSortItems;  // 4096
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr int kLookWindow = 4;  // predecessors read per look-back round trip
 
 __device__ __forceinline__ uint32_t lb_load(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -210,17 +211,30 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_pass(SortArgs<K> a, int pas
 
   uint32_t excl = 0;
   if (tile > 0) {
+    // decoupled look-back over a window of kLookWindow predecessors whose
+    // words are loaded together (one round trip per window, not per tile):
+    // their aggregates are added in order up to the first inclusive prefix;
+    // a word not yet published is polled until it is (tile 0 publishes an
+    // inclusive prefix, so the walk ends by it)
+    const uint32_t* lb0 = a.lookback + (uint64_t)pass * a.ntiles * 256 + t;
     int64_t prev = (int64_t)tile - 1;
-    for (;;) {
-      const uint32_t* lp = a.lookback + ((uint64_t)pass * a.ntiles + (uint64_t)prev) * 256 + t;
-      uint32_t v = lb_load(lp);
-      while ((v & ~kValMask) == 0) {
-        __builtin_amdgcn_s_sleep(1);
-        v = lb_load(lp);
+    bool done = false;
+    while (!done) {
+      uint32_t v[kLookWindow];
+#pragma unroll
+      for (int k = 0; k < kLookWindow; ++k) v[k] = prev - k >= 0 ? lb_load(lb0 + (uint64_t)(prev - k) * 256) : 0u;
+#pragma unroll
+      for (int k = 0; k < kLookWindow; ++k) {
+        if (done || prev - k < 0) break;
+        uint32_t x = v[k];
+        while ((x & ~kValMask) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          x = lb_load(lb0 + (uint64_t)(prev - k) * 256);
+        }
+        excl += x & kValMask;
+        done = (x & ~kValMask) == kFlagInc;
       }
-      excl += v & kValMask;
-      if ((v & ~kValMask) == kFlagInc) break;
-      --prev;
+      prev -= kLookWindow;
     }
     lb_store(&lb[t], kFlagInc | (excl + total));
   }
