@@ -337,14 +337,14 @@ __device__ __forceinline__ uint32_t patch_end(const uint32_t* splits, uint64_t o
 // leaf climbs; at a parent, the first child to arrive stops and the second
 // merges both children's slabs, writes the parent and continues.  Every
 // synchronization is workgroup-local:
-//  * phase 1 (k_tree_leaves): a workgroup owns 256 consecutive leaves and
+//  * phase 1 (k_tree_leaves): a workgroup owns kTreeChunk (128) consecutive leaves and
 //    counts arrivals in LDS.  A child whose parent splits exactly at a chunk
 //    boundary (at most the left child of internal node c1-1 and the right
 //    child of internal node c0-1) is handed to the next phase; a first arriver
 //    whose sibling extends past the chunk leaves its arrival word in global
 //    memory.
-//  * phase k >= 2 (k_tree_up): chunks 256x larger; a workgroup continues the
-//    handed-over nodes of its 256 sub-chunks.  An arrival is one
+//  * phase k >= 2 (k_tree_up): chunks kTreeChunk times larger; a workgroup
+//    continues the handed-over nodes of its kTreeChunk sub-chunks.  An arrival is one
 //    workgroup-scope atomic exchange on the parent's arrival word: the first
 //    arriver deposits its far end (l for a left child, r for a right child)
 //    and its patch, so the second learns the parent's range and patch from
@@ -446,7 +446,10 @@ __device__ __forceinline__ bool is_patch_root(uint32_t patch, uint32_t l, uint32
   return patch != kMixed && (l == 0 || dl >= 128) && (r == n - 1 || dr >= 128);
 }
 
-constexpr uint32_t kTreeChunk = 256;
+// leaves per phase-1 workgroup (and the growth factor of the upper phases):
+// 128 leaves keep the phase-1 LDS at 30 KB, three workgroups per CU (measured:
+// 612 us per 2M-triangle tree instead of 653 at 256)
+constexpr uint32_t kTreeChunk = kTreeLeafChunk;
 
 // Upper-phase climb (phases >= 2), one thread per handed-over node of the
 // workgroup's chunk [c0, c1).  Node [l, r] of patch `mypatch` with slabs kd
@@ -632,7 +635,7 @@ __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
   if (i < count) pk64[i] = (i & 1) ? 0ull : ~0ull;
 }
 
-// Phase-1 state of one 256-leaf chunk, all in LDS.  Slot j holds the newest
+// Phase-1 state of one chunk of kTreeChunk leaves, all in LDS.  Slot j holds the newest
 // finished node whose range starts at position c0 + j (its slabs, last
 // position, patch); endslot[j] is the slot of the newest finished node that
 // ends at c0 + j.  A parent is written to its first position's slot, so a
@@ -830,7 +833,7 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   if (t < 2) b.front_a[2 * chunk + t] = (t < L.nfront) ? L.front[t] : FrontNode{kEnd, kEnd, kEnd, 0u};
 }
 
-// Upper phase: a block of 2 x 256 threads per chunk of 256 sub-chunks, one
+// Upper phase: a block of 2 x kTreeChunk threads per chunk of kTreeChunk sub-chunks, one
 // thread per handed-over node (two per sub-chunk).  A handed-over node's
 // slabs are in its half of its parent's record; thread t's LDS slot is t.
 template <bool FP32>
@@ -1091,7 +1094,7 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // upper phases: chunks 256x larger until one chunk holds every leaf
+  // upper phases: chunks kTreeChunk times larger until one chunk holds every leaf
   uint64_t chunk = kTreeChunk;
   FrontNode* fin = b.front_a;
   FrontNode* fout = b.front_b;

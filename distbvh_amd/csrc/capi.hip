@@ -323,8 +323,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->patch_root.ensure(od * 4));
   BVH_TRY(o->root.ensure(4));
   BVH_TRY(o->status.ensure(4 * sizeof(uint32_t)));  // [error bits, -, max fp32 slab error, -]
-  BVH_TRY(o->front_a.ensure((2 * ((n1 + 255) / 256) + 2) * sizeof(FrontNode)));
-  BVH_TRY(o->front_b.ensure((2 * ((n1 + 255) / 256) + 2) * sizeof(FrontNode)));
+  const uint64_t nchunks = (n1 + kTreeLeafChunk - 1) / kTreeLeafChunk;
+  BVH_TRY(o->front_a.ensure((2 * nchunks + 2) * sizeof(FrontNode)));
+  BVH_TRY(o->front_b.ensure((2 * nchunks + 2) * sizeof(FrontNode)));
   BVH_TRY(o->patch_k64.ensure(od * kSlabs * sizeof(unsigned long long)));
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));

@@ -53,6 +53,9 @@ struct alignas(16) FrontNode {
   uint32_t l, r, patch, d;
 };
 
+// leaves per workgroup of the first tree-build phase
+constexpr uint32_t kTreeLeafChunk = 128;
+
 struct BuildBufs {
   // original order
   double* centroid;        // 3 planes of n (kPoly)
@@ -89,7 +92,7 @@ struct BuildBufs {
   int32_t* rootbox32;      // fp32 form: the root's slabs (26, ord32 images)
   double* origin;          // fp32 form: the world frame (3 doubles)
   uint32_t* status;        // device error bits (kErr*)
-  FrontNode* front_a;      // 2 per 256-leaf chunk: nodes handed to the next build phase
+  FrontNode* front_a;      // 2 per chunk of kTreeLeafChunk leaves: nodes handed to the next build phase
   FrontNode* front_b;
   unsigned long long* patch_k64;  // od x 26 exact (fp64) patch slabs, ordered_u64 encoding
 };
