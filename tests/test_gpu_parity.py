@@ -273,9 +273,16 @@ def _check_tree(tree, skd, n, od, splits):
             assert (int(b[p]), int(e[p]) - 1) in spans, p
 
 
-@pytest.mark.parametrize("case", ["wavy64_od16", "wavy200_od64", "spheres_od8", "wavy1024_od256", "flat_od1"])
+@pytest.mark.parametrize("case", ["wavy64_od16", "wavy200_od64", "spheres_od8", "wavy1024_od256", "flat_od1",
+                                  # around the build phases' chunk boundaries (128 leaves, x128 per phase)
+                                  "spheres_n127", "spheres_n128", "spheres_n129", "spheres_n16383",
+                                  "spheres_n16384", "spheres_n16385", "spheres_n2097153"])
 def test_tree_structure(torch_cuda, case):
-    if case.startswith("wavy"):
+    if case.startswith("spheres_n"):
+        n = int(case.split("_n")[1])
+        data = W.sphere_cloud(n, radius=5e-3, ncomp=8, sigma=0.05, shift=(0, 0, 0), seed=n)
+        kind, od = "sphere", 8
+    elif case.startswith("wavy"):
         nq = int(case[4:].split("_")[0])
         od = int(case.split("od")[1])
         data, _ = W.wavy_sheets(nq)
