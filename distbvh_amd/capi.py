@@ -114,6 +114,7 @@ SIGNATURES = {
     "bvh_exclusive_scan_u32": [_vp, _vp, _u64],
     "bvh_closest_point": [_vp, _vp, _u64, _vp, _vp, _u32],
     "bvh_object_release": [_vp],
+    "bvh_element_kdops": [_vp, _vp, _u32, _u64, C.c_double, _u32, _vp],
     "bvh_max_variant_axis": [_vp, _vp, _u64, _vp, _u64, _u32, _vp],
     "bvh_sort_and_sweep": [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, C.c_int32, _u32, _vp, _u64, _vp],
     "bvh_object_patch_tree": [_vp, _vp, C.c_size_t, _vp, _vp, C.c_size_t, _vp],

@@ -145,6 +145,22 @@ class CollisionWorld:
              C.c_void_p(out.ctypes.data), capi.BVH_MEM_HOST)
         return out
 
+    def element_kdops(self, verts, epsilon=0.0):
+        """bvh_element_kdops: the (n, 26) k-DOPs of (nv, 3, n) vertex planes
+        (numpy: host memory; CUDA tensor: device memory, result on device)."""
+        nv, _, n = tuple(verts.shape)
+        if hasattr(verts, "data_ptr"):
+            import torch
+            out = torch.empty((n, 26), dtype=torch.float64, device=verts.device)
+            call("bvh_element_kdops", self.handle, C.c_void_p(verts.data_ptr()), nv, n, float(epsilon),
+                 capi.BVH_MEM_DEVICE, C.c_void_p(out.data_ptr()))
+            return out
+        v = np.ascontiguousarray(verts, np.float64)
+        out = np.empty((n, 26), np.float64)
+        call("bvh_element_kdops", self.handle, C.c_void_p(v.ctypes.data), nv, n, float(epsilon),
+             capi.BVH_MEM_HOST, C.c_void_p(out.ctypes.data))
+        return out
+
     def max_variant_axis(self, ca, cb):
         """bvh_max_variant_axis: the axis of largest variance of the (na, 3)
         and (nb, 3) centroids together (numpy or CUDA tensors, both alike)."""

@@ -1500,6 +1500,35 @@ bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, cons
 }
 
 // ---- sort and sweep between two patches ---------------------------------------------
+bvh_status bvh_element_kdops(bvh_world w, const double* verts, uint32_t nv, uint64_t n, double eps, uint32_t mem,
+                             double* kdops) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "element_kdops inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (nv < 1 || nv > 8) return fail(BVH_E_INVALID, "verts_per_element must be in 1..8");
+  if (n == 0) return BVH_OK;
+  if (!verts || !kdops) return fail(BVH_E_INVALID, "null vertices or output");
+  if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements");
+  hipStream_t s = w->stream;
+  ElemInput in{};
+  in.kind = kPoly;
+  in.nv = (int)nv;
+  in.eps = eps;
+  in.n = n;
+  const double* dv = nullptr;
+  BVH_TRY(stage_in(w->sw_ka, verts, (uint64_t)nv * 3 * n, mem, s, &dv));
+  in.data = dv;
+  if (mem == BVH_MEM_DEVICE) {
+    HIP_TRY(launch_element_kdops(in, kdops, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BVH_OK;
+  }
+  BVH_TRY(w->sw_kbs.ensure(n * 26 * sizeof(double)));
+  HIP_TRY(launch_element_kdops(in, w->sw_kbs.as<double>(), s));
+  return copy_out(kdops, w->sw_kbs.p, n * 26 * sizeof(double), BVH_MEM_HOST, s);
+}
+
 bvh_status bvh_max_variant_axis(bvh_world w, const double* ca, uint64_t na, const double* cb, uint64_t nb,
                                 uint32_t mem, int32_t* axis) {
   if (!w || !axis) return fail(BVH_E_INVALID, "null argument");

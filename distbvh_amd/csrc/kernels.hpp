@@ -188,6 +188,8 @@ hipError_t launch_sweep_prepare(const double* kb, uint64_t nb, int axis, double 
 hipError_t launch_sweep_gather(const double* kb, const uint32_t* bidx, uint64_t nb, int axis, double gmin,
                                double conv, double* kbs, double* bmax, hipStream_t s);
 hipError_t launch_sweep(const SweepArgs& a, bool write, hipStream_t s);
+// 26-DOPs of in.n polytopes of in.nv vertices (kPoly planes), n x 26
+hipError_t launch_element_kdops(const ElemInput& in, double* out, hipStream_t s);
 
 // ---- queries ----------------------------------------------------------------------
 struct TreeView {

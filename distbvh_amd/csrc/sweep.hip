@@ -162,6 +162,36 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     sweep_one<WRITE>(a, ai, lane);
 }
 
+// kdop::from_vertices for a batch of polytopes (kdop.hpp:238-264): element
+// e's 26-DOP (n x 26) with the build's own projections, for callers that
+// sweep elements outside an object.
+template <int NV>
+__global__ __launch_bounds__(256) void k_element_kdops(ElemInput in, double* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= in.n) return;
+  double kd[kSlabs];
+  element_kdop<kPoly, NV>(in, (uint32_t)e, kd);
+#pragma unroll
+  for (int i = 0; i < kSlabs; ++i) out[e * kSlabs + i] = kd[i];
+}
+
+hipError_t launch_element_kdops(const ElemInput& in, double* out, hipStream_t s) {
+  if (in.n == 0) return hipSuccess;
+  const dim3 g((uint32_t)((in.n + 255) / 256)), b(256);
+  switch (in.nv) {
+    case 1: hipLaunchKernelGGL(k_element_kdops<1>, g, b, 0, s, in, out); break;
+    case 2: hipLaunchKernelGGL(k_element_kdops<2>, g, b, 0, s, in, out); break;
+    case 3: hipLaunchKernelGGL(k_element_kdops<3>, g, b, 0, s, in, out); break;
+    case 4: hipLaunchKernelGGL(k_element_kdops<4>, g, b, 0, s, in, out); break;
+    case 5: hipLaunchKernelGGL(k_element_kdops<5>, g, b, 0, s, in, out); break;
+    case 6: hipLaunchKernelGGL(k_element_kdops<6>, g, b, 0, s, in, out); break;
+    case 7: hipLaunchKernelGGL(k_element_kdops<7>, g, b, 0, s, in, out); break;
+    case 8: hipLaunchKernelGGL(k_element_kdops<8>, g, b, 0, s, in, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_variant_axis(const double* ca, uint64_t na, const double* cb, uint64_t nb, double* part,
                                int32_t* axis, hipStream_t s) {
   const uint64_t n = na + nb;

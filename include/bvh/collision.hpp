@@ -8,6 +8,7 @@
 //   bvh::narrowphase_result(_pair), typed_narrowphase_result<T>,
 //   bvh::broadphase_collision<T> collision_query.hpp:50-161
 //   bvh::patch<>                 patch.hpp:55-123
+//   bvh::sort_and_sweep_local, bvh::max_variant_axis  narrowphase/kokkos.hpp:119-248
 // Differences: there is one process and one GPU (no vt); errors throw
 // bvh::bvh_error instead of terminating; element types tell the library how
 // to bound them through bvh::element_traits (below).
@@ -19,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -437,6 +439,99 @@ inline void collision_object::broadphase(collision_object& other) {
   } else {
     BVH_FACADE_CALL(bvh_broadphase(h_, other.h_));
   }
+}
+
+// ---- sort and sweep between two patches (narrowphase/kokkos.hpp:119-248) ---------------
+// These run on the world's device; the reference's take no world.
+namespace detail {
+// the elements' 26-DOPs (n x 26, the build's own values) and centroids (n x 3)
+template <typename T>
+void element_bounds(collision_world& w, span<const T> el, std::vector<double>& kd, std::vector<double>& ce) {
+  using tr = element_traits<T>;
+  const std::size_t n = el.size();
+  kd.assign(26 * n, 0.0);
+  ce.assign(3 * n, 0.0);
+  if constexpr (tr::vertices > 0) {
+    constexpr int nv = tr::vertices;
+    std::vector<double> planes(static_cast<std::size_t>(nv) * 3 * n);
+    for (std::size_t e = 0; e < n; ++e)
+      for (int v = 0; v < nv; ++v) {
+        double xyz[3];
+        tr::vertex(el[e], v, xyz);
+        for (int c = 0; c < 3; ++c) {
+          planes[(static_cast<std::size_t>(v) * 3 + c) * n + e] = xyz[c];
+          ce[3 * e + c] += xyz[c];
+        }
+      }
+    for (double& c : ce) c /= nv;
+    if (n) BVH_FACADE_CALL(bvh_element_kdops(w.handle(), planes.data(), nv, n, 0.0, BVH_MEM_HOST, kd.data()));
+  } else {
+    for (std::size_t e = 0; e < n; ++e) {
+      const kdop26 k = tr::kdop(el[e]);
+      std::memcpy(&kd[26 * e], k.s, sizeof(k.s));
+      const std::array<double, 3> c = tr::centroid(el[e]);
+      for (int i = 0; i < 3; ++i) ce[3 * e + i] = c[i];
+    }
+  }
+}
+}  // namespace detail
+
+// patch(id, span), patch.hpp:67-78: the union of the elements' k-DOPs and
+// the mean of their centroids
+template <typename T>
+patch<> make_patch(collision_world& w, std::size_t gid, span<const T> el) {
+  std::vector<double> kd, ce;
+  detail::element_bounds(w, el, kd, ce);
+  kdop26 u;
+  for (int i = 0; i < 13; ++i) {
+    u.s[2 * i] = std::numeric_limits<double>::infinity();
+    u.s[2 * i + 1] = -std::numeric_limits<double>::infinity();
+  }
+  std::array<double, 3> c{{0.0, 0.0, 0.0}};
+  for (std::size_t e = 0; e < el.size(); ++e) {
+    for (int i = 0; i < 13; ++i) {
+      const double lo = kd[26 * e + 2 * i], hi = kd[26 * e + 2 * i + 1];
+      u.s[2 * i] = (lo < u.s[2 * i]) ? lo : u.s[2 * i];
+      u.s[2 * i + 1] = (u.s[2 * i + 1] < hi) ? hi : u.s[2 * i + 1];
+    }
+    for (int i = 0; i < 3; ++i) c[i] += ce[3 * e + i];
+  }
+  if (el.size())
+    for (double& x : c) x /= static_cast<double>(el.size());
+  return patch<>(gid, el.size(), u, c);
+}
+
+// max_variant_axis(a, b), narrowphase/kokkos.hpp:119-161: the axis of
+// largest variance of both element sets' centroids
+template <typename T>
+int max_variant_axis(collision_world& w, span<const T> a, span<const T> b) {
+  std::vector<double> ka, ca, kb, cb;
+  detail::element_bounds(w, a, ka, ca);
+  detail::element_bounds(w, b, kb, cb);
+  std::int32_t axis = 0;
+  BVH_FACADE_CALL(bvh_max_variant_axis(w.handle(), ca.data(), a.size(), cb.data(), b.size(), BVH_MEM_HOST, &axis));
+  return axis;
+}
+
+// sort_and_sweep_local(pa, a, pb, b, axis, f), narrowphase/kokkos.hpp:164-248:
+// f(a[i], b[j]) for every pair whose 26-DOPs overlap, for each element of a
+// in the order the reference's sweep finds them (B sorted by its scaled
+// minima on `axis`)
+template <typename T, typename F>
+void sort_and_sweep_local(collision_world& w, const patch<>& pa, span<const T> a, const patch<>& pb, span<const T> b,
+                          int axis, F&& f) {
+  std::vector<double> ka, ca, kb, cb;
+  detail::element_bounds(w, a, ka, ca);
+  detail::element_bounds(w, b, kb, cb);
+  const kdop26 pka = pa.kdop(), pkb = pb.kdop();
+  std::uint64_t np = 0;
+  BVH_FACADE_CALL(bvh_sort_and_sweep(w.handle(), pka.s, ka.data(), nullptr, a.size(), pkb.s, kb.data(), nullptr,
+                                     b.size(), axis, BVH_MEM_HOST, nullptr, 0, &np));
+  std::vector<std::uint64_t> pairs(2 * np);
+  if (np)
+    BVH_FACADE_CALL(bvh_sort_and_sweep(w.handle(), pka.s, ka.data(), nullptr, a.size(), pkb.s, kb.data(), nullptr,
+                                       b.size(), axis, BVH_MEM_HOST, pairs.data(), np, &np));
+  for (std::uint64_t k = 0; k < np; ++k) f(a[pairs[2 * k]], b[pairs[2 * k + 1]]);
 }
 
 }  // namespace bvh

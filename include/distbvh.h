@@ -269,6 +269,12 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 bvh_status bvh_closest_point(bvh_world w, const double *points, uint64_t n, const double *tri,
                              double *closest, uint32_t mem);
 
+/* kdop::from_vertices (kdop.hpp:238-264) for n polytopes: SoA vertex planes
+ * (as for set_entity_data) in, n x 26 k-DOPs (min, max per direction) out,
+ * both in `mem`; the same values the build computes.  For callers that sweep
+ * elements outside an object (sort_and_sweep_local). */
+bvh_status bvh_element_kdops(bvh_world w, const double *verts, uint32_t verts_per_element, uint64_t n,
+                             double epsilon, uint32_t mem, double *kdops);
 /* max_variant_axis(a, b) (narrowphase/kokkos.hpp:119-161): the axis (0, 1, 2)
  * of largest variance of the centroids of A and B together (na x 3 and
  * nb x 3 doubles in `mem`), var = sum(c^2) - sum(c)^2 / (na + nb), the first

@@ -149,8 +149,42 @@ static void patches_case() {
   o.for_each_tree([](const auto& t) { CHECK(t.count() == 4); });
 }
 
+// KokkosTest.cpp:37-139 (max variant axis, sort_and_sweep_local), rank 0
+static void sort_and_sweep_case() {
+  using S = bvh::span<const Element>;
+  bvh::collision_world world(1);
+  auto cube = grid(1, 1, 1, 0);
+  const int dims[3][3] = {{12, 1, 1}, {1, 12, 1}, {1, 1, 12}};
+  for (int a = 0; a < 3; ++a) {
+    auto v = grid(dims[a][0], dims[a][1], dims[a][2], 0);
+    CHECK(bvh::max_variant_axis(world, S(v.data(), v.size()), S(cube.data(), cube.size())) == a);
+  }
+  auto vec = grid(12, 1, 1, 0);
+  const S sv(vec.data(), vec.size()), sc(cube.data(), cube.size());
+  const auto p1 = bvh::make_patch(world, 0, sv);
+  const auto p2 = bvh::make_patch(world, 1, sc);
+  CHECK(p1.size() == 12 && p1.kdop().min(0) == 0.0 && p1.kdop().max(0) == 1.0);
+  int count = 0;
+  std::set<std::size_t> seen;
+  bvh::sort_and_sweep_local(world, p1, sv, p2, sc, 0, [&](const Element& x, const Element& y) {
+    CHECK(y.id == 0);
+    CHECK(x.id < 12);
+    seen.insert(x.id);
+    ++count;
+  });
+  CHECK(count == 12 && seen.size() == 12);
+  count = 0;
+  bvh::sort_and_sweep_local(world, p2, sc, p1, sv, 0, [&](const Element& x, const Element& y) {
+    CHECK(x.id == 0);
+    CHECK(y.id < 12);
+    ++count;
+  });
+  CHECK(count == 12);
+}
+
 int main() {
   try {
+    sort_and_sweep_case();
     narrowphase_functor_case();
     three_objects_builtin_case();
     patches_case();

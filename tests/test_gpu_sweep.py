@@ -166,3 +166,17 @@ def test_sweep_large(torch_cuda):
         got = w.sort_and_sweep(pa, to_cuda(ka), to_cuda(ga.astype(np.int64)), pb, to_cuda(kb),
                                to_cuda(gb.astype(np.int64)), axis)
         assert np.array_equal(got.cpu().numpy().astype(np.uint64), exp)
+
+
+@pytest.mark.parametrize("nv,eps", [(3, 0.0), (3, 0.01), (8, 0.0), (1, 0.0)])
+def test_element_kdops(torch_cuda, nv, eps):
+    """bvh_element_kdops (kdop::from_vertices for a batch) equals the
+    oracle's element k-DOPs bit for bit, host and device memory."""
+    rng = np.random.default_rng(nv)
+    n = 3001
+    verts = rng.normal(size=(nv, 3, n)) * np.array([2.0, 1.0, 0.5])[None, :, None] + 5.0
+    exp, _ = O.elements_poly(verts, eps)
+    with CollisionWorld(1) as w:
+        assert np.array_equal(w.element_kdops(verts, eps), exp)
+        assert np.array_equal(w.element_kdops(to_cuda(verts), eps).cpu().numpy(), exp)
+        assert w.element_kdops(verts[:, :, :0]).shape == (0, 26)
