@@ -181,3 +181,48 @@ def test_graph_outlives_its_world(torch_cuda):
     with pytest.raises(capi.BvhError):
         g.launch()
     g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_concurrent_builds_edge_sizes(torch_cuda, precision):
+    """Objects of 0, 1, 5 and 20,000 elements built concurrently (in the fp32
+    form the first object is empty, so a later build fixes the frame), every
+    pair collided, then refit on moved data: the pairs of serial builds."""
+    torch = torch_cuda
+    rng = np.random.default_rng(7)
+
+    def cloud(n, shift):
+        x = np.zeros((4, n))
+        x[:3] = rng.normal(size=(3, n)) * 0.2 + shift
+        x[3] = 0.03
+        return x
+
+    datas = [cloud(0, 0.0), cloud(1, 0.0), cloud(5, 0.05), cloud(20000, 0.1)]
+    moved = [d.copy() for d in datas]
+    for d in moved:
+        d[0] += 0.01
+    tds = [torch.from_numpy(d).cuda() for d in datas]
+    tms = [torch.from_numpy(d).cuda() for d in moved]
+    res = {}
+    for serial in (True, False):
+        out = []
+        with CollisionWorld(4, precision=precision, serial_builds=serial) as w:
+            objs = [w.create_collision_object() for _ in datas]
+            for o, t in zip(objs, tds):
+                o.set_entity_data_sphere(t)
+            for o in objs:
+                o.init_broadphase()
+            for i in range(len(objs)):
+                for j in range(len(objs)):
+                    if i != j:
+                        objs[i].broadphase(objs[j])
+                        out.append((objs[i].result_count(), objs[i].digest()))
+                objs[i].self_collision()
+                out.append((objs[i].result_count(), objs[i].digest()))
+            for o, t in zip(objs, tms):
+                o.refit(t)
+            for i in range(1, len(objs)):
+                objs[i].broadphase(objs[0 if i > 1 else 2])
+                out.append((objs[i].result_count(), objs[i].digest()))
+        res[serial] = out
+    assert res[True] == res[False]
