@@ -192,7 +192,8 @@ struct bvh_object_s {
   uint64_t npatch_pairs = 0;
   uint64_t nvisits = 0;
   uint64_t ncandidates = 0;
-  uint64_t nrecheck = 0;  // fp32 candidates re-checked in fp64
+  uint64_t nrecheck = 0;      // fp32 candidates re-checked in fp64
+  uint64_t nrecheck_max = 0;  // the fullest re-check stripe
   uint32_t slab_err_bits = 0;  // max fp32 slab error (ulps, float bits) of the queried objects
   const uint64_t* result_ptr = nullptr;  // device pointer to the (sorted) pairs
   bvh_object_s* result_other = nullptr;
@@ -623,8 +624,13 @@ static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps
 }
 
 // Counter words of a query (device, o->count): pairs, node visits, fp32
-// candidates, fp32 re-checks, patch pairs.
-constexpr int kCntPairs = 0, kCntVisits = 1, kCntCand = 2, kCntRecheck = 3, kCntPatch = 4, kCntWords = 5;
+// candidates, fp32 re-checks (sum and largest stripe), patch pairs; from word
+// kCntStripes on, the fp32 re-check list's stripe counters, one 128-byte line
+// each.
+constexpr int kCntPairs = 0, kCntVisits = 1, kCntCand = 2, kCntRecheck = 3, kCntRecheckMax = 4, kCntPatch = 5,
+              kCntWords = 6;
+constexpr uint64_t kCntStripes = 16;
+constexpr uint64_t kCntTotal = kCntStripes + (uint64_t)kRecheckStripes * kRecheckStride;
 
 // Reads the counters of the last query of `o` and the device error words of
 // both objects (one synchronization).  Sets *overflow when a list outgrew
@@ -641,13 +647,14 @@ static bvh_status read_counts(bvh_object_s* o, bvh_object_s* other, bool* overfl
   o->nvisits = w->h_count[kCntVisits];
   o->ncandidates = w->h_count[kCntCand];
   o->nrecheck = w->h_count[kCntRecheck];
+  o->nrecheck_max = w->h_count[kCntRecheckMax];
   o->npatch_pairs = o->q_self ? 0 : w->h_count[kCntPatch];
   uint32_t sw[8];
   std::memcpy(sw, &w->h_count[kCntWords], sizeof(sw));
   o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
   const uint32_t st = sw[0] | sw[4];
   if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
-  *overflow = o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck > o->q_rcap);
+  *overflow = o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck_max > o->q_rcap);
   return BVH_OK;
 }
 
@@ -692,18 +699,18 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
   o->q_narrow = narrow;
   o->q_pcap = o->ppairs.bytes / 8;
   o->q_cap = o->pairs.bytes / 8;
-  o->q_rcap = o->recheck.bytes / 8;
-  HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntWords, 0u, s));
+  o->q_rcap = o->recheck.bytes / 8 / kRecheckStripes;  // per stripe
+  HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntTotal, 0u, s));
   if (!self) {
     w->stage_begin("broadphase");
-    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr};
+    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
     HIP_TRY(launch_patch_broadphase(qv, tv, o->fp32, po, s));
     w->stage_end();
   }
   if (narrow) {
     w->stage_begin("narrowphase");
     PairOut po{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
-               o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntRecheck};
+               o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntStripes, dcount + kCntRecheck};
     HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, w->shape, po, s));
     w->stage_end();
   }
@@ -721,7 +728,7 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   if (w->capturing && (w->cfg.flags & BVH_FLAG_SORT_RESULTS))
     return fail(BVH_E_STATE, "sort_results cannot be captured (it needs the pair count on the host)");
   // buffer sizes: grown here, outside any capture (a captured query uses them as they are)
-  BVH_TRY(o->count.ensure(2 * kCntWords * sizeof(unsigned long long)));
+  BVH_TRY(o->count.ensure(kCntTotal * sizeof(unsigned long long)));
   BVH_TRY(o->ppairs.ensure(std::max<uint64_t>(1024, 4 * w->od) * 8));
   if (narrow) {
     BVH_TRY(ensure_pairs(o, std::max<uint64_t>(1u << 20, 8 * o->in.n)));
@@ -743,7 +750,8 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
     // grow to the exact counts and run again
     if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
     if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
-    if (o->fp32 && o->nrecheck > o->q_rcap) BVH_TRY(o->recheck.ensure((o->nrecheck + (o->nrecheck >> 3) + 1024) * 8));
+    if (o->fp32 && o->nrecheck_max > o->q_rcap)  // every stripe as large as the fullest one needs
+      BVH_TRY(o->recheck.ensure((o->nrecheck_max + (o->nrecheck_max >> 3) + 64) * kRecheckStripes * 8));
   }
   if (narrow && (w->cfg.flags & BVH_FLAG_SORT_RESULTS)) {
     w->stage_begin("sort_results");
@@ -990,7 +998,7 @@ bvh_status bvh_object_release(bvh_object o) {
   o->have_data = false;
   o->have_results = false;
   o->results_pending = false;
-  o->npairs = o->npatch_pairs = o->nvisits = o->ncandidates = o->nrecheck = 0;
+  o->npairs = o->npatch_pairs = o->nvisits = o->ncandidates = o->nrecheck = o->nrecheck_max = 0;
   o->result_ptr = nullptr;
   o->result_other = nullptr;
   o->ref_rp = nullptr;

@@ -216,11 +216,20 @@ struct PairOut {
   unsigned long long* count;
   unsigned long long* visits;      // optional: node visits summed over waves
   unsigned long long* candidates;  // optional: fp32 candidate pairs before the fp64 check
-  // fp32 form: candidates too close to call in fp32, re-checked in fp64
+  // fp32 form: candidates too close to call in fp32, re-checked in fp64.
+  // The list is kRecheckStripes stripes of rcap entries; wave w appends to
+  // stripe w % kRecheckStripes through that stripe's counter,
+  // rcount[stripe * kRecheckStride], so no single counter takes every
+  // flush.  The re-check kernel adds rtotal[0] = sum and rtotal[1] = largest
+  // of the stripe counts (a stripe count above rcap is an overflow).
   uint64_t* recheck;
   uint64_t rcap;
   unsigned long long* rcount;
+  unsigned long long* rtotal;
 };
+
+constexpr uint32_t kRecheckStripes = 64;
+constexpr uint32_t kRecheckStride = 16;  // counter words: one 128-byte line per stripe
 
 // launch shape of the narrowphase (tuned by the autotuner): threads per
 // block (64, 128 or 256), whether blocks are remapped so each XCD walks a

@@ -176,20 +176,21 @@ __device__ __forceinline__ bool box_hits32(const int32_t* q, const int32_t* b) {
 }
 
 // Per-wave LDS staging of the candidates queued for the fp64 re-check: one
-// atomic on the list's counter per flush (mesh self collision queues every
-// pair of elements that share a vertex, so these are many).
+// atomic on the wave's stripe counter per flush (mesh self collision queues
+// every pair of elements that share a vertex, so these are many).
 constexpr uint32_t kRecheckBuf = 128;
 
-__device__ __forceinline__ void recheck_flush(uint64_t* buf, uint32_t& fill, const PairOut& out) {
+__device__ __forceinline__ void recheck_flush(uint64_t* buf, uint32_t& fill, const PairOut& out, uint32_t stripe) {
   if (fill == 0) return;
   __builtin_amdgcn_wave_barrier();
   unsigned long long base = 0;
-  if (lane_id() == 0) base = atomicAdd(out.rcount, (unsigned long long)fill);
+  if (lane_id() == 0) base = atomicAdd(out.rcount + (uint64_t)stripe * kRecheckStride, (unsigned long long)fill);
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
   base = ((unsigned long long)hi << 32) | lo;
+  uint64_t* list = out.recheck + (uint64_t)stripe * out.rcap;
   for (uint32_t i = lane_id(); i < fill; i += kWave)
-    if (base + i < out.rcap) out.recheck[base + i] = buf[i];
+    if (base + i < out.rcap) list[base + i] = buf[i];
   __builtin_amdgcn_wave_barrier();
   fill = 0;
 }
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   uint64_t* rbuf = s_rbuf[threadIdx.x / kWave];
   uint32_t* stk = s_stack[threadIdx.x / kWave];
   uint32_t fill = 0, rfill = 0;
+  const uint32_t rstripe = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kRecheckStripes;
   const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     const uint64_t mu = __ballot(unsure);
     if (mu != 0) {
       const uint32_t cnt = (uint32_t)__popcll(mu);
-      if (rfill + cnt > kRecheckBuf) recheck_flush(rbuf, rfill, out);
+      if (rfill + cnt > kRecheckBuf) recheck_flush(rbuf, rfill, out, rstripe);
       if (unsure) rbuf[rfill + (uint32_t)__popcll(mu & lanemask_lt())] = key;
       rfill += cnt;
     }
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     }
   }
   wave_flush(buf, fill, out);
-  recheck_flush(rbuf, rfill, out);
+  recheck_flush(rbuf, rfill, out, rstripe);
   if (lane == 0) {
     if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
     if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
@@ -648,23 +650,37 @@ __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int x
 
 // fp64 re-check of the fp32 candidates that the inner bound left open: both
 // elements' k-DOPs recomputed from element data, the pair emitted on overlap.
-// The list length is read on the device, so no host round trip separates the
-// two kernels.
+// The stripe lengths are read on the device, so no host round trip separates
+// the two kernels.  The grid is kRecheckStripes x (blocks per stripe); block
+// b works on stripe b % kRecheckStripes.  Hits are staged per wave in LDS
+// (one atomic on the pair counter per flush, not per wave that hits).  Block
+// 0 also adds the stripe counts' sum and maximum to rtotal for the host.
 // KIND / NV: both objects' element type (the common case, with registers
 // sized for it); kGeneric: any mix, chosen at run time.
 constexpr int kGeneric = -1;
 
 template <int KIND, int NV>
 __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut out) {
-  const unsigned long long cnt0 = *out.rcount;
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  uint64_t* buf = s_buf[threadIdx.x / kWave];
+  uint32_t fill = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kRecheckStripes) {
+    const unsigned long long c = out.rcount[(uint64_t)threadIdx.x * kRecheckStride];
+    atomicAdd(out.rtotal, c);
+    atomicMax(out.rtotal + 1, c);
+  }
+  const uint32_t stripe = blockIdx.x % kRecheckStripes;
+  const uint32_t per = gridDim.x / kRecheckStripes;
+  const unsigned long long cnt0 = out.rcount[(uint64_t)stripe * kRecheckStride];
   const uint64_t cnt = cnt0 < out.rcap ? cnt0 : out.rcap;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < cnt; base += stride) {
+  const uint64_t* list = out.recheck + (uint64_t)stripe * out.rcap;
+  const uint64_t stride = (uint64_t)per * blockDim.x;
+  for (uint64_t base = (uint64_t)(blockIdx.x / kRecheckStripes) * blockDim.x; base < cnt; base += stride) {
     const uint64_t i = base + threadIdx.x;
     bool hit = false;
     uint64_t key = 0;
     if (i < cnt) {
-      key = out.recheck[i];
+      key = list[i];
       double ka[kSlabs];
       if constexpr (KIND == kGeneric) {
         element_kdop_any(q.elem, (uint32_t)(key >> 32), ka);
@@ -674,8 +690,9 @@ __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut
         hit = element_overlap<KIND, NV>(t.elem, (uint32_t)key, ka);
       }
     }
-    emit(out, hit, key);
+    wave_emit(buf, fill, out, hit, key);
   }
+  wave_flush(buf, fill, out);
 }
 
 static void launch_recheck(const TreeView& q, const TreeView& t, const PairOut& out, unsigned blocks, hipStream_t s) {
@@ -818,16 +835,17 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   if (fp32) {
     if (!q.rec32 || !q.rootbox32 || !q.delta || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin)
       return hipErrorInvalidValue;
-    if (!out.recheck || !out.rcount) return hipErrorInvalidValue;
+    if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
     // one record per round trip: it arrives in scalar registers (the batch
     // applies to the fp64 walk)
     if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     // in self mode both keys index the one object (q == t)
-    uint64_t rblocks = (out.rcap + 255) / 256;
-    if (rblocks > 8192) rblocks = 8192;
-    launch_recheck(q, t, out, (unsigned)rblocks, s);
+    uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
+    if (per > 8192 / kRecheckStripes) per = 8192 / kRecheckStripes;
+    if (per == 0) per = 1;
+    launch_recheck(q, t, out, (unsigned)(per * kRecheckStripes), s);
     return hipGetLastError();
   }
   if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
