@@ -331,7 +331,7 @@ def bench_batch(args, torch, stream):
         o.self_collision()
 
     eager()
-    tune = o.autotune()
+    tune = pick_launch(args, world, o.autotune)
     eager()
     npairs_eager = o.result_count()
     digest_eager = o.digest()
@@ -360,8 +360,8 @@ def bench_batch(args, torch, stream):
                        "precision": prec, "od": 4096,
                        "autotuned": {"batch": tune["batch"], "block_size": tune["block_size"],
                                      "xcd_remap": tune["xcd_remap"],
-                                     "query_ms": round(tune["ms"], 4),
-                                     "tried_ms": [round(t, 4) for t in tune["tried_ms"]]},
+                                     "query_ms": round(tune["ms"], 4) if "ms" in tune else None,
+                                     "tried_ms": [round(t, 4) for t in tune.get("tried_ms", [])]},
                        "pairs": int(npairs), "pairs_per_s": round(npairs / (ms_graph * 1e-3), 1),
                        "candidate_pairs": int(st["candidate_pairs"]), "rechecked_pairs": int(st["rechecked_pairs"]),
                        "node_visits_per_wave": round(st["node_visits"] / ((n + 63) // 64), 1),
