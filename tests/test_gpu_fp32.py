@@ -129,3 +129,30 @@ def test_fp32_full_size_matches_fp64(torch_cuda):
     assert np.array_equal(recs, recs64)
     assert np.array_equal(pp, pp64)
     assert stats["candidate_pairs"] >= len(recs)
+
+
+def test_fp32_mesh_self_collision_recheck_stripes(torch_cuda):
+    """Mesh self collision re-checks every pair of triangles that share a
+    vertex (about six per triangle), more than the re-check list holds at
+    first (max(65536, n) entries over 64 stripes): the query overflows, grows
+    every stripe to the fullest one's count and runs again.  Pairs must equal
+    the oracle's brute force on both runs and the fp64 form's."""
+    a, _ = W.wavy_sheets(128)
+    kd, _ = O.elements_poly(a)
+    brute = sort_pairs(O.brute_self(kd))
+    with CollisionWorld(16, leaf_size=1, precision="fp32") as w:
+        o = gpu_object(w, "poly", a, device=True)
+        o.self_collision()
+        first = sort_pairs(o.pairs())
+        stats = o.result_stats()
+        o.self_collision()  # buffers already large enough
+        second = sort_pairs(o.pairs())
+    with CollisionWorld(16, leaf_size=1, precision="fp64") as w:
+        o = gpu_object(w, "poly", a, device=True)
+        o.self_collision()
+        got64 = sort_pairs(o.pairs())
+    assert len(brute) > 0
+    assert stats["rechecked_pairs"] > max(65536, kd.shape[0])
+    assert np.array_equal(first, brute)
+    assert np.array_equal(second, brute)
+    assert np.array_equal(got64, brute)
