@@ -90,8 +90,13 @@ def measured_traffic(stage, workload, fp32, launch=None):
     return None
 
 
-def cpu_baseline(a, b, od, leaf):
-    """The CPU oracle (restated reference path) on the full workload."""
+def cpu_baseline(a, b, od, leaf, runs=5, gpu_records=None):
+    """The CPU oracle (the reference path restated in C) on the full
+    workload, timed as the median of `runs` whole steps (element k-DOPs,
+    bounds, Morton codes, 1-bit LSD radix sort, cluster splits, patches,
+    patch tree, per-patch-pair narrowphase) on all the cores this process may
+    use (at most 16, the host share of one GPU) and on one thread.  The
+    records of the first run are compared with the GPU's, row for row."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import build as oracle_build
     import oracle as O
@@ -99,26 +104,91 @@ def cpu_baseline(a, b, od, leaf):
     if not os.path.exists(O.LIB_PATH):
         oracle_build.build()
     try:
-        nthreads = min(len(os.sched_getaffinity(0)), 16)
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        nthreads = min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    ka, ca = O.elements_poly(a)
-    kb, cb = O.elements_poly(b)
-    oa = O.Object(ka, ca, od=od, morton_bits=63, leaf_size=leaf, nthreads=nthreads)
-    ob = O.Object(kb, cb, od=od, morton_bits=63, leaf_size=leaf, nthreads=nthreads)
-    recs = oa.collide(ob)
-    t1 = time.perf_counter()
+        affinity = os.cpu_count() or 1
+    nall = max(1, min(affinity, 16))
+
+    def step(nthreads):
+        t0 = time.perf_counter()
+        ka, ca = O.elements_poly(a)
+        kb, cb = O.elements_poly(b)
+        oa = O.Object(ka, ca, od=od, morton_bits=63, leaf_size=leaf, nthreads=nthreads)
+        ob = O.Object(kb, cb, od=od, morton_bits=63, leaf_size=leaf, nthreads=nthreads)
+        recs = oa.collide(ob)
+        return (time.perf_counter() - t0) * 1e3, recs
+
+    times_all, times_one = [], []
+    recs = None
+    for r in range(runs):
+        ms, rr = step(nall)
+        times_all.append(ms)
+        if recs is None:
+            recs = rr
+    for r in range(runs):
+        times_one.append(step(1)[0])
+    equal = None
+    if gpu_records is not None:
+        g = np.asarray(gpu_records, np.uint64).reshape(-1, 4)
+        g = g[np.lexsort((g[:, 2], g[:, 0], g[:, 3], g[:, 1]))]
+        c = recs[np.lexsort((recs[:, 2], recs[:, 0], recs[:, 3], recs[:, 1]))]
+        equal = bool(g.shape == c.shape and np.array_equal(g, c))
+    med_all = float(np.median(times_all))
+    med_one = float(np.median(times_one))
     return {
-        "value": round((t1 - t0) * 1e3, 3),
+        "value": round(med_all, 3),
         "unit": "ms/step",
-        "cores": nthreads,
+        "cores": nall,
         "kind": "port",
-        "sample": "one full step of the same workload (2 x 2,097,152 triangles); element k-DOPs "
-                  "and the per-patch-pair narrowphase on %d threads, bounds / Morton / 1-bit radix "
-                  "sort / cluster / patches serial as in the reference" % nthreads,
-        "pairs": int(len(recs)),
+        "sample": "the full step of the same workload (2 x 2,097,152 triangles), median of %d runs on %d "
+                  "threads (element k-DOPs and the per-patch-pair narrowphase in parallel; bounds, Morton, "
+                  "1-bit radix sort, cluster splits and patches serial as in the reference)" % (runs, nall),
+        "single_thread_ms": round(med_one, 3),
+        "all_core_ms": round(med_all, 3),
+        "runs_ms": {"all_core": [round(t, 1) for t in times_all], "single_thread": [round(t, 1) for t in times_one]},
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": affinity,
+        "records": int(len(recs)),
+        "records_equal_gpu": equal,
     }
+
+
+def serving_leg(args, check_oracle=True):
+    """The serving form: an open-loop Poisson stream of mesh-pair requests
+    (log-uniform 10k-2M triangles per object) served by continuous batching;
+    p50 / p99 latency from arrival to pairs on the host.  Every distinct
+    request (mesh pair, motion variant) of the stream is checked against the
+    CPU oracle (the restated reference pipeline, fp64) by pair count and
+    digest."""
+    from distbvh_amd import serve as S
+
+    prec = "fp64" if args.precision == "both" else args.precision
+    r = S.serve(rate=args.rate, duration=args.duration, precision=prec, max_batch=args.max_batch, record=True)
+    out = {k: r[k] for k in ("requests", "rate_req_s", "max_batch", "refits", "rebuilds", "evictions", "deferred",
+                             "steps", "mean_step_requests", "max_step_requests", "mesh_triangles")}
+    out.update({"p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3), "mean_ms": round(r["mean_ms"], 3),
+                "max_ms": round(r["max_ms"], 3), "pairs_per_s": round(r["pairs_per_s"], 1), "pairs": r["pairs"],
+                "wall_s": round(r["wall_s"], 3), "precision": prec})
+    if check_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        pool = S.mesh_pool(8, 1e4, 2e6, 0xC5)
+        ref = {}
+        equal = 0
+        for i, (m, (cnt, dig)) in enumerate(zip(r["picks"], r["per_request"])):
+            key = (m, i % 4)
+            if key not in ref:
+                a, b = pool[m]
+                mb = S.moved(b, 0xC5 * 1000 + 97 * m + key[1])
+                ka, ca = O.elements_poly(a)
+                kb, cb = O.elements_poly(mb)
+                recs = O.Object(ka, ca, od=64, nthreads=16).collide(O.Object(kb, cb, od=64, nthreads=16))
+                ref[key] = (len(recs), S._pair_digest(recs[:, [1, 3]]))
+            equal += int(ref[key] == (cnt, dig))
+        out["requests_equal_fp64_oracle"] = equal
+        out["distinct_requests_checked"] = len(ref)
+    return out
 
 
 def stream_copy_gbs(torch, mib=1024, reps=5):
@@ -257,9 +327,17 @@ def bench_sheets(args, torch, stream):
                                       (lambda k: measured_traffic(k, wl, fp32, tune)) if wl else None,
                                       dom="narrowphase")
         roof["kernel_name"] = stage_kernel(roof["kernel"], fp32, tune)
+        # after the timed steps: the eager step once more, then its 32-byte
+        # records materialised and copied to the host (k_records + copy)
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        recs = oa.results()
+        records_ms = (time.perf_counter() - t0) * 1e3
         out[prec] = {"ms_per_step": ms, "stages": stages, "npairs": int(npairs), "stats": qstats,
                      "roofline": roof, "step_hbm_frac": step_frac, "digest": digest, "graph": graph_ms,
-                     "launch": {k: tune[k] for k in ("batch", "block_size", "xcd_remap")}}
+                     "launch": {k: tune[k] for k in ("batch", "block_size", "xcd_remap")},
+                     "records": recs, "records_ms": records_ms}
         world.close()
     return a, b, n, out
 
@@ -307,66 +385,128 @@ def bench_spheres_self(args, torch, stream):
     return n, out
 
 
+def batch_oracle_sample(a, counts, pairs, lattice=16, nsample=8, seed=7):
+    """FP64 oracle on a sample of the batch: for `nsample` meshes, the
+    brute-force 26-DOP pairs between the mesh and every other mesh of its
+    3x3x3 lattice neighbourhood (no other mesh can reach it), against the
+    GPU's cross-mesh pairs that involve the mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    mesh = np.repeat(np.arange(len(counts)), counts)
+    rng = np.random.default_rng(seed)
+    # meshes of at most 8,192 triangles, so the brute force stays within seconds
+    small = np.flatnonzero(np.asarray(counts) <= 8192)
+    pick = rng.choice(small, size=min(nsample, len(small)), replace=False)
+    got_all = np.asarray(pairs, np.uint64)
+    ok = 0
+    npairs = 0
+    for m in pick:
+        ci, cj, cl = m % lattice, (m // lattice) % lattice, m // (lattice * lattice)
+        nb = []
+        for k in range(len(counts)):
+            if k == m:
+                continue
+            ki, kj, kl = k % lattice, (k // lattice) % lattice, k // (lattice * lattice)
+            if abs(ki - ci) <= 1 and abs(kj - cj) <= 1 and abs(kl - cl) <= 1:
+                nb.append(np.arange(starts[k], starts[k + 1]))
+        idx_m = np.arange(starts[m], starts[m + 1])
+        idx_n = np.concatenate(nb) if nb else np.zeros(0, np.int64)
+        km, _ = O.elements_poly(a[:, :, idx_m])
+        kn, _ = O.elements_poly(a[:, :, idx_n])
+        bp = O.brute_pairs(km, kn, nthreads=16) if len(idx_n) else np.zeros((0, 2), np.uint64)
+        gi = idx_m[bp[:, 0].astype(np.int64)].astype(np.uint64)
+        gj = idx_n[bp[:, 1].astype(np.int64)].astype(np.uint64)
+        want = np.sort(np.minimum(gi, gj) << np.uint64(32) | np.maximum(gi, gj))
+        sel = (mesh[(got_all >> np.uint64(32)).astype(np.int64)] == m) | \
+              (mesh[(got_all & np.uint64(0xffffffff)).astype(np.int64)] == m)
+        got = np.sort(got_all[sel])
+        ok += int(np.array_equal(got, want))
+        npairs += len(want)
+    return {"meshes": int(len(pick)), "meshes_equal": ok, "pairs_checked": int(npairs)}
+
+
 def bench_batch(args, torch, stream):
     """Batching form: a batch of 4,096 height-field meshes (lognormal sizes,
-    ~19M triangles) in one object, rebuilt and self-collided per step; the
-    step is captured once into a HIP graph and replayed, after the autotuner
-    picked the launch shape."""
-    from distbvh_amd import CollisionWorld
+    ~19M triangles) in one object with one patch per mesh (the meshes are
+    the workload's patches), rebuilt per step and collided patch against
+    patch (pairs between different meshes); the step is captured once into a
+    HIP graph and replayed, after the autotuner picked the launch shape."""
+    from distbvh_amd import CollisionWorld, GIVEN
     from distbvh_amd import workloads as W
 
     a, counts = W.patch_batch()
     n = a.shape[2]
     ta = torch.from_numpy(a).cuda()
-    del a
-    # fp32-outward k-DOPs unless a precision is named (--precision fp64)
-    prec = args.precisions[0] if args.precision_given else "fp32"
-    world = CollisionWorld(4096, morton_bits=63, leaf_size=args.leaf, timing=False, stream=stream.cuda_stream,
-                           precision=prec)
-    o = world.create_collision_object()
+    offsets = np.cumsum(counts)[:-1]
+    # fp32-outward k-DOPs unless a precision is named; the other form runs too
+    prec = args.precisions[0] if args.precision_given and args.precision != "both" else "fp32"
+    forms = {}
+    for p in ([prec, "fp64" if prec == "fp32" else "fp32"]):
+        world = CollisionWorld(4096, morton_bits=63, timing=False, stream=stream.cuda_stream, precision=p)
+        o = world.create_collision_object()
+        o.set_patch_offsets(offsets)
 
-    def eager():
-        o.set_entity_data_poly(ta)
-        o.init_broadphase()
-        o.self_collision()
+        def eager():
+            o.set_entity_data_poly(ta, split_algorithm=GIVEN)
+            o.init_broadphase()
+            o.self_collision_cross_patch()
 
-    eager()
-    tune = pick_launch(args, world, o.autotune)
-    eager()
-    npairs_eager = o.result_count()
-    digest_eager = o.digest()
-    # eager timing
-    ms_eager, _, _, _, _ = run_steps(world, lambda: (eager(), o.result_count())[1], args.steps, args.warmup,
-                                     stream, torch)
-    with world.capture() as g:
         eager()
+        tune = pick_launch(args, world, o.autotune)
+        eager()
+        npairs_eager = o.result_count()
+        digest_eager = o.digest()
+        ms_eager, _, _, _, _ = run_steps(world, lambda: (eager(), o.result_count())[1], args.steps, args.warmup,
+                                         stream, torch)
+        with world.capture() as g:
+            eager()
 
-    def replay():
-        g.launch()
-        return o.result_count()
+        def replay():
+            g.launch()
+            return o.result_count()
 
-    ms_graph, _, _, _, npairs = run_steps(world, replay, args.steps, args.warmup, stream, torch)
-    ok = npairs == npairs_eager and o.digest() == digest_eager
-    st = o.result_stats()
-    g.close()
-    world.close()
-    return {"metric": "ms/step batch build + self collision, 4096 meshes (%d triangles), one HIP graph" % n,
+        ms_graph, _, _, _, npairs = run_steps(world, replay, args.steps, args.warmup, stream, torch)
+        ok = npairs == npairs_eager and o.digest() == digest_eager
+        st = o.result_stats()
+        forms[p] = {"ms_graph": ms_graph, "ms_eager": ms_eager, "npairs": int(npairs), "digest": o.digest(),
+                    "graph_ok": bool(ok), "stats": st, "tune": tune}
+        if p == "fp64" and not args.no_cpu:
+            pr = o.pairs()
+            forms[p]["pairs_host"] = (pr[:, 0] << np.uint64(32)) | pr[:, 1]
+        g.close()
+        world.close()
+    f = forms[prec]
+    f64 = forms["fp64"]
+    sample = batch_oracle_sample(a, counts, f64["pairs_host"]) if not args.no_cpu else None
+    ms_graph = f["ms_graph"]
+    st = f["stats"]
+    tune = f["tune"]
+    return {"metric": "ms/step batch build + patch-vs-patch collision, 4096 meshes (%d triangles), one HIP graph" % n,
             "value": round(ms_graph, 4), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_graph, 4), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if prec == "fp32" else "f64",
             "data": "synthetic",
             "config": {"workload": "4,096 height-field meshes on a 16^3 lattice, lognormal sizes (median "
-                                   "2,048, clipped to [64, 65,536]), 10%% neighbour overlap; %d triangles" % n,
+                                   "2,048, clipped to [64, 65,536]), 10%% neighbour overlap; %d triangles; one "
+                                   "patch per mesh (BVH_SPLIT_GIVEN); pairs between different meshes "
+                                   "(bvh_self_collision_cross_patch)" % n,
                        "precision": prec, "od": 4096,
                        "autotuned": {"batch": tune["batch"], "block_size": tune["block_size"],
                                      "xcd_remap": tune["xcd_remap"],
                                      "query_ms": round(tune["ms"], 4) if "ms" in tune else None,
                                      "tried_ms": [round(t, 4) for t in tune.get("tried_ms", [])]},
-                       "pairs": int(npairs), "pairs_per_s": round(npairs / (ms_graph * 1e-3), 1),
+                       "pairs": f["npairs"], "pairs_per_s": round(f["npairs"] / (ms_graph * 1e-3), 1),
                        "candidate_pairs": int(st["candidate_pairs"]), "rechecked_pairs": int(st["rechecked_pairs"]),
                        "node_visits_per_wave": round(st["node_visits"] / ((n + 63) // 64), 1),
-                       "eager_ms_per_step": round(ms_eager, 4), "graph_pairs_equal_eager": bool(ok),
+                       "eager_ms_per_step": round(f["ms_eager"], 4), "graph_pairs_equal_eager": f["graph_ok"],
                        "parallelism": "single GPU"},
+            "forms": {p: {"graph_ms": round(v["ms_graph"], 4), "eager_ms": round(v["ms_eager"], 4),
+                          "pairs": v["npairs"]} for p, v in forms.items()},
+            "fp32_pairs_equal_fp64": bool(forms["fp32"]["npairs"] == f64["npairs"] and
+                                          forms["fp32"]["digest"] == f64["digest"]),
+            "fp64_oracle_sample": sample,
             "cpu_baseline": None}
 
 
@@ -381,8 +521,10 @@ def main():
                          "graph per step over a batch of 4,096 meshes; serving: Poisson request stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
     ap.add_argument("--duration", type=float, default=2.0, help="serving: seconds of arrivals")
-    ap.add_argument("--slots", type=int, default=4,
-                    help="serving: concurrent slots (host threads with a world and stream each)")
+    ap.add_argument("--max-batch", type=int, default=16,
+                    help="serving: most requests coalesced into one step (1: each request alone)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="sheets: leave the serving leg out of the line")
     ap.add_argument("--nq", type=int, default=1024, help="quads per sheet side")
     ap.add_argument("--spheres", type=int, default=16777216, help="spheres of the spheres-self workload")
     ap.add_argument("--od", type=int, default=256)
@@ -390,12 +532,13 @@ def main():
                     help="leaf size of the production workload (4), used by the CPU oracle's element "
                          "trees; the GPU trees hold one element per leaf and test leaves from their "
                          "parents' records, so it does not change their work")
-    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both",
                     help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
                          "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
     ap.add_argument("--split", choices=["clustering", "geom_axis", "ml_geom_axis"], default="clustering",
                     help="sheets: the split_algorithm of set_entity_data (patches)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="runs of each CPU baseline variant (median)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape instead of autotuning it before timing "
                          "(profiling runs: no tuner launches in the trace)")
@@ -427,22 +570,20 @@ def main():
         del src, dst
     stream = torch.cuda.current_stream()
     if args.workload == "serving":
-        from distbvh_amd import serve as S
-        prec = args.precisions[0]
-        r = S.serve(rate=args.rate, duration=args.duration, precision=prec, slots=args.slots)
+        r = serving_leg(args, check_oracle=not args.no_cpu)
         out = {"metric": "serving p99 latency (arrival -> pairs on host), Poisson %.0f req/s, mesh pairs "
                          "10k-2M triangles" % args.rate,
-               "value": round(r["p99_ms"], 3), "unit": "ms", "n_gpus": 1, "steps": r["requests"], "warmup": 0,
-               "ms_per_step": round(r["mean_ms"], 3), "higher_is_better": False, "scaling": "weak",
-               "vs_baseline": None, "dtype": "f32+f64" if prec == "fp32" else "f64", "data": "synthetic",
+               "value": r["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": r["requests"], "warmup": 0,
+               "ms_per_step": r["mean_ms"], "higher_is_better": False, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32+f64" if r["precision"] == "fp32" else "f64", "data": "synthetic",
                "config": {"workload": "open-loop Poisson stream; each request a wavy-sheet pair (log-uniform "
-                                      "10k-2M triangles per object, 8 pairs resident) with B rigidly moved + "
-                                      "0.1%% jitter, uploaded from pinned host memory; refit B, rebuild when "
-                                      "the root 26-DOP grew > 1.2x; pairs copied to the host; %d slots "
-                                      "(host threads with a world and stream each) serve requests at "
-                                      "once" % r["slots"],
-                          "slots": r["slots"],
-                          "precision": prec, "parallelism": "single GPU"},
+                                      "10k-2M triangles per object, 8 pairs, 4 motion variants) with B rigidly "
+                                      "moved + 0.1%% jitter, uploaded from pinned host memory; refit B, rebuild "
+                                      "when the root 26-DOP grew > 1.2x; continuous batching: every waiting "
+                                      "request (up to %d, one per mesh pair, within one 200 GB budget) joins "
+                                      "the next coalesced step (one narrowphase launch); pairs copied to the "
+                                      "host" % args.max_batch,
+                          "precision": r["precision"], "parallelism": "single GPU"},
                "serving": r, "cpu_baseline": None}
         print(json.dumps(out), flush=True)
         return
@@ -481,7 +622,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload,
                    "precision": "fp32 tree, exact fp64 re-check" if fp32 else "fp64",
-                   "elements_per_object": n, "od": args.od, "leaf_size": args.leaf,
+                   "elements_per_object": n, "od": args.od,
+                   "leaf_size": 1,
+                   "leaf_size_note": "the GPU trees hold one element per leaf and test both children of a "
+                                     "record at once; the workload's leaf size %d is the CPU oracle's "
+                                     "(cpu_oracle_leaf_size)" % args.leaf,
+                   "cpu_oracle_leaf_size": args.leaf,
+                   "timed_step": "both builds, patch broadphase and element narrowphase, ending when the pair "
+                                 "count is on the host; the 32-byte records are materialised after the timed "
+                                 "region (records_ms)",
                    "pairs": first["npairs"],
                    "patch_pairs": int(st["patch_pairs"]),
                    "candidate_pairs": int(st["candidate_pairs"]),
@@ -496,6 +645,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in first["stages"].items()},
         "stream_copy_gbs": stream_copy_gbs(torch),
         "graph_form": first.get("graph"),
+        "records_ms": round(first["records_ms"], 3) if "records_ms" in first else None,
         "step_hbm_frac": first["step_hbm_frac"],
         "roofline": first["roofline"],
     }
@@ -516,11 +666,14 @@ def main():
             "fp32_max_slab_error_ulps": r32["stats"]["max_slab_error_ulps"],
         }
     if not args.no_cpu and args.workload == "sheets" and args.split == "clustering":
-        out["cpu_baseline"] = cpu_baseline(a, b, args.od, args.leaf)
-        cpu_pairs = out["cpu_baseline"].pop("pairs")
-        out["config"]["cpu_pairs_equal_count"] = bool(cpu_pairs == first["npairs"])
+        out["cpu_baseline"] = cpu_baseline(a, b, args.od, args.leaf, runs=args.cpu_runs,
+                                           gpu_records=res["fp64"]["records"] if "fp64" in res else None)
+        out["config"]["records_equal_cpu_oracle"] = out["cpu_baseline"]["records_equal_gpu"]
+        out["gpu_fraction_of_cpu_time"] = round(first["ms_per_step"] / out["cpu_baseline"]["value"], 5)
     else:
         out["cpu_baseline"] = None
+    if args.workload == "sheets" and not args.no_serving:
+        out["serving"] = serving_leg(args, check_oracle=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

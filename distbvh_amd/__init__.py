@@ -6,4 +6,4 @@ its C ABI (include/distbvh.h); `workloads` generates the seeded synthetic
 inputs.
 """
 from . import capi  # noqa: F401
-from .collision import CollisionWorld, CollisionObject, CLUSTERING, GEOM_AXIS, ML_GEOM_AXIS  # noqa: F401
+from .collision import CollisionWorld, CollisionObject, CLUSTERING, GEOM_AXIS, ML_GEOM_AXIS, GIVEN  # noqa: F401

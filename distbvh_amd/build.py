@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["sort.hip", "build.hip", "query.hip", "split.hip", "primitive.hip", "sweep.hip", "capi.hip"]
+SOURCES = ["sort.hip", "build.hip", "query.hip", "split.hip", "primitive.hip", "sweep.hip", "wire.hip", "capi.hip"]
 LIB = os.path.join(HERE, "libdistbvh.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -12,12 +12,13 @@ LIB_PATH = os.path.join(HERE, "libdistbvh.so")
 
 BVH_OK = 0
 BVH_E_OVERFLOW = 3
-BVH_SPLIT_GEOM_AXIS, BVH_SPLIT_ML_GEOM_AXIS, BVH_SPLIT_CLUSTERING = 0, 1, 2
+BVH_SPLIT_GEOM_AXIS, BVH_SPLIT_ML_GEOM_AXIS, BVH_SPLIT_CLUSTERING, BVH_SPLIT_GIVEN = 0, 1, 2, 3
 BVH_MEM_HOST, BVH_MEM_DEVICE = 0, 1
 BVH_PREC_FP64, BVH_PREC_FP32_OUTWARD = 0, 1
 BVH_FLAG_SORT_RESULTS = 0x1
 BVH_FLAG_TIMING = 0x2
 BVH_FLAG_SERIAL_BUILDS = 0x4
+BVH_ELEM_POLY, BVH_ELEM_SPHERE, BVH_ELEM_SNAPSHOT = 0, 1, 2
 
 
 class BvhError(RuntimeError):
@@ -73,12 +74,14 @@ SIGNATURES = {
     "bvh_start_iteration": [_vp],
     "bvh_finish_iteration": [_vp],
     "bvh_synchronize": [_vp],
+    "bvh_world_stream": [_vp, C.POINTER(_vp)],
     "bvh_stage_times": [_vp, C.POINTER(C.c_char_p), C.POINTER(_d), _sz, C.POINTER(_sz)],
     "bvh_object_id": [_vp, C.POINTER(_sz)],
     "bvh_set_entity_data_poly": [_vp, _vp, _u32, _u64, _vp, _d, _u32, _u32],
     "bvh_set_entity_data_sphere": [_vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_set_entity_data_snapshot": [_vp, _vp, _vp, _u64, _vp, _u32, _u32],
     "bvh_refit": [_vp, _vp, _u32],
+    "bvh_object_set_patch_offsets": [_vp, _vp, _sz],
     "bvh_object_device_bytes": [_vp, C.POINTER(_u64)],
     "bvh_object_root_kdop": [_vp, _vp],
     "bvh_autotune": [_vp, _vp, _vp],
@@ -91,7 +94,9 @@ SIGNATURES = {
     "bvh_init_broadphase": [_vp],
     "bvh_broadphase": [_vp, _vp],
     "bvh_broadphase_patches": [_vp, _vp],
+    "bvh_broadphase_batch": [_vp, _vp, _u64],
     "bvh_self_collision": [_vp],
+    "bvh_self_collision_cross_patch": [_vp],
     "bvh_end_phase": [_vp],
     "bvh_result_count": [_vp, C.POINTER(_u64)],
     "bvh_result_records": [_vp, _vp, _u64, _u32],
@@ -113,6 +118,16 @@ SIGNATURES = {
     "bvh_sort_u32": [_vp, _vp, _vp, _u64],
     "bvh_exclusive_scan_u32": [_vp, _vp, _u64],
     "bvh_closest_point": [_vp, _vp, _u64, _vp, _vp, _u32],
+    "bvh_triangle_pairs_intersect": [_vp, _vp, _vp, _u64, _u32, _vp],
+    "bvh_sphere_pairs_intersect": [_vp, _vp, _vp, _u64, _u32, _vp],
+    "bvh_result_filter_exact": [_vp, C.POINTER(_u64)],
+    "bvh_pack_patches": [_vp, _vp, _u64, C.POINTER(_u64), _u32],
+    "bvh_unpack_patches": [_vp, _vp, _u64, _u32, _vp, _u64, C.POINTER(_u64)],
+    "bvh_element_record_bytes": [_vp, C.POINTER(_u64)],
+    "bvh_pack_patch_elements": [_vp, _vp, _u64, _vp, _u64, C.POINTER(_u64), _u32],
+    "bvh_unpack_elements": [_vp, _vp, _u64, _u32, _u32, _vp, _vp, _vp, _u32],
+    "bvh_pack_results": [_vp, _vp, _u64, C.POINTER(_u64), _u32],
+    "bvh_unpack_results": [_vp, _vp, _u64, _u32, _vp, _u64, C.POINTER(_u64)],
     "bvh_object_release": [_vp],
     "bvh_element_kdops": [_vp, _vp, _u32, _u64, C.c_double, _u32, _vp],
     "bvh_max_variant_axis": [_vp, _vp, _u64, _vp, _u64, _u32, _vp],

@@ -49,14 +49,17 @@ bvh_status fail(bvh_status s, const std::string& msg) {
   } while (0)
 
 // grow-only device buffer
-// Every device (re)allocation bumps the generation; a graph captured at an
-// older generation may hold freed pointers and is refused.
+// Every device (re)allocation bumps the generation of the world that owns
+// the buffer; a graph captured at an older generation of its world may hold
+// freed pointers and is refused.  Buffers of no world (temporaries of one
+// call) count on a process-wide generation that no graph reads.
 std::atomic<uint64_t> g_alloc_gen{0};
 thread_local bool t_capturing = false;
 
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  std::atomic<uint64_t>* gen = &g_alloc_gen;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -67,7 +70,7 @@ struct DevBuf {
   // become stale
   void release() {
     if (!p) return;
-    g_alloc_gen.fetch_add(1);
+    gen->fetch_add(1);
     (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -76,7 +79,7 @@ struct DevBuf {
     if (want <= bytes && p) return BVH_OK;
     if (t_capturing)
       return fail(BVH_E_STATE, "a device buffer would grow inside a capture: run the step once before capturing");
-    g_alloc_gen.fetch_add(1);
+    gen->fetch_add(1);
     if (p) {
       (void)hipFree(p);
       p = nullptr;
@@ -107,6 +110,19 @@ struct Stage {
 
 struct bvh_object_s;
 
+// The host-side state of a query launched inside a capture: which partner
+// and kind, the list capacities its kernels were launched with and where its
+// pairs land.  A replay of the graph restores it in the object, so results
+// read after the replay describe the captured query even if the object ran
+// other queries eagerly since the capture.
+struct CapturedQuery {
+  bvh_object_s* o;
+  bvh_object_s* other;
+  bool self, narrow, self_result, cross;
+  uint64_t pcap, cap, rcap;
+  const uint64_t* result_ptr;
+};
+
 struct bvh_world_s {
   size_t od = 1;
   bvh_world_config cfg{};
@@ -126,12 +142,26 @@ struct bvh_world_s {
   DevBuf frame;
   bool frame_fixed = false;  // a build that sets the frame has been enqueued in stream order
   LaunchShape shape;  // narrowphase launch shape (autotuned)
-  // stream capture (batching form): objects whose queries were captured
+  // stream capture (batching form): the queries captured so far, with the
+  // state a replay restores in their objects
   bool capturing = false;
-  std::vector<bvh_object_s*> captured;
+  std::vector<CapturedQuery> captured;
   std::vector<bvh_graph> graphs;  // live graphs captured on this world
   unsigned long long* h_count = nullptr;  // pinned, 16 words: query counters and status words
   bool iteration_open = false;
+  // allocation generation of this world's buffers (its objects' and its own)
+  std::atomic<uint64_t> alloc_gen{0};
+  // batched queries: the table on the device and its pinned host staging
+  DevBuf bq_dev;
+  BatchQuery* bq_host = nullptr;
+  size_t bq_host_cap = 0;
+
+  bvh_world_s() {
+    DevBuf* own[] = {&s_kb, &s_ka, &s_va, &s_vb, &s_hist, &s_ctl, &s_lb, &prim_in, &prim_out, &sw_ka, &sw_kb,
+                     &sw_ga, &sw_gb, &sw_bq, &sw_bidx, &sw_kbs, &sw_bmax, &sw_count, &sw_total, &sw_out,
+                     &sw_part, &frame, &bq_dev};
+    for (DevBuf* d : own) d->gen = &alloc_gen;
+  }
 
   hipEvent_t take_event() {
     if (events_used == event_pool.size()) {
@@ -182,12 +212,19 @@ struct bvh_object_s {
   // (position -> input index) that patch centroids and patch elements follow;
   // the tree's own order (perm) is patch-major, Morton order inside a patch
   uint32_t split = BVH_SPLIT_CLUSTERING;
+  // BVH_SPLIT_GIVEN: the caller's patch offsets (od - 1, input order)
+  DevBuf given;
+  std::vector<uint64_t> given_h;
   DevBuf sp_rp, sp_rp_b, sp_scan, sp_tmpf, sp_tmpt, sp_seg, sp_seg_next, sp_k64, sp_acc, sp_axis, sp_flag, sp_lb,
       sp_ctr, sp_key, key0, key_a, key_b, sp_hist, sp_x;
   const uint32_t* ref_rp = nullptr;
   const uint32_t* ref_perm() const { return split == BVH_SPLIT_CLUSTERING ? perm.as<uint32_t>() : ref_rp; }
   // results
   DevBuf pairs, pairs_a, pairs_b, records, ppairs, sort_hist, sort_ctl, sort_lb, count, recheck, digest;
+  // exact primitive filter of the results: flags / offsets, kept pairs, scan words
+  DevBuf ex_flag, ex_out, ex_lb, ex_ctr;
+  // wire formats: patch centroids, patch ranges, element sources, host staging
+  DevBuf wire_cent, wire_range, wire_src, wire_out;
   uint64_t npairs = 0;
   uint64_t npatch_pairs = 0;
   uint64_t nvisits = 0;
@@ -201,7 +238,7 @@ struct bvh_object_s {
   bool have_results = false;
   bool results_pending = false;  // a captured query: counters not read yet
   // the last launched query: kind and the list capacities it ran with
-  bool q_self = false, q_narrow = false;
+  bool q_self = false, q_narrow = false, q_cross = false;
   uint64_t q_pcap = 0, q_cap = 0, q_rcap = 0;
   // Build stream: a build forks from the world stream onto it (so the builds
   // of several objects run concurrently) and the world stream waits for it
@@ -232,7 +269,9 @@ struct bvh_object_s {
                      &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
                      &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
                      &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
-                     &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x};
+                     &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x,
+                     &ex_flag, &ex_out, &ex_lb, &ex_ctr, &wire_cent, &wire_range, &wire_src, &wire_out,
+                     &given};
     for (DevBuf* d : all) f(*d);
   }
   size_t device_bytes() {
@@ -282,6 +321,7 @@ static TreeView view_of(const bvh_object_s* o) {
   v.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
   v.elem = o->in;
   v.patch_k64 = o->patch_k64.as<unsigned long long>();
+  v.splits = o->splits.as<uint32_t>();
   return v;
 }
 
@@ -392,8 +432,9 @@ static bvh_status object_alloc(bvh_object_s* o) {
   return BVH_OK;
 }
 
-// geom_axis / ml_geom_axis: the reference order and split offsets from the
-// splitter, then the tree order: the Morton order (already in perm and
+// geom_axis / ml_geom_axis / given patches: the reference order and split
+// offsets from the splitter (given: the caller's offsets, input order), then
+// the tree order: the Morton order (already in perm and
 // codes_a) stably sorted by patch, and the codes gathered into that order, so
 // that every patch is a contiguous, Morton-sorted range of the tree order.
 static bvh_status split_and_order(bvh_object_s* o) {
@@ -422,7 +463,11 @@ static bvh_status split_and_order(bvh_object_s* o) {
   sb.xbuf = o->sp_x.as<double>();
   uint32_t* rp = nullptr;
   uint32_t* patch_key = o->sp_key.as<uint32_t>();
-  HIP_TRY(launch_split(o->in, si, sb, od, o->split == BVH_SPLIT_ML_GEOM_AXIS ? 1 : 0, o->bb.splits, patch_key, &rp, s));
+  if (o->split == BVH_SPLIT_GIVEN)
+    HIP_TRY(launch_given_split(sb, n, od, o->given.as<uint32_t>(), o->bb.splits, patch_key, &rp, s));
+  else
+    HIP_TRY(launch_split(o->in, si, sb, od, o->split == BVH_SPLIT_ML_GEOM_AXIS ? 1 : 0, o->bb.splits, patch_key, &rp,
+                         s));
   o->ref_rp = rp;
   // stable sort of the Morton order by patch
   HIP_TRY(launch_gather_by<uint32_t>(patch_key, o->bb.perm, n, o->key0.as<uint32_t>(), s));
@@ -533,11 +578,18 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
                                     const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
                                     uint32_t split) {
   if (!o) return fail(BVH_E_INVALID, "null object");
-  if (split != BVH_SPLIT_CLUSTERING && split != BVH_SPLIT_GEOM_AXIS && split != BVH_SPLIT_ML_GEOM_AXIS)
+  if (split != BVH_SPLIT_CLUSTERING && split != BVH_SPLIT_GEOM_AXIS && split != BVH_SPLIT_ML_GEOM_AXIS &&
+      split != BVH_SPLIT_GIVEN)
     return fail(BVH_E_INVALID, "unknown split algorithm");
+  if (split == BVH_SPLIT_GIVEN) {
+    if (o->given_h.size() + 1 != o->w->od)
+      return fail(BVH_E_STATE, "BVH_SPLIT_GIVEN needs bvh_object_set_patch_offsets first");
+    if (!o->given_h.empty() && o->given_h.back() > n)
+      return fail(BVH_E_INVALID, "a patch offset lies beyond the element count");
+  }
   // geom_axis splits log2(od) times: od must be a power of two
   // (collision_object.cpp:122-124 asserts splits + 1 == od)
-  if (split != BVH_SPLIT_CLUSTERING && !is_pow2(o ? o->w->od : 0))
+  if ((split == BVH_SPLIT_GEOM_AXIS || split == BVH_SPLIT_ML_GEOM_AXIS) && !is_pow2(o ? o->w->od : 0))
     return fail(BVH_E_INVALID, "geom_axis / ml_geom_axis need a power-of-two overdecomposition factor");
   if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
   if (n > 0 && !data) return fail(BVH_E_INVALID, "null element data");
@@ -690,13 +742,14 @@ static bvh_status sort_results(bvh_object_s* o) {
 
 // Launches a query with the current buffer sizes; no synchronization (so it
 // can be captured into a graph).  The counters stay on the device.
-static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow) {
+static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow, bool cross) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   unsigned long long* dcount = o->count.as<unsigned long long>();
   const TreeView qv = view_of(o), tv = view_of(other);
   o->q_self = self;
   o->q_narrow = narrow;
+  o->q_cross = self && cross;
   o->q_pcap = o->ppairs.bytes / 8;
   o->q_cap = o->pairs.bytes / 8;
   o->q_rcap = o->recheck.bytes / 8 / kRecheckStripes;  // per stripe
@@ -711,7 +764,7 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
     w->stage_begin("narrowphase");
     PairOut po{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
                o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntStripes, dcount + kCntRecheck};
-    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? 1 : 0, o->fp32, w->shape, po, s));
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? (cross ? 2 : 1) : 0, o->fp32, w->shape, po, s));
     w->stage_end();
   }
   o->result_ptr = o->pairs.as<uint64_t>();
@@ -720,7 +773,7 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
   return BVH_OK;
 }
 
-static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true) {
+static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true, bool cross = false) {
   bvh_world w = o->w;
   if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
   o->have_results = false;
@@ -735,14 +788,16 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
     if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n) * 8));
   }
   if (w->capturing) {
-    BVH_TRY(launch_query(o, other, self, narrow));
-    o->have_results = true;
-    o->results_pending = true;
-    w->captured.push_back(o);
+    BVH_TRY(launch_query(o, other, self, narrow, cross));
+    // nothing to read until the graph has run (bvh_graph_launch sets these)
+    o->have_results = false;
+    o->results_pending = false;
+    w->captured.push_back(CapturedQuery{o, other, self, narrow, self, self && cross, o->q_pcap, o->q_cap, o->q_rcap,
+                                        o->result_ptr});
     return BVH_OK;
   }
   for (int attempt = 0; attempt < 2; ++attempt) {
-    BVH_TRY(launch_query(o, other, self, narrow));
+    BVH_TRY(launch_query(o, other, self, narrow, cross));
     bool overflow = false;
     BVH_TRY(read_counts(o, other, &overflow));
     if (!overflow) break;
@@ -762,6 +817,105 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   return BVH_OK;
 }
 
+// Several object pairs o[i] x other[i] in one coalesced step: their counters
+// cleared and patch broadphases run per pair, then every element
+// narrowphase in one launch (k_narrow64_batch), then one synchronization
+// for all their counters.  A pair whose lists overflowed runs again on its
+// own with grown buffers.  The fp64 form only (the fp32 form runs the pairs
+// one after another).
+static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* others, uint64_t k) {
+  bvh_world w = os[0]->w;
+  hipStream_t s = w->stream;
+  for (uint64_t i = 0; i < k; ++i) {
+    bvh_object_s* o = os[i];
+    if (!o->built || !others[i]->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
+    o->have_results = false;
+    o->results_pending = false;
+    BVH_TRY(o->count.ensure(kCntTotal * sizeof(unsigned long long)));
+    BVH_TRY(o->ppairs.ensure(std::max<uint64_t>(1024, 4 * w->od) * 8));
+    BVH_TRY(ensure_pairs(o, std::max<uint64_t>(1u << 20, 8 * o->in.n)));
+  }
+  if (k > w->bq_host_cap) {
+    if (w->bq_host) (void)hipHostFree(w->bq_host);
+    w->bq_host = nullptr;
+    w->bq_host_cap = 0;
+    void* hp = nullptr;
+    HIP_TRY(hipHostMalloc(&hp, k * sizeof(BatchQuery), hipHostMallocDefault));
+    w->bq_host = static_cast<BatchQuery*>(hp);
+    w->bq_host_cap = k;
+  }
+  BVH_TRY(w->bq_dev.ensure(k * sizeof(BatchQuery)));
+  // the previous batch's table may still be in flight to the device
+  HIP_TRY(hipStreamSynchronize(s));
+  uint32_t blocks = 0;
+  uint32_t nq = 0;
+  for (uint64_t i = 0; i < k; ++i) {
+    bvh_object_s* o = os[i];
+    bvh_object_s* t = others[i];
+    unsigned long long* dcount = o->count.as<unsigned long long>();
+    o->q_self = false;
+    o->q_narrow = true;
+    o->q_cross = false;
+    o->q_pcap = o->ppairs.bytes / 8;
+    o->q_cap = o->pairs.bytes / 8;
+    o->q_rcap = 0;
+    o->result_ptr = o->pairs.as<uint64_t>();
+    o->result_other = t;
+    o->self_result = false;
+    HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntTotal, 0u, s));
+    const TreeView qv = view_of(o), tv = view_of(t);
+    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+    HIP_TRY(launch_patch_broadphase(qv, tv, 0, po, s));
+    if (o->in.n == 0 || t->in.n == 0) continue;
+    BatchQuery& e = w->bq_host[nq++];
+    e.q = qv;
+    e.t = tv;
+    e.out = PairOut{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
+                    nullptr, 0, nullptr, nullptr};
+    e.block0 = blocks;
+    e.blocks = (uint32_t)((o->in.n + 255) / 256);
+    blocks += e.blocks;
+  }
+  if (nq) {
+    w->stage_begin("narrowphase_batch");
+    HIP_TRY(hipMemcpyAsync(w->bq_dev.p, w->bq_host, nq * sizeof(BatchQuery), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_narrowphase_batch(w->bq_dev.as<BatchQuery>(), nq, blocks, w->shape.batch == 4 ? 4 : 2, s));
+    w->stage_end();
+  }
+  // every pair's counters and both objects' error words, one synchronization
+  std::vector<unsigned long long> h(12 * k);
+  for (uint64_t i = 0; i < k; ++i) {
+    HIP_TRY(hipMemcpyAsync(&h[12 * i], os[i]->count.p, kCntWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&h[12 * i + kCntWords], os[i]->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&h[12 * i + kCntWords + 2], others[i]->status.p, 4 * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (uint64_t i = 0; i < k; ++i) {
+    bvh_object_s* o = os[i];
+    const unsigned long long* c = &h[12 * i];
+    o->npairs = c[kCntPairs];
+    o->nvisits = c[kCntVisits];
+    o->ncandidates = c[kCntCand];
+    o->nrecheck = o->nrecheck_max = 0;
+    o->npatch_pairs = c[kCntPatch];
+    uint32_t sw[8];
+    std::memcpy(sw, &c[kCntWords], sizeof(sw));
+    o->slab_err_bits = 0;
+    if (sw[0] | sw[4])
+      return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(sw[0] | sw[4]) + ")");
+    if (o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap) {
+      // this pair alone again, with the lists grown to its counts
+      if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
+      if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+      BVH_TRY(run_query(o, others[i], false));
+      continue;
+    }
+    o->have_results = true;
+  }
+  return BVH_OK;
+}
+
 // Results of a captured query become known on the host at the first access
 // after the graph ran.
 static bvh_status results_ready(bvh_object_s* o) {
@@ -778,6 +932,41 @@ static bvh_status results_ready(bvh_object_s* o) {
 }
 
 // ---- stand-alone primitives ------------------------------------------------------------------
+// Scratch of one stand-alone sort: key and value ping-pong buffers (n keys,
+// n values each), the digit histograms of every pass (8 x 256 words), the
+// sort control block and the look-back words (npass x tiles x 256 words).
+struct SortScratch {
+  void* ka;
+  void* kb;
+  uint32_t* va;
+  uint32_t* vb;
+  uint32_t* hist;
+  SortCtl* ctl;
+  uint32_t* lb;
+};
+
+// Stable sort of n keys (with optional values) in place, in the scratch
+// `sc`; the histograms and look-back words are cleared here by fill kernels,
+// the control block by the sort's plan kernel.  Waits for the result.
+template <typename K>
+static bvh_status sort_in_place(bvh_world w, K* keys, uint32_t* vals, uint64_t n, const SortScratch& sc) {
+  hipStream_t s = w->stream;
+  const int npass = sizeof(K);
+  const uint64_t tiles = sort_tiles(n);
+  HIP_TRY(launch_fill_u32(sc.hist, 8 * 256, 0u, s));
+  HIP_TRY(launch_fill_u32(sc.lb, npass * tiles * 256, 0u, s));
+  HIP_TRY(launch_sort_histogram<K>(keys, n, npass, sc.hist, s));
+  SortArgs<K> a{};
+  a.k0 = keys; a.ka = static_cast<K*>(sc.ka); a.kb = static_cast<K*>(sc.kb);
+  a.v0 = vals; a.va = vals ? sc.va : nullptr; a.vb = vals ? sc.vb : nullptr;
+  a.n = n; a.ntiles = tiles; a.ctl = sc.ctl; a.lookback = sc.lb;
+  HIP_TRY(launch_sort_passes<K>(a, npass, sc.hist, s));
+  HIP_TRY(hipMemcpyAsync(keys, a.ka, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+  if (vals) HIP_TRY(hipMemcpyAsync(vals, a.va, n * 4, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
 template <typename K>
 static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   if (!w) return fail(BVH_E_INVALID, "null world");
@@ -785,7 +974,6 @@ static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   if (n == 0) return BVH_OK;
   if (!keys) return fail(BVH_E_INVALID, "null keys");
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 keys");
-  hipStream_t s = w->stream;
   const int npass = sizeof(K);
   const uint64_t tiles = sort_tiles(n);
   BVH_TRY(w->s_ka.ensure(n * sizeof(K)));
@@ -795,18 +983,9 @@ static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   BVH_TRY(w->s_hist.ensure(8 * 256 * 4));
   BVH_TRY(w->s_ctl.ensure(sizeof(SortCtl)));
   BVH_TRY(w->s_lb.ensure(npass * tiles * 256 * 4));
-  HIP_TRY(launch_fill_u32(w->s_hist.as<uint32_t>(), 8 * 256, 0u, s));
-  HIP_TRY(launch_fill_u32(w->s_lb.as<uint32_t>(), npass * tiles * 256, 0u, s));
-  HIP_TRY(launch_sort_histogram<K>(keys, n, npass, w->s_hist.as<uint32_t>(), s));
-  SortArgs<K> a{};
-  a.k0 = keys; a.ka = w->s_ka.as<K>(); a.kb = w->s_kb.as<K>();
-  a.v0 = vals; a.va = vals ? w->s_va.as<uint32_t>() : nullptr; a.vb = vals ? w->s_vb.as<uint32_t>() : nullptr;
-  a.n = n; a.ntiles = tiles; a.ctl = w->s_ctl.as<SortCtl>(); a.lookback = w->s_lb.as<uint32_t>();
-  HIP_TRY(launch_sort_passes<K>(a, npass, w->s_hist.as<uint32_t>(), s));
-  HIP_TRY(hipMemcpyAsync(keys, a.ka, n * sizeof(K), hipMemcpyDeviceToDevice, s));
-  if (vals) HIP_TRY(hipMemcpyAsync(vals, a.va, n * 4, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return BVH_OK;
+  const SortScratch sc{w->s_ka.p, w->s_kb.p, w->s_va.as<uint32_t>(), w->s_vb.as<uint32_t>(),
+                       w->s_hist.as<uint32_t>(), w->s_ctl.as<SortCtl>(), w->s_lb.as<uint32_t>()};
+  return sort_in_place<K>(w, keys, vals, n, sc);
 }
 
 // device copy of a caller array: the array itself (device memory) or the
@@ -897,6 +1076,7 @@ bvh_status bvh_world_destroy(bvh_world w) {
   w->objects.clear();
   for (hipEvent_t e : w->event_pool) (void)hipEventDestroy(e);
   if (w->h_count) (void)hipHostFree(w->h_count);
+  if (w->bq_host) (void)hipHostFree(w->bq_host);
   if (w->own_stream) (void)hipStreamDestroy(w->stream);
   delete w;
   return BVH_OK;
@@ -912,6 +1092,7 @@ bvh_status bvh_create_collision_object(bvh_world w, bvh_object* out) {
   if (!w || !out) return fail(BVH_E_INVALID, "null argument");
   auto o = std::make_unique<bvh_object_s>();
   o->w = w;
+  o->for_each_buf([&](DevBuf& d) { d.gen = &w->alloc_gen; });
   o->id = w->objects.size();
   *out = o.get();
   w->objects.push_back(std::move(o));
@@ -943,6 +1124,12 @@ bvh_status bvh_synchronize(bvh_world w) {
   if (!w) return fail(BVH_E_INVALID, "null world");
   BVH_TRY(join_all(w));
   HIP_TRY(hipStreamSynchronize(w->stream));
+  return BVH_OK;
+}
+
+bvh_status bvh_world_stream(bvh_world w, void** stream) {
+  if (!w || !stream) return fail(BVH_E_INVALID, "null argument");
+  *stream = (void*)w->stream;
   return BVH_OK;
 }
 
@@ -1047,6 +1234,25 @@ bvh_status bvh_object_overdecomposition(bvh_object o, size_t* od) {
   return BVH_OK;
 }
 
+bvh_status bvh_object_set_patch_offsets(bvh_object o, const uint64_t* offsets, size_t count) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "patch offsets inside a capture");
+  if (count + 1 != o->w->od) return fail(BVH_E_INVALID, "od - 1 patch offsets expected");
+  if (count && !offsets) return fail(BVH_E_INVALID, "null offsets");
+  std::vector<uint32_t> dev(count ? count : 1, 0);
+  for (size_t k = 0; k < count; ++k) {
+    if (k && offsets[k] < offsets[k - 1]) return fail(BVH_E_INVALID, "patch offsets must not decrease");
+    if (offsets[k] >= (1ull << 30)) return fail(BVH_E_INVALID, "patch offset beyond 2^30 - 1 elements");
+    dev[k] = (uint32_t)offsets[k];
+  }
+  BVH_TRY(o->given.ensure(dev.size() * 4));
+  HIP_TRY(hipMemcpyAsync(o->given.p, dev.data(), dev.size() * 4, hipMemcpyHostToDevice, o->w->stream));
+  HIP_TRY(hipStreamSynchronize(o->w->stream));
+  o->given_h.assign(offsets, offsets + count);
+  return BVH_OK;
+}
+
 bvh_status bvh_set_entity_data_poly(bvh_object o, const double* verts, uint32_t nv, uint64_t n,
                                     const uint64_t* gids, double eps, uint32_t mem, uint32_t split) {
   if (nv < 1 || nv > 8) return fail(BVH_E_INVALID, "verts_per_element must be in 1..8");
@@ -1128,10 +1334,37 @@ bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other) {
   return run_query(o, other, false, false);
 }
 
+bvh_status bvh_broadphase_batch(bvh_object* os, bvh_object* others, uint64_t k) {
+  if (k == 0) return BVH_OK;
+  if (!os || !others) return fail(BVH_E_INVALID, "null object list");
+  bvh_world w = os[0] ? os[0]->w : nullptr;
+  if (!w) return fail(BVH_E_INVALID, "null object");
+  for (uint64_t i = 0; i < k; ++i) {
+    if (!os[i] || !others[i]) return fail(BVH_E_INVALID, "null object");
+    if (os[i]->w != w || others[i]->w != w) return fail(BVH_E_INVALID, "objects of different worlds");
+    if (os[i] == others[i]) return fail(BVH_E_INVALID, "a pair of one object (self collision is not batched)");
+    for (uint64_t j = 0; j < i; ++j)
+      if (os[j] == os[i]) return fail(BVH_E_INVALID, "an object receives the results of two pairs");
+  }
+  BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "broadphase_batch inside a capture");
+  if (w->cfg.precision == BVH_PREC_FP32_OUTWARD || (w->cfg.flags & BVH_FLAG_SORT_RESULTS)) {
+    for (uint64_t i = 0; i < k; ++i) BVH_TRY(run_query(os[i], others[i], false));
+    return BVH_OK;
+  }
+  return run_query_batch(os, others, k);
+}
+
 bvh_status bvh_self_collision(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   BVH_TRY(join_all(o->w));
   return run_query(o, o, true);
+}
+
+bvh_status bvh_self_collision_cross_patch(bvh_object o) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
+  return run_query(o, o, true, true, true);
 }
 
 bvh_status bvh_end_phase(bvh_object o) {
@@ -1507,6 +1740,303 @@ bvh_status bvh_closest_point(bvh_world w, const double* points, uint64_t n, cons
   return copy_out(closest, w->prim_out.p, bytes, BVH_MEM_HOST, s);
 }
 
+// ---- exact primitive tests -------------------------------------------------------------
+static bvh_status prim_pairs(bvh_world w, const double* p, const double* q, uint64_t n, uint32_t mem, uint8_t* out,
+                             int width, bool tri) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "primitive tests inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (n == 0) return BVH_OK;
+  if (!p || !q || !out) return fail(BVH_E_INVALID, "null input or output");
+  hipStream_t s = w->stream;
+  const double *dp = nullptr, *dq = nullptr;
+  BVH_TRY(stage_in(w->sw_ka, p, n * width, mem, s, &dp));
+  BVH_TRY(stage_in(w->sw_kb, q, n * width, mem, s, &dq));
+  uint8_t* dout = out;
+  if (mem == BVH_MEM_HOST) {
+    BVH_TRY(w->sw_out.ensure(n));
+    dout = w->sw_out.as<uint8_t>();
+  }
+  if (tri) HIP_TRY(launch_tri_pairs(dp, dq, n, dout, s));
+  else HIP_TRY(launch_sphere_pairs(dp, dq, n, dout, s));
+  if (mem == BVH_MEM_HOST) return copy_out(out, dout, n, BVH_MEM_HOST, s);
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_triangle_pairs_intersect(bvh_world w, const double* p, const double* q, uint64_t n, uint32_t mem,
+                                        uint8_t* out) {
+  return prim_pairs(w, p, q, n, mem, out, 9, true);
+}
+
+bvh_status bvh_sphere_pairs_intersect(bvh_world w, const double* a, const double* b, uint64_t n, uint32_t mem,
+                                      uint8_t* out) {
+  return prim_pairs(w, a, b, n, mem, out, 4, false);
+}
+
+bvh_status bvh_result_filter_exact(bvh_object o, uint64_t* kept) {
+  if (!o) return fail(BVH_E_INVALID, "null object");
+  BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "result_filter_exact inside a capture");
+  BVH_TRY(results_ready(o));
+  if (!o->have_results) return fail(BVH_E_STATE, "no results to filter: run broadphase or self_collision first");
+  bvh_object_s* t = o->result_other;
+  const uint64_t n = o->npairs;
+  if (n == 0 || !t) {
+    if (kept) *kept = 0;
+    return BVH_OK;
+  }
+  const ElemInput& a = o->in;
+  const ElemInput& b = t->in;
+  int tri;
+  if (a.kind == kPoly && a.nv == 3 && b.kind == kPoly && b.nv == 3) tri = 1;
+  else if (a.kind == kSphere && b.kind == kSphere) tri = 0;
+  else return fail(BVH_E_UNSUPPORTED, "exact primitive tests take triangles (3-vertex polytopes) or spheres on both objects");
+  if (n >= (1ull << 32)) return fail(BVH_E_UNSUPPORTED, "result_filter_exact holds at most 2^32 - 1 pairs");
+  hipStream_t s = o->w->stream;
+  const uint64_t tiles = sort_tiles(n);
+  BVH_TRY(o->ex_flag.ensure(n * 4));
+  BVH_TRY(o->ex_out.ensure(n * 8));
+  BVH_TRY(o->ex_lb.ensure(tiles * 8));
+  BVH_TRY(o->ex_ctr.ensure(256));
+  uint32_t* flag = o->ex_flag.as<uint32_t>();
+  unsigned long long* dkept = reinterpret_cast<unsigned long long*>(o->ex_ctr.as<char>() + 128);
+  HIP_TRY(launch_fill_u32(o->ex_lb.as<uint32_t>(), 2 * tiles, 0u, s));
+  HIP_TRY(launch_fill_u32(o->ex_ctr.as<uint32_t>(), 64, 0u, s));
+  HIP_TRY(launch_pair_flags(o->result_ptr, n, a, b, tri, flag, s));
+  HIP_TRY(launch_scan_u32(flag, n, o->ex_lb.as<uint64_t>(), o->ex_ctr.as<uint32_t>(), s));
+  HIP_TRY(launch_pair_compact(o->result_ptr, n, a, b, tri, flag, o->ex_out.as<uint64_t>(), dkept, s));
+  unsigned long long k = 0;
+  BVH_TRY(copy_out(&k, dkept, sizeof(k), BVH_MEM_HOST, s));
+  o->result_ptr = o->ex_out.as<uint64_t>();
+  o->npairs = k;
+  if (kept) *kept = k;
+  return BVH_OK;
+}
+
+// ---- wire formats ------------------------------------------------------------------------
+// doubles of one element record of an element kind (after its u64 id)
+static uint32_t element_doubles(int kind, int nv) {
+  return kind == kPoly ? (uint32_t)(3 * nv) : (kind == kSphere ? 4u : (uint32_t)(kSlabs + 3));
+}
+
+// the patch's range [begin, end) of positions in the reference order
+static bvh_status patch_ranges(bvh_object_s* o, std::vector<uint64_t>& b, std::vector<uint64_t>& e) {
+  const size_t od = o->w->od;
+  const uint64_t n = o->in.n;
+  std::vector<uint32_t> splits(od > 1 ? od - 1 : 1, 0);
+  if (od > 1 && n > 0) BVH_TRY(copy_out(splits.data(), o->splits.p, (od - 1) * 4, BVH_MEM_HOST, o->w->stream));
+  b.resize(od);
+  e.resize(od);
+  for (size_t k = 0; k < od; ++k) {
+    b[k] = (k == 0) ? 0 : splits[k - 1];
+    e[k] = (k == od - 1) ? n : splits[k];
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_pack_patches(bvh_object o, void* out, uint64_t cap, uint64_t* bytes, uint32_t mem) {
+  if (!o || !bytes) return fail(BVH_E_INVALID, "null argument");
+  if (!o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "pack inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  const uint64_t od = o->w->od, n = o->in.n;
+  *bytes = od * kPatchRecordBytes;
+  if (!out) return BVH_OK;
+  if (cap < *bytes) return fail(BVH_E_OVERFLOW, "patch buffer holds " + std::to_string(cap) + " of " +
+                                                    std::to_string(*bytes) + " bytes");
+  hipStream_t s = o->w->stream;
+  BVH_TRY(o->wire_cent.ensure(3 * od * sizeof(double)));
+  double* cent = o->wire_cent.as<double>();
+  HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(cent), 6 * od, 0u, s));
+  if (n > 0) {
+    const double* cpl = o->in.kind == kPoly ? o->centroid.as<double>() : nullptr;
+    const double* spl = o->in.kind == kSphere ? o->in.data : nullptr;
+    const double* cao = o->in.kind == kSnapshot ? o->in.centroids : nullptr;
+    HIP_TRY(launch_patch_centroids(cpl, spl, cao, o->ref_perm(), o->splits.as<uint32_t>(), n, od, cent, s));
+  }
+  void* dst = out;
+  if (mem == BVH_MEM_HOST) {
+    BVH_TRY(o->wire_out.ensure(*bytes));
+    dst = o->wire_out.p;
+  }
+  HIP_TRY(launch_pack_patches(o->patch_k64.as<unsigned long long>(), o->splits.as<uint32_t>(),
+                              o->patch_root.as<uint32_t>(), cent, n, od, dst, s));
+  if (mem == BVH_MEM_HOST) return copy_out(out, dst, *bytes, BVH_MEM_HOST, s);
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_unpack_patches(bvh_world w, const void* in, uint64_t bytes, uint32_t mem, bvh_patch_info* out,
+                              uint64_t cap, uint64_t* count) {
+  if (!w || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(w));
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (bytes % kPatchRecordBytes) return fail(BVH_E_INVALID, "not a whole number of 256-byte patch records");
+  const uint64_t k = bytes / kPatchRecordBytes;
+  *count = k;
+  if (!out) return BVH_OK;
+  if (cap < k) return fail(BVH_E_OVERFLOW, "patch info buffer too small");
+  if (k == 0) return BVH_OK;
+  if (!in) return fail(BVH_E_INVALID, "null input");
+  std::vector<uint64_t> host(bytes / 8);
+  if (mem == BVH_MEM_DEVICE) BVH_TRY(copy_out(host.data(), in, bytes, BVH_MEM_HOST, w->stream));
+  else std::memcpy(host.data(), in, bytes);
+  for (uint64_t i = 0; i < k; ++i) {
+    const uint64_t* r = host.data() + i * (kPatchRecordBytes / 8);
+    bvh_patch_info& pi = out[i];
+    pi.global_id = r[0];
+    pi.size = r[1];
+    std::memcpy(pi.kdop, r + 2, sizeof(pi.kdop));
+    std::memcpy(pi.centroid, r + 2 + kSlabs, sizeof(pi.centroid));
+  }
+  return BVH_OK;
+}
+
+bvh_status bvh_element_record_bytes(bvh_object o, uint64_t* stride) {
+  if (!o || !stride) return fail(BVH_E_INVALID, "null argument");
+  if (!o->have_data) return fail(BVH_E_STATE, "no element data");
+  *stride = 8 + 8 * (uint64_t)element_doubles(o->in.kind, o->in.nv);
+  return BVH_OK;
+}
+
+bvh_status bvh_pack_patch_elements(bvh_object o, const uint64_t* patches, uint64_t npatches, void* out, uint64_t cap,
+                                   uint64_t* bytes, uint32_t mem) {
+  if (!o || !bytes) return fail(BVH_E_INVALID, "null argument");
+  if (!o->built) return fail(BVH_E_STATE, "object not built");
+  BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "pack inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (npatches && !patches) return fail(BVH_E_INVALID, "null patch list");
+  if (npatches > 65535) return fail(BVH_E_INVALID, "at most 65535 patches per pack");
+  const size_t od = o->w->od;
+  std::vector<uint64_t> b, e;
+  BVH_TRY(patch_ranges(o, b, e));
+  std::vector<uint64_t> range(3 * (npatches ? npatches : 1));
+  uint64_t total = 0, longest = 0;
+  for (uint64_t k = 0; k < npatches; ++k) {
+    if (patches[k] >= od) return fail(BVH_E_INVALID, "patch id out of range");
+    range[3 * k] = b[patches[k]];
+    range[3 * k + 1] = e[patches[k]];
+    range[3 * k + 2] = total;
+    total += e[patches[k]] - b[patches[k]];
+    longest = std::max(longest, e[patches[k]] - b[patches[k]]);
+  }
+  const uint64_t stride = 8 + 8 * (uint64_t)element_doubles(o->in.kind, o->in.nv);
+  *bytes = total * stride;
+  if (!out || total == 0) return BVH_OK;
+  if (cap < *bytes) return fail(BVH_E_OVERFLOW, "element buffer holds " + std::to_string(cap) + " of " +
+                                                    std::to_string(*bytes) + " bytes");
+  hipStream_t s = o->w->stream;
+  BVH_TRY(o->wire_range.ensure(range.size() * 8));
+  BVH_TRY(o->wire_src.ensure(total * 4));
+  HIP_TRY(hipMemcpyAsync(o->wire_range.p, range.data(), range.size() * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_patch_positions(o->ref_perm(), o->wire_range.as<uint64_t>(), npatches, longest,
+                                 o->wire_src.as<uint32_t>(), s));
+  void* dst = out;
+  if (mem == BVH_MEM_HOST) {
+    BVH_TRY(o->wire_out.ensure(*bytes));
+    dst = o->wire_out.p;
+  }
+  HIP_TRY(launch_pack_elements(o->in, o->has_gids ? o->gids.as<uint64_t>() : nullptr, o->wire_src.as<uint32_t>(),
+                               total, element_doubles(o->in.kind, o->in.nv), dst, s));
+  if (mem == BVH_MEM_HOST) return copy_out(out, dst, *bytes, BVH_MEM_HOST, s);
+  // the range table above is pageable host memory read by the copy engine:
+  // keep it alive until the stream has passed it
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_unpack_elements(bvh_world w, const void* in, uint64_t count, uint32_t kind, uint32_t nv, double* data,
+                               double* centroids, uint64_t* gids, uint32_t mem) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "unpack inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (kind != kPoly && kind != kSphere && kind != kSnapshot) return fail(BVH_E_INVALID, "unknown element kind");
+  if (kind == kPoly && (nv < 1 || nv > 8)) return fail(BVH_E_INVALID, "verts_per_element must be in 1..8");
+  if (count == 0) return BVH_OK;
+  if (!in || !data || (kind == kSnapshot && !centroids)) return fail(BVH_E_INVALID, "null input or output");
+  const uint32_t planes = element_doubles((int)kind, (int)nv);
+  const uint64_t stride = 8 + 8 * (uint64_t)planes;
+  hipStream_t s = w->stream;
+  const uint64_t nd = (kind == kSnapshot ? (uint64_t)kSlabs : planes) * count;
+  if (mem == BVH_MEM_DEVICE) {
+    HIP_TRY(launch_unpack_elements(in, count, planes, kind == kSnapshot, data, centroids, gids, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BVH_OK;
+  }
+  BVH_TRY(w->prim_in.ensure(count * stride));
+  BVH_TRY(w->prim_out.ensure(nd * 8 + (kind == kSnapshot ? 3 * count * 8 : 0) + count * 8));
+  char* o = w->prim_out.as<char>();
+  double* dd = reinterpret_cast<double*>(o);
+  double* dc = kind == kSnapshot ? dd + nd : nullptr;
+  uint64_t* dg = reinterpret_cast<uint64_t*>(o + nd * 8 + (kind == kSnapshot ? 3 * count * 8 : 0));
+  HIP_TRY(hipMemcpyAsync(w->prim_in.p, in, count * stride, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_unpack_elements(w->prim_in.p, count, planes, kind == kSnapshot, dd, dc, dg, s));
+  BVH_TRY(copy_out(data, dd, nd * 8, BVH_MEM_HOST, s));
+  if (dc) BVH_TRY(copy_out(centroids, dc, 3 * count * 8, BVH_MEM_HOST, s));
+  if (gids) BVH_TRY(copy_out(gids, dg, count * 8, BVH_MEM_HOST, s));
+  return BVH_OK;
+}
+
+bvh_status bvh_pack_results(bvh_object o, void* out, uint64_t cap, uint64_t* bytes, uint32_t mem) {
+  if (!o || !bytes) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "pack inside a capture");
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  BVH_TRY(results_ready(o));
+  const uint64_t n = o->have_results ? o->npairs : 0;
+  *bytes = 16 + n * kResultRecordBytes;
+  if (!out) return BVH_OK;
+  if (cap < *bytes) return fail(BVH_E_OVERFLOW, "result message buffer holds " + std::to_string(cap) + " of " +
+                                                    std::to_string(*bytes) + " bytes");
+  hipStream_t s = o->w->stream;
+  void* dst = out;
+  if (mem == BVH_MEM_HOST) {
+    BVH_TRY(o->wire_out.ensure(*bytes));
+    dst = o->wire_out.p;
+  }
+  HIP_TRY(launch_result_header(dst, n, s));
+  if (n > 0) {
+    bvh_object_s* q = o->result_other;
+    HIP_TRY(launch_records(o->result_ptr, n, o->patch_of_input.as<uint32_t>(),
+                           o->has_gids ? o->gids.as<uint64_t>() : nullptr, q->patch_of_input.as<uint32_t>(),
+                           q->has_gids ? q->gids.as<uint64_t>() : nullptr, 0, 0,
+                           reinterpret_cast<uint64_t*>(static_cast<char*>(dst) + 16), s));
+  }
+  if (mem == BVH_MEM_HOST) return copy_out(out, dst, *bytes, BVH_MEM_HOST, s);
+  HIP_TRY(hipStreamSynchronize(s));
+  return BVH_OK;
+}
+
+bvh_status bvh_unpack_results(bvh_world w, const void* in, uint64_t bytes, uint32_t mem, bvh_narrowphase_record* out,
+                              uint64_t cap, uint64_t* count) {
+  if (!w || !count) return fail(BVH_E_INVALID, "null argument");
+  BVH_TRY(join_all(w));
+  if (mem != BVH_MEM_HOST && mem != BVH_MEM_DEVICE) return fail(BVH_E_INVALID, "bad memory kind");
+  if (!in || bytes < 16) return fail(BVH_E_INVALID, "a result message holds at least its 16-byte header");
+  uint64_t hdr[2];
+  if (mem == BVH_MEM_DEVICE) BVH_TRY(copy_out(hdr, in, 16, BVH_MEM_HOST, w->stream));
+  else std::memcpy(hdr, in, 16);
+  if (hdr[0] != kResultRecordBytes)
+    return fail(BVH_E_INVALID, "result stride " + std::to_string(hdr[0]) + ", expected 32");
+  if (hdr[1] % kResultRecordBytes || hdr[1] > bytes - 16)
+    return fail(BVH_E_INVALID, "result byte count inconsistent with the message size");
+  const uint64_t k = hdr[1] / kResultRecordBytes;
+  *count = k;
+  if (!out || k == 0) return BVH_OK;
+  if (cap < k) return fail(BVH_E_OVERFLOW, "record buffer too small");
+  const char* src = static_cast<const char*>(in) + 16;
+  if (mem == BVH_MEM_HOST) {
+    std::memcpy(out, src, hdr[1]);
+    return BVH_OK;
+  }
+  return copy_out(out, src, hdr[1], BVH_MEM_DEVICE, w->stream);
+}
+
 // ---- sort and sweep between two patches ---------------------------------------------
 bvh_status bvh_element_kdops(bvh_world w, const double* verts, uint32_t nv, uint64_t n, double eps, uint32_t mem,
                              double* kdops) {
@@ -1631,6 +2161,8 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "autotune inside a capture");
   const bool self = o == other;
+  // a self query is tuned in the kind last run on o (all pairs or cross-patch)
+  const bool cross = self && o->q_cross;
   static const uint32_t kBatch[3] = {1, 2, 4};
   static const uint32_t kBlock[3] = {64, 128, 256};
   const LaunchShape keep_shape = w->shape;
@@ -1649,11 +2181,11 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
         w->shape.batch = o->fp32 ? keep_shape.batch : kBatch[ki];
         w->shape.block = kBlock[bi];
         w->shape.xcd_remap = xi == 1;
-        st = run_query(o, other, self);  // sizes the buffers
+        st = run_query(o, other, self, true, cross);  // sizes the buffers
         double t = 0.0;
         for (int r = 0; r < 3 && st == BVH_OK; ++r) {
           (void)hipEventRecord(e0, w->stream);
-          st = run_query(o, other, self);
+          st = run_query(o, other, self, true, cross);
           (void)hipEventRecord(e1, w->stream);
           (void)hipEventSynchronize(e1);
           float f = 0.0f;
@@ -1680,7 +2212,7 @@ bvh_status bvh_autotune(bvh_object o, bvh_object other, bvh_tuning* best) {
   w->shape.block = b.block_size;
   w->shape.xcd_remap = b.xcd_remap != 0;
   *best = b;
-  return run_query(o, other, self);  // leave the results of the chosen setting
+  return run_query(o, other, self, true, cross);  // leave the results of the chosen setting
 }
 
 bvh_status bvh_set_launch(bvh_world w, const bvh_tuning* shape) {
@@ -1709,8 +2241,8 @@ struct bvh_graph_s {
   bvh_world w = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint64_t gen = 0;
-  std::vector<bvh_object_s*> objects;
+  uint64_t gen = 0;                     // the world's allocation generation at capture
+  std::vector<CapturedQuery> queries;   // the captured queries, the last one per object
 };
 
 static void orphan_graphs(bvh_world w) {
@@ -1748,8 +2280,13 @@ bvh_status bvh_capture_end(bvh_world w, bvh_graph* out) {
   auto g = std::make_unique<bvh_graph_s>();
   g->w = w;
   g->graph = graph;
-  g->gen = g_alloc_gen.load();
-  g->objects = w->captured;
+  g->gen = w->alloc_gen.load();
+  // the last query captured on an object is the one whose results it holds
+  for (const CapturedQuery& c : w->captured) {
+    auto it = std::find_if(g->queries.begin(), g->queries.end(), [&](const CapturedQuery& x) { return x.o == c.o; });
+    if (it != g->queries.end()) *it = c;
+    else g->queries.push_back(c);
+  }
   w->captured.clear();
   const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
   if (ei != hipSuccess) {
@@ -1766,10 +2303,21 @@ bvh_status bvh_graph_launch(bvh_graph g) {
   if (!g->w) return fail(BVH_E_STATE, "the graph's world was destroyed");
   BVH_TRY(join_all(g->w));
   if (g->w->capturing) return fail(BVH_E_STATE, "graph launch inside a capture");
-  if (g->gen != g_alloc_gen.load())
-    return fail(BVH_E_STATE, "device buffers were reallocated since the capture; capture again");
+  if (g->gen != g->w->alloc_gen.load())
+    return fail(BVH_E_STATE, "device buffers of this world were reallocated since the capture; capture again");
   HIP_TRY(hipGraphLaunch(g->exec, g->w->stream));
-  for (bvh_object_s* o : g->objects) {
+  // the captured queries' state, whatever the objects ran since the capture
+  for (const CapturedQuery& c : g->queries) {
+    bvh_object_s* o = c.o;
+    o->result_other = c.other;
+    o->q_self = c.self;
+    o->q_narrow = c.narrow;
+    o->q_cross = c.cross;
+    o->self_result = c.self_result;
+    o->q_pcap = c.pcap;
+    o->q_cap = c.cap;
+    o->q_rcap = c.rcap;
+    o->result_ptr = c.result_ptr;
     o->have_results = true;
     o->results_pending = true;
   }

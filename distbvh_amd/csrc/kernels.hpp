@@ -152,6 +152,10 @@ struct SplitBufs {
 // od a power of two.
 hipError_t launch_split(const ElemInput& in, const SplitInput& si, SplitBufs sb, uint64_t od, int ml,
                         uint32_t* splits, uint32_t* patch_key, uint32_t** result, hipStream_t s);
+// caller-given patches: patch k = input indices [given[k-1], given[k]); the
+// reference order is the input order (*result = sb.rp, the identity)
+hipError_t launch_given_split(SplitBufs sb, uint64_t n, uint64_t od, const uint32_t* given, uint32_t* splits,
+                              uint32_t* patch_key, uint32_t** result, hipStream_t s);
 template <typename T>
 hipError_t launch_gather_by(const T* src, const uint32_t* idx, uint64_t n, T* dst, hipStream_t s);
 
@@ -161,6 +165,34 @@ struct Tri {
 };
 // closest point on the triangle for each of n points (n x 3 in, n x 3 out)
 hipError_t launch_closest_point(const double* pts, uint64_t n, const Tri& tri, double* out, hipStream_t s);
+// closed triangle / ball intersection of n pairs: p, q n x 9 (triangles a, b,
+// c) or n x 4 (x, y, z, r), out[i] in {0, 1}
+hipError_t launch_tri_pairs(const double* p, const double* q, uint64_t n, uint8_t* out, hipStream_t s);
+hipError_t launch_sphere_pairs(const double* a, const double* b, uint64_t n, uint8_t* out, hipStream_t s);
+// result pairs whose primitives meet (tri: triangles of 3-vertex polytope
+// planes, else spheres): flag[i] = 0 / 1, then after an exclusive scan of
+// the flags (off), the kept pairs in order into out and their count into *kept
+hipError_t launch_pair_flags(const uint64_t* pairs, uint64_t count, const ElemInput& qa, const ElemInput& tb, int tri,
+                             uint32_t* flag, hipStream_t s);
+hipError_t launch_pair_compact(const uint64_t* pairs, uint64_t count, const ElemInput& qa, const ElemInput& tb,
+                               int tri, const uint32_t* off, uint64_t* out, unsigned long long* kept, hipStream_t s);
+
+// ---- wire formats (wire.hip) ---------------------------------------------------------
+constexpr uint64_t kPatchRecordBytes = 256;  // u64 id, u64 size, 26 f64 slabs, 4 f64 centroid storage
+constexpr uint64_t kResultRecordBytes = 32;  // detailed_narrowphase_result
+hipError_t launch_pack_patches(const unsigned long long* patch_k64, const uint32_t* splits, const uint32_t* roots,
+                               const double* cent, uint64_t n, uint64_t od, void* out, hipStream_t s);
+// src[offset_k + i] = ref_perm[begin_k + i] for each listed patch k (range:
+// {begin, end, offset} per patch); max_len = the longest range
+hipError_t launch_patch_positions(const uint32_t* ref_perm, const uint64_t* range, uint64_t npatch,
+                                  uint64_t max_len, uint32_t* src, hipStream_t s);
+// element records {u64 global id, `planes` doubles} of the elements src[0..count)
+hipError_t launch_pack_elements(const ElemInput& in, const uint64_t* gids, const uint32_t* src, uint64_t count,
+                                uint32_t planes, void* out, hipStream_t s);
+hipError_t launch_unpack_elements(const void* in, uint64_t count, uint32_t planes, int snapshot, double* data,
+                                  double* cent, uint64_t* gids, hipStream_t s);
+// result message header: stride, byte count
+hipError_t launch_result_header(void* out, uint64_t count, hipStream_t s);
 
 // ---- sort and sweep between two patches (sweep.hip) --------------------------------
 struct SweepArgs {
@@ -208,6 +240,7 @@ struct TreeView {
   const double* origin;  // fp32 form: frame origin (3 doubles)
   ElemInput elem;        // the object's element data (fp32 form re-checks in fp64)
   const unsigned long long* patch_k64;  // exact patch slabs (ordered_u64)
+  const uint32_t* splits;               // od - 1 patch starts in tree-order positions
 };
 
 struct PairOut {
@@ -245,9 +278,22 @@ struct LaunchShape {
 };
 
 // element narrowphase: every leaf of `q` (queries, in its sorted order)
-// against tree `t`; self != 0 emits each unordered pair of one tree once.
+// against tree `t`; self = 1 emits each unordered pair of one tree once,
+// self = 2 only those whose elements lie in different patches.
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self,
                               int fp32, const LaunchShape& shape, const PairOut& out, hipStream_t s);
+// One query of a batched narrowphase launch: tree views, outputs and the
+// blocks (256 threads, one query element each) [block0, block0 + blocks)
+struct BatchQuery {
+  TreeView q;
+  TreeView t;
+  PairOut out;
+  uint32_t block0;
+  uint32_t blocks;
+};
+// the fp64 pair walks of nq queries (no self, no XCD remap) in one launch of
+// `blocks` blocks of 256 threads; bq in device memory; batch 2 or 4
+hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, hipStream_t s);
 // patch broadphase: patch roots of `q` against the patch level of `t`
 hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
                                    hipStream_t s);

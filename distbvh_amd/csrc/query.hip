@@ -195,7 +195,21 @@ __device__ __forceinline__ void recheck_flush(uint64_t* buf, uint32_t& fill, con
   fill = 0;
 }
 
-template <bool SELF>
+// The self walks take CROSS = true for cross-patch self collision: a lane
+// skips every child whose position range lies inside its own patch's range
+// [pb, pe) (patches are contiguous position ranges of the tree order), so
+// pairs within one patch are neither tested nor emitted.
+__device__ __forceinline__ void own_patch_range(const TreeView& q, uint32_t qa, bool valid, uint32_t n, uint32_t& pb,
+                                                uint32_t& pe) {
+  pb = 0;
+  pe = 0;
+  if (!valid) return;
+  const uint32_t k = q.patch_of_input[qa];
+  pb = k == 0 ? 0u : q.splits[k - 1];
+  pe = (uint64_t)k + 1 >= q.od ? n : q.splits[k];
+}
+
+template <bool SELF, bool CROSS = false>
 __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xcd, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
   __shared__ uint64_t s_rbuf[256 / kWave][kRecheckBuf];
@@ -237,6 +251,8 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
   const uint32_t tleaf0 = n - 1;
   const uint32_t lane = lane_id();
   const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
+  uint32_t pb = 0, pe = 0;
+  if (CROSS) own_patch_range(q, qa, valid, n, pb, pe);
   CRec32* recs = (CRec32*)t.rec32;
   uint32_t steps = 0;
   uint32_t ncand = 0;
@@ -288,6 +304,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
 #pragma unroll
     for (int k = 0; k < 56; ++k) rw[k] = rp[k];
     const uint32_t rlast = SELF ? recs[x].last : 0u;
+    const uint32_t rfirst = CROSS ? recs[x].first : 0u;
     __builtin_amdgcn_sched_barrier(0);
     const uint32_t cl = (uint32_t)rw[52], cr = (uint32_t)rw[53];
     const uint32_t el = (uint32_t)rw[54], er = (uint32_t)rw[55];
@@ -312,6 +329,10 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
       // position bounds (non-short-circuit &: no divergent branch)
       hl = hl & (x > p32);  // the left child ends at position x
       hr = hr & (rlast > p32);
+    }
+    if (CROSS) {  // children [rfirst, x] and [x + 1, rlast] inside the lane's own patch
+      hl = hl & !((rfirst >= pb) & (x < pe));
+      hr = hr & !((x + 1 >= pb) & (rlast < pe));
     }
     const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
     if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl & (cl - tleaf0 > p32)) : hl);
@@ -414,17 +435,13 @@ __device__ __forceinline__ void overlap_pair(const double* qk, const Rec64& r, b
 // set of visited nodes does not depend on K (a node is visited when some lane
 // overlaps it and its parent was visited), only the number of dependent
 // round trips does.  Leaf tests are exact, so hits are emitted directly.
-template <bool SELF, int K>
-__global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xcd, PairOut out) {
-  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
-  __shared__ uint32_t s_stack[256 / kWave][kStack];
-  __shared__ Rec64 s_rec[256 / kWave][K];
-  const uint32_t wv = threadIdx.x / kWave;
-  uint64_t* buf = s_buf[wv];
-  uint32_t* stk = s_stack[wv];
-  Rec64* R = s_rec[wv];
+// The walk of one tile of blockDim.x queries (the body of k_narrow64 and of
+// its batched form k_narrow64_batch); buf / stk / R are the wave's LDS pair
+// staging, stack and record slots.
+template <bool SELF, int K, bool CROSS>
+__device__ __forceinline__ void walk64(const TreeView& q, const TreeView& t, uint32_t tile, const PairOut& out,
+                                       uint64_t* buf, uint32_t* stk, Rec64* R) {
   uint32_t fill = 0;
-  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
   double qk[kSlabs];
@@ -452,6 +469,8 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
   const uint32_t tleaf0 = n - 1;
   const uint32_t lane = lane_id();
   const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
+  uint32_t pb = 0, pe = 0;
+  if (CROSS) own_patch_range(q, qa, valid, n, pb, pe);
   uint32_t steps = 0;
   uint32_t sp = 0;
   {
@@ -508,6 +527,11 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
           hl = hl & (xs[k] > p32);  // the left child ends at position x
           hr = hr & (uniform(Rk.last) > p32);
         }
+        if (CROSS) {  // children inside the lane's own patch
+          const uint32_t f = uniform(Rk.first), l = uniform(Rk.last);
+          hl = hl & !((f >= pb) & (xs[k] < pe));
+          hr = hr & !((xs[k] + 1 >= pb) & (l < pe));
+        }
         // ballots and readfirstlane values: every branch below is wave-uniform,
         // so the stack pointer and counters stay in scalar registers
         const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
@@ -537,6 +561,36 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
   if (lane == 0 && out.visits) atomicAdd(out.visits, (unsigned long long)steps);
 }
 
+template <bool SELF, int K, bool CROSS = false>
+__global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xcd, PairOut out) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  __shared__ Rec64 s_rec[256 / kWave][K];
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  walk64<SELF, K, CROSS>(q, t, tile, out, s_buf[wv], s_stack[wv], s_rec[wv]);
+}
+
+// Several queries (object pairs) in one launch: block b walks tile
+// b - block0 of the query whose [block0, block0 + tiles) holds b (a
+// wave-uniform binary search over the table).
+template <int K>
+__global__ __launch_bounds__(256) void k_narrow64_batch(const BatchQuery* __restrict__ bq, uint32_t nq) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  __shared__ Rec64 s_rec[256 / kWave][K];
+  const uint32_t wv = threadIdx.x / kWave;
+  uint32_t lo = 0, hi = nq;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bq[mid].block0 <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const BatchQuery& e = bq[lo];
+  if (blockIdx.x - e.block0 >= e.blocks) return;
+  walk64<false, K, false>(e.q, e.t, blockIdx.x - e.block0, e.out, s_buf[wv], s_stack[wv], s_rec[wv]);
+}
+
 // fp64 form, one record per visit in scalar registers (batch 1): box 0
 // with the child ids and the cache lines of box 1 in one round of scalar
 // loads, then box 1 from the scalar cache; each box is tested with one
@@ -545,7 +599,7 @@ __global__ __launch_bounds__(256) void k_narrow64(TreeView q, TreeView t, int xc
 // right child of a two-child visit goes on the per-wave LDS stack.
 typedef __attribute__((address_space(4))) const double CDbl;
 
-template <bool SELF>
+template <bool SELF, bool CROSS = false>
 __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int xcd, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
   __shared__ uint32_t s_stack[256 / kWave][kStack];
@@ -580,6 +634,8 @@ __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int x
   const uint32_t tleaf0 = n - 1;
   const uint32_t lane = lane_id();
   const uint32_t p32 = valid ? (uint32_t)p : 0u;  // positions fit 32 bits; invalid lanes hit nothing
+  uint32_t pb = 0, pe = 0;
+  if (CROSS) own_patch_range(q, qa, valid, n, pb, pe);
   uint32_t steps = 0;
   uint32_t sp = 0;
   uint32_t x = kEnd;  // the node to visit next (kEnd: pop the stack)
@@ -614,6 +670,7 @@ __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int x
     const uint32_t cl = (uint32_t)ip[104], cr = (uint32_t)ip[105];
     const uint32_t el = (uint32_t)ip[106], er = (uint32_t)ip[107];
     const uint32_t rlast = SELF ? (uint32_t)ip[109] : 0u;
+    const uint32_t rfirst = CROSS ? (uint32_t)ip[108] : 0u;
     const uint32_t t0 = (uint32_t)ip[64], t1 = (uint32_t)ip[80];
     __builtin_amdgcn_sched_barrier(0);
     bool hl, hr;
@@ -624,6 +681,10 @@ __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int x
     if (SELF) {
       hl = hl & (x > p32);  // the left child ends at position x
       hr = hr & (rlast > p32);
+    }
+    if (CROSS) {  // children inside the lane's own patch
+      hl = hl & !((rfirst >= pb) & (x < pe));
+      hr = hr & !((x + 1 >= pb) & (rlast < pe));
     }
     const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
     const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
@@ -819,7 +880,8 @@ __global__ __launch_bounds__(256) void k_digest(const uint64_t* __restrict__ pai
 template <int K>
 static void narrow64_dispatch(const TreeView& q, const TreeView& t, int self, int xcd, unsigned blocks, unsigned bs,
                               const PairOut& out, hipStream_t s) {
-  if (self) hipLaunchKernelGGL((k_narrow64<true, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+  if (self == 2) hipLaunchKernelGGL((k_narrow64<true, K, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+  else if (self) hipLaunchKernelGGL((k_narrow64<true, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
   else hipLaunchKernelGGL((k_narrow64<false, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
 }
 
@@ -830,6 +892,7 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   const unsigned bs = shape.block;
   if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
   if (!q.perm || !t.perm || !t.root || !t.status || !out.pairs || !out.count) return hipErrorInvalidValue;
+  if (self == 2 && (!q.patch_of_input || (q.od > 1 && !q.splits))) return hipErrorInvalidValue;
   const unsigned blocks = (unsigned)((q.n + bs - 1) / bs);
   const int xcd = shape.xcd_remap ? 1 : 0;
   if (fp32) {
@@ -839,7 +902,8 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
     // one record per round trip: it arrives in scalar registers (the batch
     // applies to the fp64 walk)
-    if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+    if (self == 2) hipLaunchKernelGGL((k_narrow32<true, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+    else if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
     // in self mode both keys index the one object (q == t)
     uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
@@ -851,13 +915,21 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   if (!q.rec64 || !q.rootbox64 || !q.delta || !t.rec64 || !t.rootbox64) return hipErrorInvalidValue;
   switch (shape.batch) {
     case 1:
-      if (self) hipLaunchKernelGGL(k_narrow64s<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+      if (self == 2) hipLaunchKernelGGL((k_narrow64s<true, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+      else if (self) hipLaunchKernelGGL(k_narrow64s<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
       else hipLaunchKernelGGL(k_narrow64s<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
       break;
     case 2: narrow64_dispatch<2>(q, t, self, xcd, blocks, bs, out, s); break;
     case 4: narrow64_dispatch<4>(q, t, self, xcd, blocks, bs, out, s); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, hipStream_t s) {
+  if (nq == 0 || blocks == 0) return hipSuccess;
+  if (batch == 4) hipLaunchKernelGGL(k_narrow64_batch<4>, dim3(blocks), dim3(256), 0, s, bq, nq);
+  else hipLaunchKernelGGL(k_narrow64_batch<2>, dim3(blocks), dim3(256), 0, s, bq, nq);
   return hipGetLastError();
 }
 

@@ -411,6 +411,24 @@ hipError_t launch_gather_by(const T* src, const uint32_t* idx, uint64_t n, T* ds
 template hipError_t launch_gather_by<uint32_t>(const uint32_t*, const uint32_t*, uint64_t, uint32_t*, hipStream_t);
 template hipError_t launch_gather_by<uint64_t>(const uint64_t*, const uint32_t*, uint64_t, uint64_t*, hipStream_t);
 
+// Patches given by the caller: patch k holds input indices [given[k-1],
+// given[k]) (given[-1] = 0, given[od-1] = n); the reference order is the
+// input order.  given: od - 1 non-decreasing offsets (device).
+hipError_t launch_given_split(SplitBufs sb, uint64_t n, uint64_t od, const uint32_t* given, uint32_t* splits,
+                              uint32_t* patch_key, uint32_t** result, hipStream_t s) {
+  *result = sb.rp;
+  if (n == 0) return hipSuccess;
+  hipError_t e = launch_iota_u32(sb.rp, n, s);
+  if (e != hipSuccess) return e;
+  if (od > 1) {
+    e = launch_copy<uint32_t>(splits, given, od - 1, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_split_patch_key, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sb, n, splits,
+                     (uint32_t)(od - 1), patch_key);
+  return hipGetLastError();
+}
+
 hipError_t launch_split(const ElemInput& in, const SplitInput& si, SplitBufs sb, uint64_t od, int ml,
                         uint32_t* splits, uint32_t* patch_key, uint32_t** result, hipStream_t s) {
   const uint64_t n = si.n;

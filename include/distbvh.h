@@ -50,7 +50,10 @@ typedef enum bvh_status {
 typedef enum bvh_split_algorithm {
   BVH_SPLIT_GEOM_AXIS = 0,
   BVH_SPLIT_ML_GEOM_AXIS = 1,
-  BVH_SPLIT_CLUSTERING = 2
+  BVH_SPLIT_CLUSTERING = 2,
+  /* no reference counterpart: patches given by the caller
+   * (bvh_object_set_patch_offsets), e.g. one patch per mesh of a batch */
+  BVH_SPLIT_GIVEN = 3
 } bvh_split_algorithm;
 
 typedef enum bvh_memory { BVH_MEM_HOST = 0, BVH_MEM_DEVICE = 1 } bvh_memory;
@@ -126,6 +129,11 @@ bvh_status bvh_start_iteration(bvh_world w);
 bvh_status bvh_finish_iteration(bvh_world w);
 /* wait for all device work of the world */
 bvh_status bvh_synchronize(bvh_world w);
+/* the world's HIP stream (hipStream_t): callers that write element data on
+ * another stream make this one wait for that work (an event) before passing
+ * the data in (no reference counterpart: the reference's Kokkos views are
+ * ordered by Kokkos::fence) */
+bvh_status bvh_world_stream(bvh_world w, void **stream);
 /* Per-stage device times (ms) recorded (BVH_FLAG_TIMING) since the previous
  * call of this function, in call order; reading clears the record (at most
  * 512 stages are kept).  names[i] gets a static string. */
@@ -141,6 +149,14 @@ bvh_status bvh_object_id(bvh_object o, size_t *id);
 bvh_status bvh_set_entity_data_poly(bvh_object o, const double *verts, uint32_t verts_per_element,
                                     uint64_t n, const uint64_t *global_ids, double epsilon,
                                     uint32_t mem, uint32_t split_algorithm);
+/* Patches for set_entity_data(..., BVH_SPLIT_GIVEN): patch k holds the
+ * elements of input indices [offsets[k-1], offsets[k]) (offsets[-1] = 0,
+ * offsets[od-1] = n), count = od - 1 non-decreasing offsets (host memory),
+ * kept for the object's later builds.  Inside each patch the tree orders
+ * elements by Morton code, as for geom_axis.  No reference counterpart: the
+ * reference's split_algorithm (types.hpp:52-57) always computes its patches;
+ * a batch of independent meshes names its own. */
+bvh_status bvh_object_set_patch_offsets(bvh_object o, const uint64_t *offsets, size_t count);
 /* set_entity_data for sphere elements: kdop = from_sphere (kdop.hpp:273-288),
  * centroid = center.  xyzr: planes x, y, z, r. */
 bvh_status bvh_set_entity_data_sphere(bvh_object o, const double *xyzr, uint64_t n,
@@ -170,9 +186,26 @@ bvh_status bvh_broadphase(bvh_object o, bvh_object other);
  * run their own narrowphase per patch pair (the reference's narrowphase
  * functor, collision_world.hpp:75-94); the result count is then 0. */
 bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other);
+/* broadphase(other) for k pairs at once, objects[i] x others[i] (k distinct
+ * receiving objects, no object with itself): the coalesced step of a
+ * serving loop.  Each pair's patch broadphase is its own small launch; the
+ * element narrowphases of all pairs are one launch (fp64 form; the fp32 form
+ * and sort_results run the pairs one after another); one synchronization
+ * reads every pair's counts.  Results land in objects[i] as after
+ * broadphase.  No reference counterpart (the reference runs one broadphase
+ * per call, collision_object.hpp:170). */
+bvh_status bvh_broadphase_batch(bvh_object *objects, bvh_object *others, uint64_t k);
 /* self_collision_set(tree), collision_query.hpp:360-368 at element level:
  * unordered element pairs (i < j) of one object with overlapping 26-DOPs. */
 bvh_status bvh_self_collision(bvh_object o);
+/* self_collision restricted to pairs whose elements lie in different
+ * patches (i < j): for an object holding a batch of independent meshes, one
+ * patch per mesh (BVH_SPLIT_GIVEN), the pairs between meshes -- what the
+ * reference's broadphase gives between separate collision objects, whose
+ * narrowphase never pairs an object with itself
+ * (collision_object/impl.cpp:229-235).  The walk skips every subtree inside
+ * the querying element's own patch. */
+bvh_status bvh_self_collision_cross_patch(bvh_object o);
 /* end_phase(), collision_object.hpp:172 */
 bvh_status bvh_end_phase(bvh_object o);
 
@@ -269,6 +302,32 @@ bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 bvh_status bvh_closest_point(bvh_world w, const double *points, uint64_t n, const double *tri,
                              double *closest, uint32_t mem);
 
+/* Exact primitive tests beyond k-DOP overlap.  The reference's primitive
+ * layer is the triangle type (primitive/triangle.hpp:40-52) and closest_point
+ * (primitive/kokkos_closest_point.hpp:55-133); it has no intersection test,
+ * so these state one rule: closed sets (touching counts as intersecting);
+ * every decision is the sign of an fp64 expression evaluated in a fixed
+ * order (orient3d / orient2d determinants, squared distances), so ties are
+ * broken the same way on every run (on integer coordinates up to 2^15 every
+ * determinant is exact and so is the answer); a triangle whose fp64 normal is
+ * exactly zero meets nothing.  Triangle-triangle: plane-side rejection, then
+ * the edge-crossing formulation (an edge of one triangle meets the other)
+ * with orient3d sign tests, coplanar cases in 2D.
+ * n pairs: p, q are n x 9 doubles (triangle vertices a, b, c) or n x 4
+ * (x, y, z, r), out[i] = 1 if pair i intersects; all in `mem`. */
+bvh_status bvh_triangle_pairs_intersect(bvh_world w, const double *p, const double *q, uint64_t n, uint32_t mem,
+                                        uint8_t *out);
+bvh_status bvh_sphere_pairs_intersect(bvh_world w, const double *a, const double *b, uint64_t n, uint32_t mem,
+                                      uint8_t *out);
+/* The primitive narrowphase on the object's results: keeps the pairs (of the
+ * last broadphase / self_collision) whose triangles (3-vertex polytope
+ * elements on both objects) or spheres intersect by the rule above, in their
+ * order; the records and counts then describe the kept pairs.  *kept (may be
+ * NULL) is their number.  The element data must still be valid.  No
+ * reference counterpart: it is a narrowphase functor's job in the reference
+ * (collision_world.hpp:75-94). */
+bvh_status bvh_result_filter_exact(bvh_object o, uint64_t *kept);
+
 /* kdop::from_vertices (kdop.hpp:238-264) for n polytopes: SoA vertex planes
  * (as for set_entity_data) in, n x 26 k-DOPs (min, max per direction) out,
  * both in `mem`; the same values the build computes.  For callers that sweep
@@ -297,6 +356,55 @@ bvh_status bvh_sort_and_sweep(bvh_world w, const double *pa, const double *ka, c
                               const double *pb, const double *kb, const uint64_t *gb, uint64_t nb, int32_t axis,
                               uint32_t mem, uint64_t *pairs, uint64_t cap, uint64_t *npairs);
 
+/* ---- wire formats (the byte layouts the reference sends between ranks) ------------
+ * Packed and unpacked on the device (or from / to host memory), for a
+ * multi-GPU driver to move with RCCL.  Each layout is the field order of the
+ * reference's serialize() with every field as its in-memory bytes (DARMA
+ * checkpoint copies byte-copyable types whole,
+ * serialization/bvh_serialize.hpp:66-79); checkpoint's own message framing
+ * is not reproduced. */
+typedef enum bvh_element_kind {
+  BVH_ELEM_POLY = 0,     /* polytope vertices (3 * verts_per_element doubles) */
+  BVH_ELEM_SPHERE = 1,   /* x, y, z, r */
+  BVH_ELEM_SNAPSHOT = 2  /* 26 k-DOP slabs, then 3 centroid coordinates */
+} bvh_element_kind;
+#define BVH_PATCH_RECORD_BYTES 256u
+#define BVH_RESULT_RECORD_BYTES 32u
+/* patch<>::serialize (patch.hpp:93-97): per patch of o, in patch order, one
+ * 256-byte record: u64 global_id, u64 size, the 13 extents {f64 min, f64 max}
+ * of the 26-DOP (kdop.hpp:57-60; an empty patch keeps {DBL_MAX, -DBL_MAX}),
+ * the centroid as vec3<double> storage (4 doubles: x, y, z, 0;
+ * math/storage/vec_storage.hpp:64-71).  *bytes = od * 256; out may be NULL to
+ * ask for it.  bvh_unpack_patches reads such records back (*count of them). */
+bvh_status bvh_pack_patches(bvh_object o, void *out, uint64_t cap, uint64_t *bytes, uint32_t mem);
+bvh_status bvh_unpack_patches(bvh_world w, const void *in, uint64_t bytes, uint32_t mem, bvh_patch_info *out,
+                              uint64_t cap, uint64_t *count);
+/* The element payload of patches (the bytes of a patch a rank sends to the
+ * ranks that ghost it: narrowphase_patch_collection_type::bytes,
+ * collision_object/types.hpp:187-201, gathered in patch order as
+ * prepare_local_patch_for_sending does, collision_object/impl.hpp:90-124):
+ * for each listed patch in list order, each element in patch order, one
+ * record of `stride` bytes: u64 global id, then the element's doubles
+ * (bvh_element_kind).  bvh_element_record_bytes gives the stride of o's
+ * elements; out may be NULL to ask for *bytes.  At most 65535 patches. */
+bvh_status bvh_element_record_bytes(bvh_object o, uint64_t *stride);
+bvh_status bvh_pack_patch_elements(bvh_object o, const uint64_t *patches, uint64_t npatches, void *out,
+                                   uint64_t cap, uint64_t *bytes, uint32_t mem);
+/* count element records back into set_entity_data's layout: SoA planes
+ * (BVH_ELEM_POLY: 3 * verts_per_element planes; BVH_ELEM_SPHERE: x, y, z, r
+ * planes) or, for snapshots, count x 26 k-DOPs and count x 3 centroids;
+ * gids (may be NULL) gets the global ids.  All arrays in `mem`. */
+bvh_status bvh_unpack_elements(bvh_world w, const void *in, uint64_t count, uint32_t kind,
+                               uint32_t verts_per_element, double *data, double *centroids, uint64_t *gids,
+                               uint32_t mem);
+/* result_msg::serialize (collision_object/types.hpp:109-138): u64 stride
+ * (32), u64 byte count, then the 32-byte records of o's results.  *bytes =
+ * 16 + 32 * results; out may be NULL to ask for it.  bvh_unpack_results
+ * checks the stride and byte count and copies the records out (*count). */
+bvh_status bvh_pack_results(bvh_object o, void *out, uint64_t cap, uint64_t *bytes, uint32_t mem);
+bvh_status bvh_unpack_results(bvh_world w, const void *in, uint64_t bytes, uint32_t mem, bvh_narrowphase_record *out,
+                              uint64_t cap, uint64_t *count);
+
 /* ---- autotuner ------------------------------------------------------------------ */
 typedef struct bvh_tuning {
   uint32_t batch;       /* fp64 walk: 1 = one record per visit in scalar registers, 2 (default)
@@ -307,7 +415,8 @@ typedef struct bvh_tuning {
   double ms;            /* best query time */
   double tried_ms[24];  /* in the order batch (1, 2, 4) x block (64, 128, 256) x remap (off, on) */
 } bvh_tuning;
-/* Times the query o x other (self collision when o == other) over the batch
+/* Times the query o x other (self collision when o == other, cross-patch if
+ * that was o's last query) over the batch
  * (1, 2, 4; fp64 form only: the fp32 walk takes one record per round trip),
  * block size (64, 128, 256) and XCD remap (off, on), best of 3 runs each,
  * keeps the fastest setting for the world and leaves that query's results in

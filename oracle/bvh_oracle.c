@@ -1106,3 +1106,158 @@ orc_pairs *orc_collide(const orc_object *a, const orc_object *b, const orc_pipel
   }
   return res;
 }
+
+/* ---- exact primitive tests (no reference counterpart) --------------------------------
+ * The reference's primitive layer is the triangle type
+ * (primitive/triangle.hpp:40-52) and closest_point; it has no intersection
+ * test.  The rule stated here is the GPU primitive narrowphase's:
+ *  * closed sets: triangles (or balls) that only touch intersect;
+ *  * every decision is the sign of an fp64 expression evaluated in the order
+ *    written (no contraction), so the CPU and the GPU break ties (zero
+ *    determinants, equal distances) identically.  For integer coordinates of
+ *    magnitude <= 2^15 every determinant below is exact, so the rule is then
+ *    the exact geometric answer;
+ *  * a triangle whose fp64 normal (b - a) x (c - a) is exactly zero is
+ *    degenerate and intersects nothing.
+ * Triangle-triangle: the two plane-side tests first (all three vertices of
+ * one triangle strictly on one side of the other's plane: disjoint); then
+ * the edge-crossing formulation -- two closed triangles meet iff an edge of
+ * one meets the other triangle -- with each segment-triangle test made of
+ * orient3d signs (the line through the segment crosses the triangle iff the
+ * three signed volumes of the segment with the triangle's edges agree in
+ * sign, zeros allowed) and, for a segment in the triangle's plane or for
+ * coplanar triangles, orient2d tests in the coordinate plane that drops the
+ * largest component of the normal. */
+typedef struct { double x, y, z; } orc_v3;
+
+static orc_v3 orc_v(const double *p) {
+  orc_v3 r = {p[0], p[1], p[2]};
+  return r;
+}
+
+/* sign source: det [a - d; b - d; c - d] */
+static double orc_orient3d(orc_v3 a, orc_v3 b, orc_v3 c, orc_v3 d) {
+  const double adx = a.x - d.x, ady = a.y - d.y, adz = a.z - d.z;
+  const double bdx = b.x - d.x, bdy = b.y - d.y, bdz = b.z - d.z;
+  const double cdx = c.x - d.x, cdy = c.y - d.y, cdz = c.z - d.z;
+  return adx * (bdy * cdz - bdz * cdy) + bdx * (cdy * adz - cdz * ady) + cdx * (ady * bdz - adz * bdy);
+}
+
+static orc_v3 orc_normal3(orc_v3 a, orc_v3 b, orc_v3 c) {
+  const double ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+  const double vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+  orc_v3 n = {uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx};
+  return n;
+}
+
+/* the axis dropped by the 2D tests: largest |n| component, the first on ties */
+static int orc_drop_axis(orc_v3 n) {
+  const double ax = fabs(n.x), ay = fabs(n.y), az = fabs(n.z);
+  if (ax >= ay && ax >= az) return 0;
+  return ay >= az ? 1 : 2;
+}
+
+typedef struct { double u, v; } orc_v2;
+
+static orc_v2 orc_proj(orc_v3 p, int drop) {
+  orc_v2 r;
+  if (drop == 0) { r.u = p.y; r.v = p.z; }
+  else if (drop == 1) { r.u = p.x; r.v = p.z; }
+  else { r.u = p.x; r.v = p.y; }
+  return r;
+}
+
+static double orc_orient2d(orc_v2 a, orc_v2 b, orc_v2 c) {
+  return (a.u - c.u) * (b.v - c.v) - (a.v - c.v) * (b.u - c.u);
+}
+
+/* p on the closed segment [a, b], given that the three are collinear */
+static int orc_on_seg(orc_v2 a, orc_v2 b, orc_v2 p) {
+  const double ulo = a.u < b.u ? a.u : b.u, uhi = a.u < b.u ? b.u : a.u;
+  const double vlo = a.v < b.v ? a.v : b.v, vhi = a.v < b.v ? b.v : a.v;
+  return ulo <= p.u && p.u <= uhi && vlo <= p.v && p.v <= vhi;
+}
+
+/* closed segments [p1, p2] and [q1, q2] in the plane */
+static int orc_seg_seg2(orc_v2 p1, orc_v2 p2, orc_v2 q1, orc_v2 q2) {
+  const double d1 = orc_orient2d(q1, q2, p1), d2 = orc_orient2d(q1, q2, p2);
+  const double d3 = orc_orient2d(p1, p2, q1), d4 = orc_orient2d(p1, p2, q2);
+  if (((d1 > 0 && d2 < 0) || (d1 < 0 && d2 > 0)) && ((d3 > 0 && d4 < 0) || (d3 < 0 && d4 > 0))) return 1;
+  if (d1 == 0 && orc_on_seg(q1, q2, p1)) return 1;
+  if (d2 == 0 && orc_on_seg(q1, q2, p2)) return 1;
+  if (d3 == 0 && orc_on_seg(p1, p2, q1)) return 1;
+  if (d4 == 0 && orc_on_seg(p1, p2, q2)) return 1;
+  return 0;
+}
+
+/* p in the closed triangle t (plane) */
+static int orc_in_tri2(const orc_v2 *t, orc_v2 p) {
+  const double d1 = orc_orient2d(t[0], t[1], p), d2 = orc_orient2d(t[1], t[2], p), d3 = orc_orient2d(t[2], t[0], p);
+  return (d1 >= 0 && d2 >= 0 && d3 >= 0) || (d1 <= 0 && d2 <= 0 && d3 <= 0);
+}
+
+/* closed segment [a, b] lying in the plane of triangle t, in 2D */
+static int orc_seg_tri2(orc_v3 a, orc_v3 b, const orc_v3 *t, int drop) {
+  const orc_v2 a2 = orc_proj(a, drop), b2 = orc_proj(b, drop);
+  const orc_v2 t2[3] = {orc_proj(t[0], drop), orc_proj(t[1], drop), orc_proj(t[2], drop)};
+  if (orc_in_tri2(t2, a2) || orc_in_tri2(t2, b2)) return 1;
+  for (int i = 0; i < 3; ++i)
+    if (orc_seg_seg2(a2, b2, t2[i], t2[(i + 1) % 3])) return 1;
+  return 0;
+}
+
+/* closed segment [a, b] against the closed (non-degenerate) triangle t */
+static int orc_seg_tri(orc_v3 a, orc_v3 b, const orc_v3 *t, int drop) {
+  const double oa = orc_orient3d(t[0], t[1], t[2], a), ob = orc_orient3d(t[0], t[1], t[2], b);
+  if ((oa > 0 && ob > 0) || (oa < 0 && ob < 0)) return 0;
+  if (oa == 0 && ob == 0) return orc_seg_tri2(a, b, t, drop);
+  const double s1 = orc_orient3d(a, b, t[0], t[1]), s2 = orc_orient3d(a, b, t[1], t[2]);
+  const double s3 = orc_orient3d(a, b, t[2], t[0]);
+  return (s1 >= 0 && s2 >= 0 && s3 >= 0) || (s1 <= 0 && s2 <= 0 && s3 <= 0);
+}
+
+int orc_tri_tri(const double *pp, const double *qq) {
+  const orc_v3 P[3] = {orc_v(pp), orc_v(pp + 3), orc_v(pp + 6)};
+  const orc_v3 Q[3] = {orc_v(qq), orc_v(qq + 3), orc_v(qq + 6)};
+  const orc_v3 np = orc_normal3(P[0], P[1], P[2]), nq = orc_normal3(Q[0], Q[1], Q[2]);
+  if ((np.x == 0 && np.y == 0 && np.z == 0) || (nq.x == 0 && nq.y == 0 && nq.z == 0)) return 0;
+  double dq[3], dp[3];
+  for (int i = 0; i < 3; ++i) dq[i] = orc_orient3d(P[0], P[1], P[2], Q[i]);
+  if ((dq[0] > 0 && dq[1] > 0 && dq[2] > 0) || (dq[0] < 0 && dq[1] < 0 && dq[2] < 0)) return 0;
+  for (int i = 0; i < 3; ++i) dp[i] = orc_orient3d(Q[0], Q[1], Q[2], P[i]);
+  if ((dp[0] > 0 && dp[1] > 0 && dp[2] > 0) || (dp[0] < 0 && dp[1] < 0 && dp[2] < 0)) return 0;
+  const int dropp = orc_drop_axis(np), dropq = orc_drop_axis(nq);
+  if ((dq[0] == 0 && dq[1] == 0 && dq[2] == 0) || (dp[0] == 0 && dp[1] == 0 && dp[2] == 0)) {
+    /* coplanar: an edge crossing, or one triangle inside the other (P's plane) */
+    orc_v2 p2[3], q2[3];
+    for (int i = 0; i < 3; ++i) {
+      p2[i] = orc_proj(P[i], dropp);
+      q2[i] = orc_proj(Q[i], dropp);
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        if (orc_seg_seg2(p2[i], p2[(i + 1) % 3], q2[j], q2[(j + 1) % 3])) return 1;
+    return orc_in_tri2(q2, p2[0]) || orc_in_tri2(p2, q2[0]);
+  }
+  for (int i = 0; i < 3; ++i)
+    if (orc_seg_tri(P[i], P[(i + 1) % 3], Q, dropq)) return 1;
+  for (int i = 0; i < 3; ++i)
+    if (orc_seg_tri(Q[i], Q[(i + 1) % 3], P, dropp)) return 1;
+  return 0;
+}
+
+int orc_sphere_sphere(const double *a, const double *b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  const double rs = a[3] + b[3];
+  return dx * dx + dy * dy + dz * dz <= rs * rs;
+}
+
+void orc_tri_tri_batch(const double *p, const double *q, size_t n, uint8_t *out) {
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) out[i] = (uint8_t)orc_tri_tri(p + 9 * i, q + 9 * i);
+}
+
+void orc_sphere_sphere_batch(const double *a, const double *b, size_t n, uint8_t *out) {
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) out[i] = (uint8_t)orc_sphere_sphere(a + 4 * i, b + 4 * i);
+}

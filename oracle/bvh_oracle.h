@@ -22,6 +22,8 @@
  *   patches, top-down tree, query_tree, self/potential collision sets,
  *   sort_and_sweep, pipeline: CollisionTest.cpp / KokkosTest.cpp counts and
  *       brute-force all-pairs overlap (the pair set is partition invariant).
+ *   triangle-triangle, sphere-sphere: separating-axis tests in exact rational
+ *       arithmetic and closed-form configurations (no reference counterpart).
  */
 #ifndef BVH_ORACLE_H
 #define BVH_ORACLE_H
@@ -147,6 +149,22 @@ typedef struct {
 
 /* bvh::kokkos::closest_point: n x 3 points, tri = a, b, c (9 doubles) */
 void orc_closest_point(const double *points, size_t n, const double *tri, double *out);
+
+/* ---- exact primitive tests beyond k-DOP overlap ----------------------------
+ * The reference has a triangle type (primitive/triangle.hpp:40-52) and the
+ * closest point kernel, but no primitive intersection test; these state the
+ * rule the GPU's primitive narrowphase implements (closed sets: touching
+ * counts as intersecting; fp64 evaluation in a fixed order, so CPU and GPU
+ * decide ties identically; a triangle whose fp64 normal is exactly zero is
+ * degenerate and intersects nothing).  Pinned against exact rational
+ * arithmetic (separating axes over Fractions) on integer and random inputs
+ * and against closed-form configurations (tests/test_oracle_primitive.py).
+ * tri: 9 doubles a, b, c; sphere: 4 doubles x, y, z, r. */
+int orc_tri_tri(const double *p, const double *q);
+int orc_sphere_sphere(const double *a, const double *b);
+/* n pairs: p, q are n x 9 (n x 4), out[i] = 0 / 1 */
+void orc_tri_tri_batch(const double *p, const double *q, size_t n, uint8_t *out);
+void orc_sphere_sphere_batch(const double *a, const double *b, size_t n, uint8_t *out);
 
 /* split_permutations / split_permutations_ml with split::mean and
  * axis::longest: idx (n) and 2^depth - 1 split offsets */

@@ -108,6 +108,10 @@ def lib():
             "orc_now": (C.c_double, []),
             "orc_split_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
             "orc_closest_point": (None, [dp, C.c_size_t, dp, dp]),
+            "orc_tri_tri": (C.c_int, [dp, dp]),
+            "orc_sphere_sphere": (C.c_int, [dp, dp]),
+            "orc_tri_tri_batch": (None, [dp, dp, C.c_size_t, C.POINTER(C.c_uint8)]),
+            "orc_sphere_sphere_batch": (None, [dp, dp, C.c_size_t, C.POINTER(C.c_uint8)]),
             "orc_split_ml_geom_axis": (None, [dp, dp, C.c_size_t, C.c_int, u32p, u64p]),
         }
         for name, (res, args) in sig.items():
@@ -260,6 +264,27 @@ def closest_point(points, tri):
     out = np.empty_like(p)
     if len(p):
         lib().orc_closest_point(_p(p, dp), len(p), _p(t, dp), _p(out, dp))
+    return out
+
+
+def tri_tri(p, q):
+    """Closed triangles p, q ((n, 9) or (n, 3, 3): a, b, c) intersect: (n,)
+    uint8 (the rule of orc_tri_tri, no reference counterpart)."""
+    p = _c(np.asarray(p, np.float64).reshape(-1, 9), np.float64)
+    q = _c(np.asarray(q, np.float64).reshape(-1, 9), np.float64)
+    out = np.zeros(len(p), np.uint8)
+    if len(p):
+        lib().orc_tri_tri_batch(_p(p, dp), _p(q, dp), len(p), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def sphere_sphere(a, b):
+    """Closed balls a, b ((n, 4): x, y, z, r) intersect: (n,) uint8."""
+    a = _c(np.asarray(a, np.float64).reshape(-1, 4), np.float64)
+    b = _c(np.asarray(b, np.float64).reshape(-1, 4), np.float64)
+    out = np.zeros(len(a), np.uint8)
+    if len(a):
+        lib().orc_sphere_sphere_batch(_p(a, dp), _p(b, dp), len(a), out.ctypes.data_as(C.POINTER(C.c_uint8)))
     return out
 
 

@@ -214,3 +214,86 @@ def test_launch_shapes_full_size(torch_cuda, precision):
             st = oa.result_stats()
             seen.add((oa.result_count(), oa.digest(), st["node_visits"]))
         assert len(seen) == 1, seen
+
+
+# ---- patches given by the batch, pairs between patches ---------------------------------------
+def _mesh_of(counts):
+    return np.repeat(np.arange(len(counts)), counts)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_given_patches_cross_patch_pairs(torch_cuda, precision):
+    """One patch per mesh of the batch (BVH_SPLIT_GIVEN): the patches are the
+    meshes, plain self collision is brute force, and cross-patch self
+    collision is brute force without the pairs inside one mesh."""
+    from distbvh_amd import GIVEN
+    a, counts = W.patch_batch(npatch=64, median=256, lo=16, hi=2048, lattice=4, seed=0xC6)
+    kd, _ = O.elements_poly(a)
+    allp = sort_pairs(O.brute_self(kd))
+    mesh = _mesh_of(counts)
+    cross = allp[mesh[allp[:, 0].astype(np.int64)] != mesh[allp[:, 1].astype(np.int64)]]
+    assert 0 < len(cross) < len(allp)
+    offsets = np.cumsum(counts)[:-1]
+    with CollisionWorld(64, precision=precision) as w:
+        o = w.create_collision_object()
+        o.set_patch_offsets(offsets)
+        o.set_entity_data_poly(to_cuda(a), split_algorithm=GIVEN)
+        o.init_broadphase()
+        pats = o.local_patches()
+        assert [p["size"] for p in pats] == [int(c) for c in counts]
+        starts = np.concatenate([[0], offsets])
+        for k in (0, 17, 63):
+            assert np.array_equal(o.patch_elements(k), np.arange(starts[k], starts[k] + counts[k], dtype=np.uint64))
+        o.self_collision()
+        assert np.array_equal(sort_pairs(o.pairs()), allp)
+        o.self_collision_cross_patch()
+        got = sort_pairs(o.pairs())
+        stats_cross = o.result_stats()
+    assert np.array_equal(got, cross)
+    if precision == "fp32":
+        # the vertex-sharing pairs of a mesh are never candidates in cross-patch mode
+        assert stats_cross["rechecked_pairs"] < len(allp) - len(cross)
+
+
+def test_cross_patch_graph_replays(torch_cuda):
+    """The cross-patch step captured in a graph and replayed on new heights:
+    every replay is the brute-force cross-mesh set of its data."""
+    torch = torch_cuda
+    from distbvh_amd import GIVEN
+    a0, counts = W.patch_batch(npatch=64, median=256, lo=16, hi=2048, lattice=4, seed=0xC7)
+    n = a0.shape[2]
+    mesh = _mesh_of(counts)
+    datas = [a0]
+    for k in range(2):
+        d = a0.copy()
+        d[:, 2, :] += 0.003 * np.cos(np.arange(n) * (0.11 + k))[None, :]
+        datas.append(d)
+
+    def want(d):
+        kd, _ = O.elements_poly(d)
+        p = sort_pairs(O.brute_self(kd))
+        return p[mesh[p[:, 0].astype(np.int64)] != mesh[p[:, 1].astype(np.int64)]]
+
+    wants = [want(d) for d in datas]
+    buf = to_cuda(a0)
+    with CollisionWorld(64) as w:
+        o = w.create_collision_object()
+        o.set_patch_offsets(np.cumsum(counts)[:-1])
+
+        def step():
+            o.set_entity_data_poly(buf, split_algorithm=GIVEN)
+            o.init_broadphase()
+            o.self_collision_cross_patch()
+
+        step()
+        with w.capture() as g:
+            step()
+        try:
+            for d, wp in zip(datas, wants):
+                buf.copy_(to_cuda(d))
+                torch.cuda.synchronize()
+                g.launch()
+                assert o.result_count() == len(wp)
+                assert o.digest() == pair_digest(wp)
+        finally:
+            g.close()

@@ -156,3 +156,74 @@ def test_fp32_mesh_self_collision_recheck_stripes(torch_cuda):
     assert np.array_equal(first, brute)
     assert np.array_equal(second, brute)
     assert np.array_equal(got64, brute)
+
+
+# ---- the precision workload (16,777,216 spheres, r = 1e-3, 64-component mixture) ------------
+def test_precision_workload_generator_against_brute_force(torch_cuda):
+    """The precision workload's generator exactly as configured (64
+    components, sigma 0.01, r 1e-3, cloud translated by (1e4, -1e4, 1e4)),
+    at 32,768 spheres: self collision in the fp32-outward and fp64 forms,
+    both equal to the oracle's brute-force pairs."""
+    s = W.sphere_cloud(32768)  # defaults: ncomp=64, sigma=0.01, radius=1e-3, shift=(1e4, -1e4, 1e4)
+    kd, _ = O.elements_sphere(s)
+    brute = sort_pairs(O.brute_self(kd))
+    assert len(brute) > 0
+    for prec in ("fp32", "fp64"):
+        with CollisionWorld(64, precision=prec) as w:
+            o = gpu_object(w, "sphere", s, device=True)
+            o.self_collision()
+            got = sort_pairs(o.pairs())
+            stats = o.result_stats()
+        assert np.array_equal(got, brute), prec
+        if prec == "fp32":
+            assert stats["candidate_pairs"] >= len(brute)
+            assert 0.0 < stats["max_slab_error_ulps"] <= 2.0
+
+
+def test_precision_workload_full_size(torch_cuda):
+    """16,777,216 spheres, self collision: the fp32-outward form's pair count
+    and digest equal the fp64 form's, the largest fp32 slab error is at most
+    2 ulps and every fp64 pair was a candidate."""
+    import torch
+    s = W.sphere_cloud()
+    ts = torch.from_numpy(s).cuda()
+    del s
+    res = {}
+    for prec in ("fp64", "fp32"):
+        with CollisionWorld(64, precision=prec) as w:
+            o = w.create_collision_object()
+            o.set_entity_data_sphere(ts)
+            o.init_broadphase()
+            o.self_collision()
+            res[prec] = (o.result_count(), o.digest(), o.result_stats())
+        torch.cuda.empty_cache()
+    n64, d64, _ = res["fp64"]
+    n32, d32, st = res["fp32"]
+    print("precision workload: %d pairs, fp32 candidates %d, re-checked %d, excess false-positive rate %.3g, "
+          "max slab error %.3f ulps" % (n64, st["candidate_pairs"], st["rechecked_pairs"],
+                                        (st["candidate_pairs"] - n64) / max(n64, 1), st["max_slab_error_ulps"]))
+    assert n64 > 10 ** 9
+    assert (n32, d32) == (n64, d64)
+    assert st["candidate_pairs"] >= n64
+    assert 0.0 < st["max_slab_error_ulps"] <= 2.0
+
+
+def test_batch_workload_full_size_fp32_matches_fp64(torch_cuda):
+    """The 4,096-mesh batch (lognormal sizes, ~19M triangles) in one object,
+    self collision: the fp32-outward form's pairs (count and digest) are the
+    fp64 form's."""
+    import torch
+    a, counts = W.patch_batch()
+    ta = torch.from_numpy(a).cuda()
+    del a
+    res = {}
+    for prec in ("fp64", "fp32"):
+        with CollisionWorld(4096, precision=prec) as w:
+            o = w.create_collision_object()
+            o.set_entity_data_poly(ta)
+            o.init_broadphase()
+            o.self_collision()
+            res[prec] = (o.result_count(), o.digest())
+        torch.cuda.empty_cache()
+    assert res["fp64"][0] > 0
+    assert res["fp32"] == res["fp64"]
