@@ -33,9 +33,10 @@ HBM_PEAK_GBS = 8000.0
 def kernel_bytes(stage, n, npairs, n_other, fp32=False, elem_bytes=72):
     """Algorithmic (compulsory) HBM bytes of one launch of a stage for an
     object of n elements (a triangle: 3 fp64 vertices = 72 B).  A node's 13
-    (min, max) slab pairs take 208 B in fp64 and 104 B in the fp32 form; the
-    tree is one record per internal node holding both children's slabs plus
-    16 B of links (448 B fp64, 256 B fp32 with padding)."""
+    (min, max) slab pairs take 208 B in fp64 and 104 B in fp32; the tree is
+    one record per internal node holding both children's slabs plus links
+    (448 B with fp64 slabs, 256 B with fp32-outward slabs: the fp64 and fp32
+    forms; `fp32` names the record format)."""
     slab = 104 if fp32 else 208
     rec = 256 if fp32 else 448
     if stage == "bounds":      # read elements, write centroid
@@ -66,8 +67,9 @@ STAGE_KERNEL = {"tree": ("k_tree_leaves<0, 3, false>", "k_tree_leaves<0, 3, true
 
 
 def stage_kernel(stage, fp32, launch):
-    """The stage's main kernel as rocprofv3 names it (the fp64 walk depends
-    on the launch batch: one record in scalar registers, or K staged in LDS)."""
+    """The stage's main kernel as rocprofv3 names it (`fp32`: fp32-outward
+    records, the fp64 and fp32 forms; the walk over fp64 records depends on
+    the launch batch: one record in scalar registers, or K staged in LDS)."""
     if stage == "narrowphase":
         if fp32:
             return "k_narrow32<false>"
@@ -76,11 +78,12 @@ def stage_kernel(stage, fp32, launch):
     return STAGE_KERNEL.get(stage, (None, None))[1 if fp32 else 0]
 
 
-def measured_traffic(stage, workload, fp32, launch=None):
+def measured_traffic(stage, workload, form, launch=None):
     """HBM bytes per launch of the stage's main kernel from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE table under profiles/ (None if absent)."""
-    name = stage_kernel(stage, fp32, launch)
-    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (workload, "fp32" if fp32 else "fp64"))
+    FETCH_SIZE / WRITE_SIZE table of the form under profiles/ (None if
+    absent)."""
+    name = stage_kernel(stage, form != "fp64_records", launch)
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (workload, form))
     if name is None or not os.path.exists(path):
         return None
     for line in open(path):
@@ -281,7 +284,7 @@ def bench_sheets(args, torch, stream):
     tb = torch.from_numpy(b).cuda()
     out = {}
     for prec in args.precisions:
-        fp32 = prec == "fp32"
+        fp32 = prec != "fp64_records"  # fp32-outward records
         world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
                                stream=stream.cuda_stream, precision=prec, serial_builds=args.serial_builds)
         oa = world.create_collision_object()
@@ -324,7 +327,7 @@ def bench_sheets(args, torch, stream):
         # builds of A and B overlap, so their stage times are wall times)
         roof, step_frac = roofline_of(stages, per_launch, counts,
                                       lambda k: kernel_bytes(k, n, npairs, n, fp32), ms,
-                                      (lambda k: measured_traffic(k, wl, fp32, tune)) if wl else None,
+                                      (lambda k: measured_traffic(k, wl, prec, tune)) if wl else None,
                                       dom="narrowphase")
         roof["kernel_name"] = stage_kernel(roof["kernel"], fp32, tune)
         # after the timed steps: the eager step once more, then its 32-byte
@@ -354,7 +357,7 @@ def bench_spheres_self(args, torch, stream):
     del s
     out = {}
     for prec in args.precisions:
-        fp32 = prec == "fp32"
+        fp32 = prec != "fp64_records"  # fp32-outward records
         world = CollisionWorld(args.od, morton_bits=63, leaf_size=args.leaf, timing=True,
                                stream=stream.cuda_stream, precision=prec)
         o = world.create_collision_object()
@@ -440,10 +443,10 @@ def bench_batch(args, torch, stream):
     n = a.shape[2]
     ta = torch.from_numpy(a).cuda()
     offsets = np.cumsum(counts)[:-1]
-    # fp32-outward k-DOPs unless a precision is named; the other form runs too
+    # fp32-outward k-DOPs unless a precision is named; the other forms run too
     prec = args.precisions[0] if args.precision_given and args.precision != "both" else "fp32"
     forms = {}
-    for p in ([prec, "fp64" if prec == "fp32" else "fp32"]):
+    for p in [prec] + [f for f in ("fp32", "fp64", "fp64_records") if f != prec]:
         world = CollisionWorld(4096, morton_bits=63, timing=False, stream=stream.cuda_stream, precision=p)
         o = world.create_collision_object()
         o.set_patch_offsets(offsets)
@@ -486,7 +489,7 @@ def bench_batch(args, torch, stream):
     return {"metric": "ms/step batch build + patch-vs-patch collision, 4096 meshes (%d triangles), one HIP graph" % n,
             "value": round(ms_graph, 4), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_graph, 4), "higher_is_better": False,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if prec == "fp32" else "f64",
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == "fp64_records" else "f32+f64",
             "data": "synthetic",
             "config": {"workload": "4,096 height-field meshes on a 16^3 lattice, lognormal sizes (median "
                                    "2,048, clipped to [64, 65,536]), 10%% neighbour overlap; %d triangles; one "
@@ -506,6 +509,8 @@ def bench_batch(args, torch, stream):
                           "pairs": v["npairs"]} for p, v in forms.items()},
             "fp32_pairs_equal_fp64": bool(forms["fp32"]["npairs"] == f64["npairs"] and
                                           forms["fp32"]["digest"] == f64["digest"]),
+            "fp64_records_pairs_equal_fp64": bool(forms["fp64_records"]["npairs"] == f64["npairs"] and
+                                                  forms["fp64_records"]["digest"] == f64["digest"]),
             "fp64_oracle_sample": sample,
             "cpu_baseline": None}
 
@@ -532,9 +537,11 @@ def main():
                     help="leaf size of the production workload (4), used by the CPU oracle's element "
                          "trees; the GPU trees hold one element per leaf and test leaves from their "
                          "parents' records, so it does not change their work")
-    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both",
-                    help="fp64 tree, the fp32-outward tree with an exact fp64 re-check, or both "
-                         "(the line reports the first; 'both' adds the fp32/fp64 comparison)")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "fp64_records", "both"], default="both",
+                    help="fp64: fp32-outward bounds in the tree, every pair decided in fp64 (the "
+                         "default form); fp32: the fp32-outward form (pairs fp32 proves are kept, the "
+                         "rest decided in fp64); fp64_records: fp64 slabs in every tree record; both: "
+                         "all three (the line reports fp64 and compares the others)")
     ap.add_argument("--split", choices=["clustering", "geom_axis", "ml_geom_axis"], default="clustering",
                     help="sheets: the split_algorithm of set_entity_data (patches)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -553,7 +560,7 @@ def main():
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     args.precision_given = any(a == "--precision" or a.startswith("--precision=") for a in sys.argv[1:])
-    args.precisions = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    args.precisions = ["fp64", "fp32", "fp64_records"] if args.precision == "both" else [args.precision]
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this library has no multi-GPU path", file=sys.stderr)
@@ -575,7 +582,8 @@ def main():
                          "10k-2M triangles" % args.rate,
                "value": r["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": r["requests"], "warmup": 0,
                "ms_per_step": r["mean_ms"], "higher_is_better": False, "scaling": "weak",
-               "vs_baseline": None, "dtype": "f32+f64" if r["precision"] == "fp32" else "f64", "data": "synthetic",
+               "vs_baseline": None, "dtype": "f64" if r["precision"] == "fp64_records" else "f32+f64",
+               "data": "synthetic",
                "config": {"workload": "open-loop Poisson stream; each request a wavy-sheet pair (log-uniform "
                                       "10k-2M triangles per object, 8 pairs, 4 motion variants) with B rigidly "
                                       "moved + 0.1%% jitter, uploaded from pinned host memory; refit B, rebuild "
@@ -604,7 +612,7 @@ def main():
                     % (n, args.od, args.leaf))
         metric = "ms/step build+self-collision @%d spheres" % n
     first = res[args.precisions[0]]
-    fp32 = args.precisions[0] == "fp32"
+    form = args.precisions[0]
     waves = (n + 63) // 64
     st = first["stats"]
     out = {
@@ -618,10 +626,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if fp32 else "f64",
+        "dtype": {"fp64": "f64 (pair decisions) + f32 (outward pruning bounds)",
+                  "fp32": "f32 (outward slabs) + f64 (undecided pairs)", "fp64_records": "f64"}[form],
         "data": "synthetic",
         "config": {"workload": workload,
-                   "precision": "fp32 tree, exact fp64 re-check" if fp32 else "fp64",
+                   "precision": {"fp64": "fp64: fp64 element slabs, every pair decided in fp64; tree records "
+                                         "hold fp32 bounds rounded outward (pruning only)",
+                                 "fp32": "fp32-outward slabs, exact fp64 re-check of what fp32 cannot prove",
+                                 "fp64_records": "fp64 slabs in every tree record"}[form],
                    "elements_per_object": n, "od": args.od,
                    "leaf_size": 1,
                    "leaf_size_note": "the GPU trees hold one element per leaf and test both children of a "
@@ -649,12 +661,18 @@ def main():
         "step_hbm_frac": first["step_hbm_frac"],
         "roofline": first["roofline"],
     }
-    if len(args.precisions) == 2:
-        r64, r32 = res["fp64"], res["fp32"]
+    if len(args.precisions) == 3:
+        r64, r32, rr = res["fp64"], res["fp32"], res["fp64_records"]
         out["precision_forms"] = {
             "fp64_ms": round(r64["ms_per_step"], 4), "fp32_ms": round(r32["ms_per_step"], 4),
+            "fp64_records_ms": round(rr["ms_per_step"], 4),
             "fp32_speedup": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
-            "pairs_identical": bool(r64["npairs"] == r32["npairs"] and r64["digest"] == r32["digest"]),
+            "fp32_speedup_vs_fp64_records": round(rr["ms_per_step"] / r32["ms_per_step"], 3),
+            "pairs_identical": bool(r64["npairs"] == r32["npairs"] == rr["npairs"] and
+                                    r64["digest"] == r32["digest"] == rr["digest"]),
+            "fp64_records_stages_ms": {k: round(v, 4) for k, v in rr["stages"].items()},
+            "fp64_records_launch_autotuned": rr["launch"],
+            "fp64_records_roofline": rr["roofline"],
             "fp32_stages_ms": {k: round(v, 4) for k, v in r32["stages"].items()},
             "fp32_graph_form": r32.get("graph"),
             "fp32_launch_autotuned": r32["launch"],

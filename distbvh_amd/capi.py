@@ -11,10 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdistbvh.so")
 
 BVH_OK = 0
+BVH_E_INVALID = 1
 BVH_E_OVERFLOW = 3
 BVH_SPLIT_GEOM_AXIS, BVH_SPLIT_ML_GEOM_AXIS, BVH_SPLIT_CLUSTERING, BVH_SPLIT_GIVEN = 0, 1, 2, 3
 BVH_MEM_HOST, BVH_MEM_DEVICE = 0, 1
-BVH_PREC_FP64, BVH_PREC_FP32_OUTWARD = 0, 1
+BVH_PREC_FP64, BVH_PREC_FP32_OUTWARD, BVH_PREC_FP64_RECORDS = 0, 1, 2
 BVH_FLAG_SORT_RESULTS = 0x1
 BVH_FLAG_TIMING = 0x2
 BVH_FLAG_SERIAL_BUILDS = 0x4

@@ -76,7 +76,11 @@ class CollisionWorld:
         capi.load().bvh_world_config_init(C.byref(cfg))
         cfg.morton_bits = morton_bits
         cfg.leaf_size = leaf_size
-        cfg.precision = capi.BVH_PREC_FP32_OUTWARD if precision == "fp32" else capi.BVH_PREC_FP64
+        forms = {"fp64": capi.BVH_PREC_FP64, "fp32": capi.BVH_PREC_FP32_OUTWARD,
+                 "fp64_records": capi.BVH_PREC_FP64_RECORDS}
+        if precision not in forms:
+            raise ValueError("precision must be one of %s" % sorted(forms))
+        cfg.precision = forms[precision]
         cfg.flags = (capi.BVH_FLAG_SORT_RESULTS if sort_results else 0) | \
                     (capi.BVH_FLAG_TIMING if timing else 0) | \
                     (capi.BVH_FLAG_SERIAL_BUILDS if serial_builds else 0)

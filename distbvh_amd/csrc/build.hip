@@ -898,6 +898,24 @@ __global__ __launch_bounds__(256) void k_sorted_kdops(const Rec64* __restrict__ 
   }
 }
 
+// fp64 form with fp32 records: leaf p's exact slabs recomputed from its
+// element (the build's fp64 k-DOP, bit for bit), n x 26 in sorted order
+__global__ __launch_bounds__(256) void k_sorted_elem_kdops(ElemInput in, const uint32_t* __restrict__ perm,
+                                                           double* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < in.n; p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t e = perm[p];
+    double kd[kSlabs];
+    if (e < in.n) {
+      element_kdop_any(in, e, kd);
+    } else {
+      empty_kdop(kd);
+    }
+    double2* d = (double2*)(out + p * kSlabs);
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) d[a] = make_double2(kd[2 * a], kd[2 * a + 1]);
+  }
+}
+
 // The patch-level tree (the levels of the tree above the patch roots) in
 // pre-order, for for_each_tree: per output node its parent and children
 // (output indices, -1 for none) and its patch (a leaf) or -1.  One thread;
@@ -1134,6 +1152,15 @@ hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const ui
   uint64_t blocks = (n * kSlabs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_sorted_kdops, dim3((unsigned)blocks), dim3(256), 0, s, rec, rootbox, delta, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sorted_elem_kdops(const ElemInput& in, const uint32_t* perm, double* out, hipStream_t s) {
+  if (in.n == 0) return hipSuccess;
+  if (!perm || !out || !in.data) return hipErrorInvalidValue;
+  uint64_t blocks = (in.n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_sorted_elem_kdops, dim3((unsigned)blocks), dim3(256), 0, s, in, perm, out);
   return hipGetLastError();
 }
 

@@ -207,7 +207,8 @@ struct bvh_object_s {
       front_a, front_b, patch_k64;
   BuildBufs bb{};
   int bits = 63;
-  int fp32 = 0;
+  int fp32 = 0;  // fp32-outward tree records (forms kFormFp32 and kFormMixed)
+  int form = kFormMixed;
   // geom_axis / ml_geom_axis: the splitter's buffers and the reference order
   // (position -> input index) that patch centroids and patch elements follow;
   // the tree's own order (perm) is patch-major, Morton order inside a patch
@@ -607,7 +608,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   // serial or this fp32 build would be the one that fixes the world's frame
   // (later builds read it, so it must precede them in stream order).
   const bool fork = !(w->cfg.flags & BVH_FLAG_SERIAL_BUILDS) &&
-                    !(w->cfg.precision == BVH_PREC_FP32_OUTWARD && !w->frame_fixed) &&
+                    !(w->cfg.precision != BVH_PREC_FP64_RECORDS && !w->frame_fixed) &&
                     !(w->capturing && !o->bs);  // no stream is created inside a capture
   if (fork) {
     if (!o->bs) {
@@ -640,7 +641,10 @@ static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps
   o->in.eps = eps;
   o->in.n = n;
   o->bits = (int)w->cfg.morton_bits;
-  o->fp32 = w->cfg.precision == BVH_PREC_FP32_OUTWARD ? 1 : 0;
+  o->form = w->cfg.precision == BVH_PREC_FP32_OUTWARD ? kFormFp32
+                                                       : (w->cfg.precision == BVH_PREC_FP64_RECORDS ? kFormRecords64
+                                                                                                    : kFormMixed);
+  o->fp32 = o->form != kFormRecords64 ? 1 : 0;
   o->split = split;
   size_t planes = kind == kPoly ? (size_t)nv * 3 : (kind == kSphere ? 4 : kSlabs);
   size_t bytes = planes * n * sizeof(double);
@@ -764,7 +768,7 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
     w->stage_begin("narrowphase");
     PairOut po{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
                o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntStripes, dcount + kCntRecheck};
-    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? (cross ? 2 : 1) : 0, o->fp32, w->shape, po, s));
+    HIP_TRY(launch_narrowphase(qv, tv, (int)w->cfg.leaf_size, self ? (cross ? 2 : 1) : 0, o->form, w->shape, po, s));
     w->stage_end();
   }
   o->result_ptr = o->pairs.as<uint64_t>();
@@ -819,10 +823,10 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
 
 // Several object pairs o[i] x other[i] in one coalesced step: their counters
 // cleared and patch broadphases run per pair, then every element
-// narrowphase in one launch (k_narrow64_batch), then one synchronization
+// narrowphase in one launch (k_narrow64_batch / k_narrow32_batch), then one synchronization
 // for all their counters.  A pair whose lists overflowed runs again on its
-// own with grown buffers.  The fp64 form only (the fp32 form runs the pairs
-// one after another).
+// own with grown buffers.  The fp32-outward forms add one re-check launch per
+// pair after the walk.
 static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* others, uint64_t k) {
   bvh_world w = os[0]->w;
   hipStream_t s = w->stream;
@@ -834,6 +838,7 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     BVH_TRY(o->count.ensure(kCntTotal * sizeof(unsigned long long)));
     BVH_TRY(o->ppairs.ensure(std::max<uint64_t>(1024, 4 * w->od) * 8));
     BVH_TRY(ensure_pairs(o, std::max<uint64_t>(1u << 20, 8 * o->in.n)));
+    if (o->fp32) BVH_TRY(o->recheck.ensure(std::max<uint64_t>(1u << 16, o->in.n) * 8));
   }
   if (k > w->bq_host_cap) {
     if (w->bq_host) (void)hipHostFree(w->bq_host);
@@ -858,7 +863,7 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     o->q_cross = false;
     o->q_pcap = o->ppairs.bytes / 8;
     o->q_cap = o->pairs.bytes / 8;
-    o->q_rcap = 0;
+    o->q_rcap = o->fp32 ? o->recheck.bytes / 8 / kRecheckStripes : 0;  // per stripe
     o->result_ptr = o->pairs.as<uint64_t>();
     o->result_other = t;
     o->self_result = false;
@@ -870,8 +875,11 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     BatchQuery& e = w->bq_host[nq++];
     e.q = qv;
     e.t = tv;
-    e.out = PairOut{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits, dcount + kCntCand,
-                    nullptr, 0, nullptr, nullptr};
+    e.out = o->fp32 ? PairOut{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits,
+                              dcount + kCntCand, o->recheck.as<uint64_t>(), o->q_rcap, dcount + kCntStripes,
+                              dcount + kCntRecheck}
+                    : PairOut{o->pairs.as<uint64_t>(), o->q_cap, dcount + kCntPairs, dcount + kCntVisits,
+                              dcount + kCntCand, nullptr, 0, nullptr, nullptr};
     e.block0 = blocks;
     e.blocks = (uint32_t)((o->in.n + 255) / 256);
     blocks += e.blocks;
@@ -879,7 +887,11 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
   if (nq) {
     w->stage_begin("narrowphase_batch");
     HIP_TRY(hipMemcpyAsync(w->bq_dev.p, w->bq_host, nq * sizeof(BatchQuery), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_narrowphase_batch(w->bq_dev.as<BatchQuery>(), nq, blocks, w->shape.batch == 4 ? 4 : 2, s));
+    HIP_TRY(launch_narrowphase_batch(w->bq_dev.as<BatchQuery>(), nq, blocks, w->shape.batch == 4 ? 4 : 2,
+                                     os[0]->form, s));
+    // the fp32-outward forms: each query's re-check list, decided in fp64
+    for (uint32_t j = 0; j < nq && os[0]->fp32; ++j)
+      HIP_TRY(launch_recheck_query(w->bq_host[j].q, w->bq_host[j].t, w->bq_host[j].out, s));
     w->stage_end();
   }
   // every pair's counters and both objects' error words, one synchronization
@@ -897,17 +909,20 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     o->npairs = c[kCntPairs];
     o->nvisits = c[kCntVisits];
     o->ncandidates = c[kCntCand];
-    o->nrecheck = o->nrecheck_max = 0;
+    o->nrecheck = o->fp32 ? c[kCntRecheck] : 0;
+    o->nrecheck_max = o->fp32 ? c[kCntRecheckMax] : 0;
     o->npatch_pairs = c[kCntPatch];
     uint32_t sw[8];
     std::memcpy(sw, &c[kCntWords], sizeof(sw));
-    o->slab_err_bits = 0;
+    o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
     if (sw[0] | sw[4])
       return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(sw[0] | sw[4]) + ")");
-    if (o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap) {
+    if (o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck_max > o->q_rcap)) {
       // this pair alone again, with the lists grown to its counts
       if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
       if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
+      if (o->fp32 && o->nrecheck_max > o->q_rcap)
+        BVH_TRY(o->recheck.ensure((o->nrecheck_max + (o->nrecheck_max >> 3) + 64) * kRecheckStripes * 8));
       BVH_TRY(run_query(o, others[i], false));
       continue;
     }
@@ -1042,7 +1057,7 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
   if (cfg) c = *cfg;
   if (c.morton_bits != 63 && c.morton_bits != 30) return fail(BVH_E_INVALID, "morton_bits must be 63 or 30");
   if (c.leaf_size < 1 || c.leaf_size > 8) return fail(BVH_E_INVALID, "leaf_size must be in 1..8");
-  if (c.precision != BVH_PREC_FP64 && c.precision != BVH_PREC_FP32_OUTWARD)
+  if (c.precision != BVH_PREC_FP64 && c.precision != BVH_PREC_FP32_OUTWARD && c.precision != BVH_PREC_FP64_RECORDS)
     return fail(BVH_E_INVALID, "unknown precision");
   HIP_TRY(hipSetDevice(c.device));
   auto w = std::make_unique<bvh_world_s>();
@@ -1054,7 +1069,7 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
     HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     w->own_stream = true;
   }
-  if (c.precision == BVH_PREC_FP32_OUTWARD) {
+  if (c.precision != BVH_PREC_FP64_RECORDS) {
     BVH_TRY(w->frame.ensure(4 * sizeof(double)));
     HIP_TRY(launch_fill_u32(w->frame.as<uint32_t>(), 8, 0u, w->stream));
   }
@@ -1348,7 +1363,7 @@ bvh_status bvh_broadphase_batch(bvh_object* os, bvh_object* others, uint64_t k) 
   }
   BVH_TRY(join_all(w));
   if (w->capturing) return fail(BVH_E_STATE, "broadphase_batch inside a capture");
-  if (w->cfg.precision == BVH_PREC_FP32_OUTWARD || (w->cfg.flags & BVH_FLAG_SORT_RESULTS)) {
+  if (w->cfg.flags & BVH_FLAG_SORT_RESULTS) {
     for (uint64_t i = 0; i < k; ++i) BVH_TRY(run_query(os[i], others[i], false));
     return BVH_OK;
   }
@@ -1518,10 +1533,22 @@ bvh_status bvh_object_splits(bvh_object o, uint64_t* splits, size_t cap) {
 bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
   BVH_TRY(join_all(o->w));
-  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "the fp32 form stores fp32 slabs only");
+  if (o->form == kFormFp32) return fail(BVH_E_UNSUPPORTED, "the fp32 form stores fp32 slabs only");
   const uint64_t n = o->in.n;
   if (n == 0) return BVH_OK;
   hipStream_t s = o->w->stream;
+  if (o->form == kFormMixed) {
+    // the records hold fp32 bounds: each leaf's fp64 slabs from its element
+    if (mem == BVH_MEM_DEVICE) {
+      HIP_TRY(launch_sorted_elem_kdops(o->in, o->perm.as<uint32_t>(), kdops, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      return BVH_OK;
+    }
+    DevBuf tmp;
+    BVH_TRY(tmp.ensure(n * kSlabs * sizeof(double)));
+    HIP_TRY(launch_sorted_elem_kdops(o->in, o->perm.as<uint32_t>(), tmp.as<double>(), s));
+    return copy_out(kdops, tmp.p, n * kSlabs * sizeof(double), BVH_MEM_HOST, s);
+  }
   // each leaf's slabs from its parent's record
   if (mem == BVH_MEM_DEVICE) {
     HIP_TRY(launch_sorted_kdops(o->rec64.as<Rec64>(), o->rootbox64.as<double>(), o->delta.as<uint8_t>(), n, kdops, s));
@@ -1660,7 +1687,7 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
                            size_t* nnodes, size_t* root) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
   BVH_TRY(join_all(o->w));
-  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "tree export is fp64 only");
+  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "tree export needs fp64 records (BVH_PREC_FP64_RECORDS)");
   const uint64_t n = o->in.n;
   const size_t nn = n ? 2 * n - 1 : 0;
   if (nnodes) *nnodes = nn;

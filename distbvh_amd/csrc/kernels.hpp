@@ -118,6 +118,8 @@ hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hip
 // parents' records
 hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
                                hipStream_t s);
+// every leaf's fp64 slabs recomputed from its element, sorted order (n x 26)
+hipError_t launch_sorted_elem_kdops(const ElemInput& in, const uint32_t* perm, double* out, hipStream_t s);
 uint64_t prep_blocks(uint64_t n);
 // the patch-level tree in pre-order: 4 words per node (parent, left, right:
 // output indices or -1; patch or -1), *count nodes; stack: 4 od words
@@ -277,11 +279,18 @@ struct LaunchShape {
   unsigned batch = 2;
 };
 
+// Tree forms: fp64 slabs in every record (kFormRecords64); fp32-outward
+// records whose leaf candidates fp32 cannot prove are decided in fp64
+// (kFormFp32); fp32-outward records with every leaf candidate decided in
+// fp64 (kFormMixed, the default fp64 form).
+constexpr int kFormRecords64 = 0;
+constexpr int kFormFp32 = 1;
+constexpr int kFormMixed = 2;
 // element narrowphase: every leaf of `q` (queries, in its sorted order)
 // against tree `t`; self = 1 emits each unordered pair of one tree once,
 // self = 2 only those whose elements lie in different patches.
 hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self,
-                              int fp32, const LaunchShape& shape, const PairOut& out, hipStream_t s);
+                              int form, const LaunchShape& shape, const PairOut& out, hipStream_t s);
 // One query of a batched narrowphase launch: tree views, outputs and the
 // blocks (256 threads, one query element each) [block0, block0 + blocks)
 struct BatchQuery {
@@ -291,9 +300,13 @@ struct BatchQuery {
   uint32_t block0;
   uint32_t blocks;
 };
-// the fp64 pair walks of nq queries (no self, no XCD remap) in one launch of
-// `blocks` blocks of 256 threads; bq in device memory; batch 2 or 4
-hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, hipStream_t s);
+// the pair walks of nq queries (no self, no XCD remap) in one launch of
+// `blocks` blocks of 256 threads; bq in device memory; batch 2 or 4 for the
+// walk over fp64 records.  The fp32-outward forms leave their re-check
+// lists to launch_recheck_query, one launch per query.
+hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, int form,
+                                    hipStream_t s);
+hipError_t launch_recheck_query(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s);
 // patch broadphase: patch roots of `q` against the patch level of `t`
 hipError_t launch_patch_broadphase(const TreeView& q, const TreeView& t, int fp32, const PairOut& out,
                                    hipStream_t s);

@@ -209,17 +209,16 @@ __device__ __forceinline__ void own_patch_range(const TreeView& q, uint32_t qa, 
   pe = (uint64_t)k + 1 >= q.od ? n : q.splits[k];
 }
 
-template <bool SELF, bool CROSS = false>
-__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xcd, PairOut out) {
-  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
-  __shared__ uint64_t s_rbuf[256 / kWave][kRecheckBuf];
-  __shared__ uint32_t s_stack[256 / kWave][kStack];
-  uint64_t* buf = s_buf[threadIdx.x / kWave];
-  uint64_t* rbuf = s_rbuf[threadIdx.x / kWave];
-  uint32_t* stk = s_stack[threadIdx.x / kWave];
+// The fp32-outward record walk of one tile of blockDim.x queries (the body of
+// k_narrow32 and of its batched form k_narrow32_batch); buf / rbuf / stk are
+// the wave's LDS pair staging, re-check staging and stack, rstripe the
+// wave's re-check stripe.  Leaf candidates with a gap above `inner` are
+// emitted, the rest queued for the fp64 re-check.
+template <bool SELF, bool CROSS>
+__device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uint32_t tile, const PairOut& out,
+                                       int32_t inner, uint64_t* buf, uint64_t* rbuf, uint32_t* stk,
+                                       uint32_t rstripe) {
   uint32_t fill = 0, rfill = 0;
-  const uint32_t rstripe = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kRecheckStripes;
-  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
   int32_t qf[kSlabs];  // ord32 images of the query's outward fp32 slabs
@@ -262,7 +261,7 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     const uint64_t m = __ballot(h);
     if (m == 0) return;
     ncand += (uint32_t)__popcll(m);
-    const bool sure = h & (g > kInnerUlps);
+    const bool sure = h & (g > inner);
     const uint64_t key = pair_key(qa, elem, SELF);
     wave_emit(buf, fill, out, sure, key);
     const bool unsure = h & !sure;
@@ -356,6 +355,39 @@ __global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xc
     if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
     if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
   }
+}
+
+template <bool SELF, bool CROSS = false>
+__global__ __launch_bounds__(256) void k_narrow32(TreeView q, TreeView t, int xcd, PairOut out, int32_t inner) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint64_t s_rbuf[256 / kWave][kRecheckBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint32_t rstripe = (blockIdx.x * (blockDim.x / kWave) + wv) % kRecheckStripes;
+  const uint32_t tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  walk32<SELF, CROSS>(q, t, tile, out, inner, s_buf[wv], s_rbuf[wv], s_stack[wv], rstripe);
+}
+
+// Several queries (object pairs) in one launch of the fp32-outward walk:
+// block b walks tile b - block0 of the query whose [block0, block0 +
+// blocks) holds b (a wave-uniform binary search over the table).
+__global__ __launch_bounds__(256) void k_narrow32_batch(const BatchQuery* __restrict__ bq, uint32_t nq,
+                                                        int32_t inner) {
+  __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
+  __shared__ uint64_t s_rbuf[256 / kWave][kRecheckBuf];
+  __shared__ uint32_t s_stack[256 / kWave][kStack];
+  const uint32_t wv = threadIdx.x / kWave;
+  uint32_t lo = 0, hi = nq;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bq[mid].block0 <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const BatchQuery& e = bq[lo];
+  if (blockIdx.x - e.block0 >= e.blocks) return;
+  const uint32_t tile = blockIdx.x - e.block0;
+  const uint32_t rstripe = (tile * (blockDim.x / kWave) + wv) % kRecheckStripes;
+  walk32<false, false>(e.q, e.t, tile, e.out, inner, s_buf[wv], s_rbuf[wv], s_stack[wv], rstripe);
 }
 
 // Strict overlap of the lane's query slabs q with one box b whose 26 slabs
@@ -885,7 +917,7 @@ static void narrow64_dispatch(const TreeView& q, const TreeView& t, int self, in
   else hipLaunchKernelGGL((k_narrow64<false, K>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
 }
 
-hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int fp32,
+hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_size, int self, int form,
                               const LaunchShape& shape, const PairOut& out, hipStream_t s) {
   (void)leaf_size;  // the record walk tests every leaf from its parent's record
   if (q.n == 0 || t.n == 0) return hipSuccess;
@@ -895,16 +927,18 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   if (self == 2 && (!q.patch_of_input || (q.od > 1 && !q.splits))) return hipErrorInvalidValue;
   const unsigned blocks = (unsigned)((q.n + bs - 1) / bs);
   const int xcd = shape.xcd_remap ? 1 : 0;
-  if (fp32) {
+  if (form == kFormFp32 || form == kFormMixed) {
     if (!q.rec32 || !q.rootbox32 || !q.delta || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin)
       return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
     // one record per round trip: it arrives in scalar registers (the batch
-    // applies to the fp64 walk)
-    if (self == 2) hipLaunchKernelGGL((k_narrow32<true, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
-    else if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
-    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out);
+    // applies to the fp64 walk).  The mixed form proves nothing in fp32: every
+    // leaf candidate goes to the fp64 re-check.
+    const int32_t inner = form == kFormMixed ? 0x7fffffff : kInnerUlps;
+    if (self == 2) hipLaunchKernelGGL((k_narrow32<true, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
+    else if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
+    else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
     // in self mode both keys index the one object (q == t)
     uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
     if (per > 8192 / kRecheckStripes) per = 8192 / kRecheckStripes;
@@ -926,10 +960,27 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   return hipGetLastError();
 }
 
-hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, hipStream_t s) {
+hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, int form,
+                                    hipStream_t s) {
   if (nq == 0 || blocks == 0) return hipSuccess;
-  if (batch == 4) hipLaunchKernelGGL(k_narrow64_batch<4>, dim3(blocks), dim3(256), 0, s, bq, nq);
-  else hipLaunchKernelGGL(k_narrow64_batch<2>, dim3(blocks), dim3(256), 0, s, bq, nq);
+  if (form == kFormFp32 || form == kFormMixed) {
+    const int32_t inner = form == kFormMixed ? 0x7fffffff : kInnerUlps;
+    hipLaunchKernelGGL(k_narrow32_batch, dim3(blocks), dim3(256), 0, s, bq, nq, inner);
+  } else if (batch == 4) {
+    hipLaunchKernelGGL(k_narrow64_batch<4>, dim3(blocks), dim3(256), 0, s, bq, nq);
+  } else {
+    hipLaunchKernelGGL(k_narrow64_batch<2>, dim3(blocks), dim3(256), 0, s, bq, nq);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_recheck_query(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s) {
+  if (q.n == 0 || t.n == 0) return hipSuccess;
+  if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
+  uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
+  if (per > 8192 / kRecheckStripes) per = 8192 / kRecheckStripes;
+  if (per == 0) per = 1;
+  launch_recheck(q, t, out, (unsigned)(per * kRecheckStripes), s);
   return hipGetLastError();
 }
 

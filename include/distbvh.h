@@ -59,8 +59,12 @@ typedef enum bvh_split_algorithm {
 typedef enum bvh_memory { BVH_MEM_HOST = 0, BVH_MEM_DEVICE = 1 } bvh_memory;
 
 typedef enum bvh_precision {
-  BVH_PREC_FP64 = 0,          /* fp64 slabs, bit-exact with the CPU path */
-  BVH_PREC_FP32_OUTWARD = 1   /* fp32 slabs rounded outward + fp64 re-check */
+  BVH_PREC_FP64 = 0,          /* fp64 element slabs, every pair decided in fp64 (bit-exact with
+                                 the CPU path); the tree's internal bounds are fp32 rounded
+                                 outward, so they only prune */
+  BVH_PREC_FP32_OUTWARD = 1,  /* fp32 slabs rounded outward; leaf pairs fp32 proves are kept,
+                                 the rest decided in fp64 (same pairs) */
+  BVH_PREC_FP64_RECORDS = 2   /* fp64 slabs in every tree record (the all-fp64 walk) */
 } bvh_precision;
 
 /* bvh_world_config.flags */
@@ -189,9 +193,10 @@ bvh_status bvh_broadphase_patches(bvh_object o, bvh_object other);
 /* broadphase(other) for k pairs at once, objects[i] x others[i] (k distinct
  * receiving objects, no object with itself): the coalesced step of a
  * serving loop.  Each pair's patch broadphase is its own small launch; the
- * element narrowphases of all pairs are one launch (fp64 form; the fp32 form
- * and sort_results run the pairs one after another); one synchronization
- * reads every pair's counts.  Results land in objects[i] as after
+ * element narrowphases of all pairs are one launch (every form; the forms
+ * with fp32-outward records add one fp64 re-check launch per pair; with
+ * sort_results the pairs run one after another); one synchronization reads
+ * every pair's counts.  Results land in objects[i] as after
  * broadphase.  No reference counterpart (the reference runs one broadphase
  * per call, collision_object.hpp:170). */
 bvh_status bvh_broadphase_batch(bvh_object *objects, bvh_object *others, uint64_t k);

@@ -6,6 +6,11 @@ import oracle as O  # oracle/oracle.py (checker)
 from distbvh_amd import workloads as W
 
 
+# the three tree forms: fp32-outward records with every pair decided in fp64
+# (the default), the fp32-outward form, fp64 slabs in every record
+FORMS = ["fp64", "fp32", "fp64_records"]
+
+
 def to_cuda(a):
     import torch
     a = np.ascontiguousarray(a)

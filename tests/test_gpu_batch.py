@@ -10,7 +10,7 @@ from distbvh_amd import CollisionWorld
 from distbvh_amd import capi
 from distbvh_amd.capi import BvhError
 from distbvh_amd import workloads as W
-from helpers import gpu_object, pair_digest, sort_pairs, to_cuda
+from helpers import FORMS, gpu_object, pair_digest, sort_pairs, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def _brute_self(a):
     return sort_pairs(O.brute_self(kd))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_graph_replays_match_brute_force(torch_cuda, precision):
     torch = torch_cuda
     a0 = _batch(0xC4)
@@ -135,7 +135,7 @@ def test_capture_refuses_buffer_growth(torch_cuda):
         assert np.array_equal(sort_pairs(o.pairs()), sort_pairs(O.brute_self(kd)))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_autotune(torch_cuda, precision):
     a, b = W.wavy_sheets(64)
     ka, _ = O.elements_poly(a)
@@ -147,7 +147,8 @@ def test_autotune(torch_cuda, precision):
         t = oa.autotune(ob)
         assert t["batch"] in (1, 2, 4)
         assert t["block_size"] in (64, 128, 256)
-        assert len(t["tried_ms"]) == (6 if precision == "fp32" else 18)
+        # the fp32-outward record walks take one record per round trip
+        assert len(t["tried_ms"]) == (18 if precision == "fp64_records" else 6)
         assert all(x > 0 for x in t["tried_ms"]) and t["ms"] == min(t["tried_ms"])
         assert np.array_equal(sort_pairs(oa.results()[:, [1, 3]]), want)
         oa.broadphase(ob)  # the chosen setting stays
@@ -161,7 +162,7 @@ def test_autotune(torch_cuda, precision):
         assert np.array_equal(sort_pairs(o.pairs()), want_self)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_launch_shapes(torch_cuda, precision):
     """Every batch (records per round trip), block size and XCD remap gives
     the oracle's pairs, for two objects and for self collision."""
@@ -190,7 +191,7 @@ def test_launch_shapes(torch_cuda, precision):
             w.set_launch(block_size=96)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_launch_shapes_full_size(torch_cuda, precision):
     """2 x 2,097,152 triangles: the batches give one pair set (digest and
     count), and the same node visits (the visited set does not depend on the
@@ -221,7 +222,7 @@ def _mesh_of(counts):
     return np.repeat(np.arange(len(counts)), counts)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_given_patches_cross_patch_pairs(torch_cuda, precision):
     """One patch per mesh of the batch (BVH_SPLIT_GIVEN): the patches are the
     meshes, plain self collision is brute force, and cross-patch self
@@ -250,7 +251,7 @@ def test_given_patches_cross_patch_pairs(torch_cuda, precision):
         got = sort_pairs(o.pairs())
         stats_cross = o.result_stats()
     assert np.array_equal(got, cross)
-    if precision == "fp32":
+    if precision != "fp64_records":
         # the vertex-sharing pairs of a mesh are never candidates in cross-patch mode
         assert stats_cross["rechecked_pairs"] < len(allp) - len(cross)
 

@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import workloads as W
-from helpers import gpu_object, oracle_elements, sort_pairs, sort_records, to_cuda, from_cuda
+from helpers import FORMS, from_cuda, gpu_object, oracle_elements, sort_pairs, sort_records, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -293,7 +293,7 @@ def test_tree_structure(torch_cuda, case):
     else:
         data, _ = W.flat_sheets(64)
         kind, od = "poly", 1
-    with CollisionWorld(od, morton_bits=63, leaf_size=4) as w:
+    with CollisionWorld(od, morton_bits=63, leaf_size=4, precision="fp64_records") as w:
         obj = gpu_object(w, kind, data, device=True)
         tree = obj.tree()
         skd = obj.sorted_kdops()
@@ -302,7 +302,7 @@ def test_tree_structure(torch_cuda, case):
     _check_tree(tree, skd, n, od, splits)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_refit_pairs(torch_cuda, precision):
     """bvh_refit keeps the topology and recomputes every k-DOP: after moving
     sheet B the pair set is brute force on the moved sheet, and refitting back

@@ -10,7 +10,7 @@ import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import capi
 from distbvh_amd import workloads as W
-from helpers import to_cuda
+from helpers import FORMS, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -55,7 +55,7 @@ def _check_tree(obj):
     assert seen == len(nonempty)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 @pytest.mark.parametrize("split", [capi.BVH_SPLIT_CLUSTERING, capi.BVH_SPLIT_GEOM_AXIS,
                                    capi.BVH_SPLIT_ML_GEOM_AXIS])
 @pytest.mark.parametrize("od", [1, 16, 64])

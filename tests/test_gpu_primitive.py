@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import workloads as W
-from helpers import gpu_object, sort_pairs, to_cuda
+from helpers import FORMS, gpu_object, sort_pairs, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -110,7 +110,7 @@ def _tris_of(planes, idx):
     return np.ascontiguousarray(planes[:, :, idx].transpose(2, 0, 1).reshape(-1, 9))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 @pytest.mark.parametrize("sort", [False, True])
 def test_filter_exact_contact_sheets(torch_cuda, precision, sort):
     """k-DOP pairs of two wavy sheets, then the primitive narrowphase: the kept

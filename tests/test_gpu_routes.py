@@ -10,7 +10,7 @@ import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import workloads as W
 from distbvh_amd.capi import BvhError
-from helpers import gpu_object, pair_digest, sort_pairs, sort_records, to_cuda
+from helpers import FORMS, gpu_object, pair_digest, sort_pairs, sort_records, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -69,7 +69,7 @@ def test_snapshot_route_matches_polytope_route(torch_cuda, device, bits, od):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_snapshot_route_refit_and_self(torch_cuda, precision):
     """Snapshots through refit and self collision, both forms, against brute
     force (the fp32 form re-checks on the caller's fp64 k-DOPs)."""
@@ -133,7 +133,7 @@ def test_records_carry_global_ids(torch_cuda, kind, device):
 
 
 # ---- device-sorted results -----------------------------------------------------------------
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_sort_results_flag_gives_the_oracle_order(torch_cuda, precision):
     """BVH_FLAG_SORT_RESULTS: the pairs and records come in ascending (input
     index of element_p, input index of element_q) order, the order of the

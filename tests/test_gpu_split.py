@@ -11,7 +11,7 @@ import oracle as O
 from distbvh_amd import CollisionWorld
 from distbvh_amd import capi
 from distbvh_amd import workloads as W
-from helpers import sort_pairs, to_cuda
+from helpers import FORMS, sort_pairs, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -99,7 +99,7 @@ def test_split_random_spheres(torch_cuda, split, osplit, ml, seed, n, od):
 
 
 @pytest.mark.parametrize("split,osplit,ml", SPLITS, ids=SPLIT_IDS)
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_split_wavy_sheets(torch_cuda, split, osplit, ml, precision):
     a, b = W.wavy_sheets(64)
     _check("poly", b, 32, split, osplit, ml, precision=precision)

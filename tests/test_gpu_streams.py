@@ -8,7 +8,7 @@ import pytest
 
 from distbvh_amd import CollisionWorld
 from distbvh_amd import workloads as W
-from helpers import sort_pairs
+from helpers import FORMS, sort_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -33,10 +33,10 @@ def _step(w, objs, datas):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_concurrent_builds_equal_serial(torch_cuda, precision):
     """Three objects rebuilt over several steps with moving data: every
-    step's pairs, each object's patches and (fp64) sorted k-DOPs equal the
+    step's pairs, each object's patches and (fp64 forms) sorted k-DOPs equal the
     serial builds'."""
     torch = torch_cuda
     a, b = W.wavy_sheets(96)
@@ -52,12 +52,12 @@ def test_concurrent_builds_equal_serial(torch_cuda, precision):
                 shift = torch.tensor([0.0, 0.0, 0.002 * k], dtype=torch.float64, device="cuda").view(1, 3, 1)
                 steps.append(_step(w, objs, [ta, tb + shift, tc - shift]))
             patches = [[(p["size"], p["kdop"].tobytes()) for p in o.local_patches()] for o in objs]
-            kd = [o.sorted_kdops() for o in objs] if precision == "fp64" else None
+            kd = [o.sorted_kdops() for o in objs] if precision != "fp32" else None
             res[serial] = (steps, patches, kd)
     s, c_ = res[True], res[False]
     assert s[0] == c_[0]
     assert s[1] == c_[1]
-    if precision == "fp64":
+    if precision != "fp32":
         for x, y in zip(s[2], c_[2]):
             assert np.array_equal(x, y)
 
@@ -117,7 +117,7 @@ def test_concurrent_builds_stage_times(torch_cuda):
         assert (oa.result_count(), oa.digest()) == ref
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_capture_of_concurrent_builds(torch_cuda, precision):
     """A captured two-object step whose builds fork onto the objects' streams
     (a graph with a branch per build), replayed on new contents of the
@@ -183,7 +183,7 @@ def test_graph_outlives_its_world(torch_cuda):
     g.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_concurrent_builds_edge_sizes(torch_cuda, precision):
     """Objects of 0, 1, 5 and 20,000 elements built concurrently (in the fp32
     form the first object is empty, so a later build fixes the frame), every

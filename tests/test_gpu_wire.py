@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 from distbvh_amd import CollisionWorld, capi
 from distbvh_amd import workloads as W
-from helpers import gpu_object, sort_pairs, sort_records, to_cuda
+from helpers import FORMS, gpu_object, sort_pairs, sort_records, to_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -101,7 +101,7 @@ def test_element_payload_round_trip(torch_cuda, kind, device):
     assert np.array_equal(raw[:, 0], g[idx])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", FORMS)
 def test_ghost_from_packed_patches_finds_the_same_pairs(torch_cuda, precision):
     """What the reference's ghosting does with the payload: the patches of A
     that B's broadphase names are packed, unpacked into a ghost object of a
