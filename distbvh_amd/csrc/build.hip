@@ -251,8 +251,138 @@ __global__ __launch_bounds__(256) void k_cluster_depth(const K* __restrict__ cod
 }
 
 // The first c entries of the (depth, index) order, sorted by position
-// (split/cluster.hpp:157-168), by a bitonic sort in LDS; remaining split slots
-// take n - 1 (split/cluster.hpp:101-116).
+// (split/cluster.hpp:157-168), without sorting the depths: with cum(d) the
+// number of boundaries of depth <= d (the histogram of k_cluster_depth), the
+// threshold D is the smallest depth with cum(D) >= c.  Every boundary of
+// depth < D is taken, and the first need = c - cum(D - 1) boundaries of depth
+// D in position order.  A boundary's output slot is the number of taken
+// boundaries before it: (#depth < D before it) + min(#depth == D before it,
+// need).  Remaining split slots take n - 1 (split/cluster.hpp:101-116).
+constexpr uint32_t kClusterItems = 16;  // boundaries per thread of the count / emit passes
+constexpr uint32_t kClusterTile = 256 * kClusterItems;
+
+struct ClusterCut {
+  uint32_t D, need;
+};
+
+// threshold of the histogram, computed by every block (256 bins in LDS)
+__device__ __forceinline__ ClusterCut cluster_cut(const uint32_t* __restrict__ hist, uint64_t c) {
+  __shared__ uint32_t h[256];
+  __shared__ ClusterCut cut;
+  h[threadIdx.x] = hist[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t cum = 0;
+    uint32_t d = 0;
+    for (; d < 255; ++d) {
+      if (cum + h[d] >= c) break;
+      cum += h[d];
+    }
+    cut = ClusterCut{d, (uint32_t)(c - cum)};
+  }
+  __syncthreads();
+  return cut;
+}
+
+// per tile: boundaries of depth < D and of depth == D
+__global__ __launch_bounds__(256) void k_cluster_count(const uint32_t* __restrict__ depth, uint64_t m,
+                                                       const uint32_t* __restrict__ hist, uint64_t c,
+                                                       uint32_t* __restrict__ counts) {
+  const ClusterCut cut = cluster_cut(hist, c);
+  const uint64_t base = (uint64_t)blockIdx.x * kClusterTile;
+  uint32_t lt = 0, eq = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kClusterItems; ++k) {
+    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    if (i < m) {
+      const uint32_t d = depth[i];
+      lt += d < cut.D;
+      eq += d == cut.D;
+    }
+  }
+  // block sums: wave reductions, then LDS
+  for (int o = 32; o > 0; o >>= 1) {
+    lt += __shfl_xor(lt, o, 64);
+    eq += __shfl_xor(eq, o, 64);
+  }
+  __shared__ uint32_t s[2][4];
+  if (lane_id() == 0) {
+    s[0][threadIdx.x / 64] = lt;
+    s[1][threadIdx.x / 64] = eq;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    counts[2 * blockIdx.x] = s[0][0] + s[0][1] + s[0][2] + s[0][3];
+    counts[2 * blockIdx.x + 1] = s[1][0] + s[1][1] + s[1][2] + s[1][3];
+  }
+}
+
+// exclusive scan of the tiles' counts in place (one block)
+__global__ __launch_bounds__(1024) void k_cluster_scan(uint32_t* __restrict__ counts, uint32_t ntiles) {
+  __shared__ uint32_t s[2][1024];
+  uint32_t run[2] = {0, 0};
+  for (uint32_t base = 0; base < ntiles; base += 1024) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t a = t < ntiles ? counts[2 * t] : 0u, b = t < ntiles ? counts[2 * t + 1] : 0u;
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+      const uint32_t x = threadIdx.x >= o ? s[0][threadIdx.x - o] : 0u;
+      const uint32_t y = threadIdx.x >= o ? s[1][threadIdx.x - o] : 0u;
+      __syncthreads();
+      s[0][threadIdx.x] += x;
+      s[1][threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (t < ntiles) {
+      counts[2 * t] = run[0] + s[0][threadIdx.x] - a;
+      counts[2 * t + 1] = run[1] + s[1][threadIdx.x] - b;
+    }
+    run[0] += s[0][1023];
+    run[1] += s[1][1023];
+    __syncthreads();
+  }
+}
+
+// the taken boundaries in position order; slots c .. nsplits - 1 take n - 1
+__global__ __launch_bounds__(256) void k_cluster_emit(const uint32_t* __restrict__ depth, uint64_t m,
+                                                      const uint32_t* __restrict__ hist, uint64_t c,
+                                                      const uint32_t* __restrict__ offs, uint64_t nsplits,
+                                                      uint32_t* __restrict__ splits) {
+  const ClusterCut cut = cluster_cut(hist, c);
+  const uint64_t base = (uint64_t)blockIdx.x * kClusterTile;
+  __shared__ uint32_t wl[4], we[4];
+  uint32_t lt0 = offs[2 * blockIdx.x], eq0 = offs[2 * blockIdx.x + 1];
+  const uint32_t wv = threadIdx.x / 64;
+  for (uint32_t k = 0; k < kClusterItems; ++k) {
+    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    const uint32_t d = i < m ? depth[i] : 0xffffffffu;
+    const bool isl = i < m && d < cut.D, ise = i < m && d == cut.D;
+    const uint64_t ml = __ballot(isl), me = __ballot(ise);
+    if (lane_id() == 0) {
+      wl[wv] = (uint32_t)__popcll(ml);
+      we[wv] = (uint32_t)__popcll(me);
+    }
+    __syncthreads();
+    uint32_t lb = lt0, eb = eq0;
+    for (uint32_t v = 0; v < wv; ++v) {
+      lb += wl[v];
+      eb += we[v];
+    }
+    const uint32_t lr = lb + (uint32_t)__popcll(ml & lanemask_lt());
+    const uint32_t er = eb + (uint32_t)__popcll(me & lanemask_lt());
+    if (isl || (ise && er < cut.need)) splits[lr + (er < cut.need ? er : cut.need)] = (uint32_t)i;
+    lt0 += wl[0] + wl[1] + wl[2] + wl[3];
+    eq0 += we[0] + we[1] + we[2] + we[3];
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    const uint32_t fillv = (uint32_t)m;  // n - 1
+    for (uint64_t i = c + threadIdx.x; i < nsplits; i += 256) splits[i] = fillv;
+  }
+}
+
 constexpr int kSelectMax = 16384;
 __global__ __launch_bounds__(1024) void k_cluster_select(const uint32_t* __restrict__ depth_idx, uint64_t n,
                                                          uint64_t nsplits, uint32_t* __restrict__ splits) {
@@ -297,21 +427,58 @@ __device__ __forceinline__ uint32_t msb32(uint32_t x) { return 31u - (uint32_t)_
 __device__ __forceinline__ uint32_t msb64(uint64_t x) { return 63u - (uint32_t)__clzll((long long)x); }
 
 // patch of every sorted position, the split "height" delta(i) of the
-// augmented key between positions i and i+1, zeroed arrival counters.
+// augmented key between positions i and i+1, zeroed arrival counters.  A
+// block covers kDeltaTile consecutive positions [base, base + kDeltaTile):
+// the split values that fall in [base, base + kDeltaTile] (usually none or
+// one) are staged in LDS, so a position's patch is the block's first patch
+// plus a search over those few values instead of the whole split list.
+constexpr uint32_t kDeltaItems = 8;
+constexpr uint32_t kDeltaTile = 256 * kDeltaItems;
+constexpr uint32_t kDeltaLds = 1024;  // staged split values per block
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_delta(const K* __restrict__ codes, uint64_t n, const uint32_t* __restrict__ splits,
                                                uint32_t nsplits, uint64_t od, BuildBufs b) {
-  const uint64_t total = n > od ? n : od;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    if (i < od) b.patch_root[i] = kEnd;
-    if (i == 0) *b.root = kEnd;  // set again by the build; stays empty if it fails
-    if (i >= n) continue;
-    const uint32_t pi = patch_of(splits, nsplits, (uint32_t)i);
+  __shared__ uint32_t s_split[kDeltaLds];
+  __shared__ uint32_t s_lo, s_cnt;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < od; i += (uint64_t)gridDim.x * blockDim.x)
+    b.patch_root[i] = kEnd;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *b.root = kEnd;  // set again by the build; stays empty if it fails
+  const uint64_t base = (uint64_t)blockIdx.x * kDeltaTile;
+  if (base >= n) return;
+  if (threadIdx.x == 0) {
+    // splits[lo, hi) are the values in [base, base + kDeltaTile]
+    const uint32_t lo = base ? patch_of(splits, nsplits, (uint32_t)(base - 1)) : 0u;
+    const uint64_t last = base + kDeltaTile < n ? base + kDeltaTile : n;
+    const uint32_t hi = patch_of(splits, nsplits, (uint32_t)last);
+    s_lo = lo;
+    s_cnt = hi - lo;
+  }
+  __syncthreads();
+  const uint32_t lo = s_lo, cnt = s_cnt;
+  const bool staged = cnt <= kDeltaLds;
+  if (staged)
+    for (uint32_t k = threadIdx.x; k < cnt; k += 256) s_split[k] = splits[lo + k];
+  __syncthreads();
+  // patch of position i >= base: lo + the staged values <= i
+  auto patch_at = [&](uint32_t i) -> uint32_t {
+    if (!staged) return patch_of(splits, nsplits, i);
+    uint32_t a = 0, z = cnt;
+    while (a < z) {
+      const uint32_t mid = (a + z) >> 1;
+      if (s_split[mid] <= i) a = mid + 1;
+      else z = mid;
+    }
+    return lo + a;
+  };
+  for (uint32_t k = 0; k < kDeltaItems; ++k) {
+    const uint64_t i = base + (uint64_t)k * 256 + threadIdx.x;
+    if (i >= n) break;
+    const uint32_t pi = patch_at((uint32_t)i);
     b.patch_of_pos[i] = pi;
     if (i + 1 < n) {
       b.counters[i] = 0;
-      const uint32_t pj = patch_of(splits, nsplits, (uint32_t)(i + 1));
+      const uint32_t pj = patch_at((uint32_t)(i + 1));
       uint32_t d;
       if (pi != pj) {
         d = 128u + msb32(pi ^ pj);
@@ -620,14 +787,14 @@ __device__ __forceinline__ void patch_flush(unsigned long long* pk64, const doub
   }
 }
 
+// the object's frame origin: its Morton bounds minimum (non-finite
+// components 0)
 __global__ void k_set_frame(const Bounds* bounds, double* frame) {
-  uint32_t* flag = (uint32_t*)(frame + 3);
-  if (threadIdx.x != 0 || *flag != 0) return;
+  if (threadIdx.x != 0) return;
   for (int c = 0; c < 3; ++c) {
     const double v = bounds->min[c];
     frame[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
   }
-  *flag = 1;
 }
 
 __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
@@ -654,7 +821,6 @@ struct ChunkLds {
   uint8_t delta[kTreeChunk + 1];  // delta[c0 - 1 + j]
   FrontNode front[2];
   uint32_t nfront;
-  uint32_t err;
 };
 
 template <bool FP32>
@@ -741,6 +907,136 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
   }
 }
 
+// The fp32-outward leaf slabs, one axis at a time: each axis of the
+// element's fp64 k-DOP is formed (the operations and order of element_kdop),
+// folded into the exact patch slabs, rounded outward in the object's frame
+// and measured (error in fp32 ulps) before the next axis is formed, so only
+// one axis is live in fp64 (the whole k-DOP held 52 more registers).
+template <int KIND, int NV>
+struct LeafSource {
+  double v[NV][3];     // kPoly: the vertices; kSphere: v[0] the centre
+  double eps;          // kPoly: from_vertices epsilon; kSphere: the radius
+  const double* kd;    // kSnapshot: the element's 26 slabs
+};
+
+template <int KIND, int NV>
+__device__ __forceinline__ void leaf_source(const ElemInput& in, uint32_t e, bool act, LeafSource<KIND, NV>& s) {
+  const uint64_t n = in.n;
+  const uint64_t ee = act ? e : 0u;
+  if constexpr (KIND == kPoly) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) s.v[j][c] = act ? in.data[(uint64_t)(3 * j + c) * n + ee] : 0.0;
+    s.eps = in.eps;
+  } else if constexpr (KIND == kSphere) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.v[0][c] = act ? in.data[(uint64_t)c * n + ee] : 0.0;
+    s.eps = act ? in.data[3 * n + ee] : 0.0;
+  } else {
+    s.kd = in.data + ee * kSlabs;
+  }
+}
+
+template <int KIND, int NV, int A>
+__device__ __forceinline__ void leaf_axis(const LeafSource<KIND, NV>& s, bool act, double& lo, double& hi) {
+  if (!act) {
+    lo = __builtin_inf();
+    hi = -__builtin_inf();
+    return;
+  }
+  if constexpr (KIND == kPoly) {
+    const double p0 = project<A>(s.v[0][0], s.v[0][1], s.v[0][2]);
+    lo = p0 - s.eps;
+    hi = p0 + s.eps;
+#pragma unroll
+    for (int j = 1; j < NV; ++j) {
+      const double pj = project<A>(s.v[j][0], s.v[j][1], s.v[j][2]);
+      lo = sel_min(lo, pj - s.eps);
+      hi = sel_max(hi, pj + s.eps);
+    }
+  } else if constexpr (KIND == kSphere) {
+    const double pc = project<A>(s.v[0][0], s.v[0][1], s.v[0][2]);
+    lo = pc - s.eps;
+    hi = pc + s.eps;
+  } else {
+    lo = s.kd[2 * A];
+    hi = s.kd[2 * A + 1];
+  }
+}
+
+// distance of an fp32 slab from its fp64 value, in fp32 ulps: an ulp is a
+// power of two 2^k (frexp exponent k + 1), so the division is the exponent
+// shift ldexp(d, 1 - (k + 1)), exact
+__device__ __forceinline__ float slab_error_ulps(float f, double exact) {
+  const float mag = __builtin_fabsf(f);
+  const float ulp = next_up(mag) - mag;
+  const int e = __builtin_amdgcn_frexp_expf(ulp);
+  return (float)__builtin_ldexp(__builtin_fabs((double)f - exact), 1 - e);
+}
+
+// uni: the wave's elements lie in one patch (its partials are reduced in
+// registers, lane 0 leaves them in `part`); else each lane with an element
+// folds its slabs in with atomics.  Each axis's outward slabs go straight to
+// the leaf's LDS slot (column `slot` of slab), so no slab stays in a register
+// while the next axis is formed.
+template <int KIND, int NV, int A = 0>
+__device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool act, bool uni, uint32_t pk,
+                                            unsigned long long* pk64, double* part, const double* org,
+                                            int32_t (*slab)[kTreeLeafChunk], uint32_t slot) {
+  double lo, hi;
+  leaf_axis<KIND, NV, A>(s, act, lo, hi);
+  if (uni) {
+    const double wl = wave_min(lo);
+    const double wh = wave_max(hi);
+    if (lane_id() == 0) {
+      part[2 * A] = wl;
+      part[2 * A + 1] = wh;
+    }
+  } else if (pk != kEnd) {
+    unsigned long long* d = pk64 + (uint64_t)pk * kSlabs;
+    atomicMin(d + 2 * A, (unsigned long long)ordered_u64(lo));
+    atomicMax(d + 2 * A + 1, (unsigned long long)ordered_u64(hi));
+  }
+  if (act) {
+    const double po = project<A>(org[0], org[1], org[2]);
+    const float fl = round_down_1ulp(lo - po), fh = round_up_1ulp(hi - po);
+    const int32_t kl = (fl == fl) ? ord32(fl) : 0x7fffffff;
+    const int32_t kh = (fh == fh) ? ord32(fh) : (int32_t)0x80000000;
+    slab[2 * A][slot] = kl;
+    slab[2 * A + 1][slot] = kh;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (A + 1 < kAxes) leaf_axes32<KIND, NV, A + 1>(s, act, uni, pk, pk64, part, org, slab, slot);
+}
+
+// Largest distance of an object's fp32-outward leaf slabs from their fp64
+// values, in fp32 ulps (atomic max of the float bits into *out): the leaf
+// slabs recomputed from element data exactly as the build rounds them.  A
+// diagnostic, launched when the statistic is asked for.
+template <int KIND, int NV>
+__global__ __launch_bounds__(256) void k_slab_error(ElemInput in, const double* __restrict__ org,
+                                                    uint32_t* __restrict__ out) {
+  float err = 0.0f;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < in.n; e += (uint64_t)gridDim.x * blockDim.x) {
+    double kd[kSlabs], po[kAxes];
+    element_kdop<KIND, NV>(in, (uint32_t)e, kd);
+    proj_all(org[0], org[1], org[2], po);
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      const double xl = kd[2 * a] - po[a], xh = kd[2 * a + 1] - po[a];
+      const float el = slab_error_ulps(round_down_1ulp(xl), xl), eh = slab_error_ulps(round_up_1ulp(xh), xh);
+      err = (el > err) ? el : err;  // NaN distances are skipped
+      err = (eh > err) ? eh : err;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float x = __shfl_xor(err, o, 64);
+    err = x > err ? x : err;
+  }
+  if (lane_id() == 0 && err > 0.0f) atomicMax(out, __float_as_uint(err));
+}
+
 template <int KIND, int NV, bool FP32>
 __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildBufs b, uint64_t od) {
   __shared__ ChunkLds<FP32> L;
@@ -760,73 +1056,68 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       const uint64_t dj = (uint64_t)c0 + kTreeChunk;
       L.delta[kTreeChunk] = (dj - 1 < n - 1) ? b.delta[dj - 1] : 0;
       L.nfront = 0;
-      L.err = 0;
     }
   }
   const uint32_t p = c0 + t;
   const uint32_t e = p < c1 ? b.perm[p] : kEnd;
   if (p < c1 && e >= n) atomicOr(b.status, kErrIndex);
   const bool act = p < c1 && e < n;
-  double kd[kSlabs];
-  uint32_t pk = kEnd;
-  if (act) {
-    element_kdop<KIND, NV>(in, e, kd);
-    pk = b.patch_of_pos[p];
-  } else {
-    empty_kdop(kd);
-  }
+  uint32_t pk = act ? b.patch_of_pos[p] : kEnd;
   // exact patch slabs for the patch broadphase (fp32 form; the fp64 form
   // takes them from the patch roots)
   __shared__ double s_part[FP32 ? kTreeChunk / kWave : 1][kSlabs];
   __shared__ uint32_t s_ppatch[FP32 ? kTreeChunk / kWave : 1];
-  if constexpr (FP32) patch_partial(b.patch_k64, pk, kd, s_part[t / kWave], &s_ppatch[t / kWave]);
-  SlabT<FP32> ks[kSlabs];
   const uint32_t leaf = (uint32_t)(n - 1 + p);
+  // the leaf's half of its parent's record (the root's array if n = 1)
+  const uint32_t dl = (act && p) ? b.delta[p - 1] : 0u, dr = (act && p != n - 1) ? b.delta[p] : 0u;
+  const bool is_left = (p == 0) || (p != n - 1 && dr < dl);
+  const uint32_t parent = is_left ? p : p - 1;
+  const int side = is_left ? 0 : 1;
+  if constexpr (FP32) {
+    LeafSource<KIND, NV> src;
+    leaf_source<KIND, NV>(in, e, act, src);
+    const uint64_t m = __ballot(pk != kEnd);
+    const uint32_t pk0 = m ? __builtin_amdgcn_readlane(pk, __ffsll((long long)m) - 1) : kEnd;
+    const bool uni = m != 0 && __ballot(pk != kEnd && pk != pk0) == 0;
+    leaf_axes32<KIND, NV>(src, act, uni, pk, b.patch_k64, s_part[t / kWave], b.origin, L.slab, t);
+    if (lane_id() == 0) s_ppatch[t / kWave] = uni ? pk0 : kEnd;
+  }
   if (act) {
     b.patch_of_input[e] = pk;
     const bool single = patch_begin(b.splits, od, pk) == p && patch_end(b.splits, od, n, pk) == p + 1;
     if (single) b.patch_root[pk] = leaf;
     if constexpr (FP32) {
-      double po[kAxes];
-      proj_all(b.origin[0], b.origin[1], b.origin[2], po);
-      float kf[kSlabs];
-      to_fp32_outward(kd, po, kf);
-      // largest distance of an fp32 slab from its fp64 value, in fp32 ulps;
-      // an ulp is a power of two 2^k (frexp exponent k + 1), so the division
-      // is the exponent shift ldexp(d, 1 - (k + 1)), exact
-      float err = 0.0f;
+      L.last[t] = p;
+      L.patch[t] = pk;
+      L.endslot[t] = t;
+      // the leaf's half of its parent's record (the root's array if n = 1),
+      // from its LDS slot, all its stores together
+      int32_t ks[kSlabs];
 #pragma unroll
-      for (int i = 0; i < kSlabs; ++i) {
-        const double exact = kd[i] - po[i / 2];
-        const float mag = __builtin_fabsf(kf[i]);
-        const float ulp = next_up(mag) - mag;
-        const int e = __builtin_amdgcn_frexp_expf(ulp);
-        const float d = (float)__builtin_ldexp(__builtin_fabs((double)kf[i] - exact), 1 - e);
-        err = d > err ? d : err;
+      for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][t];
+      if (n == 1) {
+        *b.root = leaf;
+        store_rootbox<true>(b, ks);
+      } else {
+        rec_store_half<true>(b, parent, side, ks, leaf, e);
       }
-      atomicMax(&L.err, __float_as_uint(err));
-      ord32_slabs(kf, ks);
     } else {
-#pragma unroll
-      for (int i = 0; i < kSlabs; ++i) ks[i] = kd[i];
+      double kd[kSlabs];
+      element_kdop<KIND, NV>(in, e, kd);
       if (single) store_patch_box<false>(b, pk, kd);
-    }
-    publish<FP32>(L, t, t, ks, p, pk);
-    if (n == 1) {
-      *b.root = leaf;
-      store_rootbox<FP32>(b, ks);
-    } else {
-      // the leaf's half of its parent's record
-      const uint32_t dl = p ? b.delta[p - 1] : 0u, dr = (p != n - 1) ? b.delta[p] : 0u;
-      const bool is_left = (p == 0) || (p != n - 1 && dr < dl);
-      rec_store_half<FP32>(b, is_left ? p : p - 1, is_left ? 0 : 1, ks, leaf, e);
+      publish<false>(L, t, t, kd, p, pk);
+      if (n == 1) {
+        *b.root = leaf;
+        store_rootbox<false>(b, kd);
+      } else {
+        rec_store_half<false>(b, parent, side, kd, leaf, e);
+      }
     }
   }
   __syncthreads();  // counters, deltas, leaf slots and patch partials ready
   if constexpr (FP32) patch_flush<kTreeChunk / kWave>(b.patch_k64, s_part, s_ppatch);
   if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L);
   __syncthreads();
-  if (FP32 && t == 0 && L.err != 0) atomicMax(b.status + 2, L.err);
   // a first arrival whose sibling lies outside the chunk leaves its arrival word
   if (c0 + t + 1 < c1 && L.cnt[t] == 1)
     b.counters[c0 + t] = ((unsigned long long)(L.far[t] + 1u) << 32) | L.fpatch[t];
@@ -1049,14 +1340,15 @@ hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsp
                          (const uint32_t*)b.codes_a, n, b.depth0, hist);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    SortArgs<uint32_t> a{};
-    a.k0 = b.depth0; a.ka = b.depth_a; a.kb = b.depth_b;
-    a.v0 = nullptr; a.va = b.depth_idx; a.vb = b.depth_idx_b;
-    a.n = m; a.ntiles = sort_tiles(m);
-    a.ctl = b.ctl + 1;
-    a.lookback = b.lookback + 8 * sort_tiles(n) * 256;
-    e = launch_sort_passes<uint32_t>(a, 1, hist, s);
-    if (e != hipSuccess) return e;
+    const uint64_t c = nsplits < m ? nsplits : m;
+    const uint64_t tiles = (m + kClusterTile - 1) / kClusterTile;
+    uint32_t* counts = b.depth_idx;  // 2 words per tile (n words available)
+    hipLaunchKernelGGL(k_cluster_count, dim3((unsigned)tiles), dim3(256), 0, s, (const uint32_t*)b.depth0, m,
+                       (const uint32_t*)hist, c, counts);
+    hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, s, counts, (uint32_t)tiles);
+    hipLaunchKernelGGL(k_cluster_emit, dim3((unsigned)tiles), dim3(256), 0, s, (const uint32_t*)b.depth0, m,
+                       (const uint32_t*)hist, c, (const uint32_t*)counts, nsplits, b.splits);
+    return hipGetLastError();
   }
   hipLaunchKernelGGL(k_cluster_select, dim3(1), dim3(1024), 0, s, (const uint32_t*)b.depth_idx, n, nsplits, b.splits);
   return hipGetLastError();
@@ -1072,9 +1364,10 @@ static void tree_dispatch(const ElemInput& in, const BuildBufs& b, uint64_t od, 
 hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s) {
   const uint64_t n = in.n;
   if (n == 0) return hipSuccess;
-  const uint64_t total = n > od ? n : od;
-  uint64_t blocks = (total + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  // one block per kDeltaTile positions (the od patch roots are cleared by a
+  // grid-stride loop over the same blocks)
+  uint64_t blocks = (n + kDeltaTile - 1) / kDeltaTile;
+  if (blocks == 0) blocks = 1;
   const uint32_t nsplits = (uint32_t)(od - 1);
   if (bits == 63)
     hipLaunchKernelGGL(k_delta<uint64_t>, dim3((unsigned)blocks), dim3(256), 0, s,
@@ -1152,6 +1445,31 @@ hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const ui
   uint64_t blocks = (n * kSlabs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_sorted_kdops, dim3((unsigned)blocks), dim3(256), 0, s, rec, rootbox, delta, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_slab_error(const ElemInput& in, const double* org, uint32_t* out, hipStream_t s) {
+  if (in.n == 0) return hipSuccess;
+  if (!org || !out || !in.data) return hipErrorInvalidValue;
+  uint64_t blocks = (in.n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  const dim3 g((unsigned)blocks), th(256);
+  if (in.kind == kSphere) {
+    hipLaunchKernelGGL((k_slab_error<kSphere, 1>), g, th, 0, s, in, org, out);
+  } else if (in.kind == kSnapshot) {
+    hipLaunchKernelGGL((k_slab_error<kSnapshot, 1>), g, th, 0, s, in, org, out);
+  } else {
+    switch (in.nv) {
+      case 1: hipLaunchKernelGGL((k_slab_error<kPoly, 1>), g, th, 0, s, in, org, out); break;
+      case 2: hipLaunchKernelGGL((k_slab_error<kPoly, 2>), g, th, 0, s, in, org, out); break;
+      case 3: hipLaunchKernelGGL((k_slab_error<kPoly, 3>), g, th, 0, s, in, org, out); break;
+      case 4: hipLaunchKernelGGL((k_slab_error<kPoly, 4>), g, th, 0, s, in, org, out); break;
+      case 5: hipLaunchKernelGGL((k_slab_error<kPoly, 5>), g, th, 0, s, in, org, out); break;
+      case 6: hipLaunchKernelGGL((k_slab_error<kPoly, 6>), g, th, 0, s, in, org, out); break;
+      case 7: hipLaunchKernelGGL((k_slab_error<kPoly, 7>), g, th, 0, s, in, org, out); break;
+      default: hipLaunchKernelGGL((k_slab_error<kPoly, 8>), g, th, 0, s, in, org, out); break;
+    }
+  }
   return hipGetLastError();
 }
 

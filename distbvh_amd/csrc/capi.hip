@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -138,9 +139,6 @@ struct bvh_world_s {
   DevBuf prim_in, prim_out;  // host-memory staging of bvh_closest_point
   // bvh_max_variant_axis / bvh_sort_and_sweep: staging, sorted B, counts
   DevBuf sw_ka, sw_kb, sw_ga, sw_gb, sw_bq, sw_bidx, sw_kbs, sw_bmax, sw_count, sw_total, sw_out, sw_part;
-  // fp32 form: the frame origin shared by every object (3 doubles + a set flag)
-  DevBuf frame;
-  bool frame_fixed = false;  // a build that sets the frame has been enqueued in stream order
   LaunchShape shape;  // narrowphase launch shape (autotuned)
   // stream capture (batching form): the queries captured so far, with the
   // state a replay restores in their objects
@@ -156,12 +154,10 @@ struct bvh_world_s {
   BatchQuery* bq_host = nullptr;
   size_t bq_host_cap = 0;
 
-  bvh_world_s() {
-    DevBuf* own[] = {&s_kb, &s_ka, &s_va, &s_vb, &s_hist, &s_ctl, &s_lb, &prim_in, &prim_out, &sw_ka, &sw_kb,
-                     &sw_ga, &sw_gb, &sw_bq, &sw_bidx, &sw_kbs, &sw_bmax, &sw_count, &sw_total, &sw_out,
-                     &sw_part, &frame, &bq_dev};
-    for (DevBuf* d : own) d->gen = &alloc_gen;
-  }
+  // Only buffers that captured work can reference count on the world's
+  // generation (its objects' buffers).  The stand-alone primitives' and the
+  // batched query's buffers serve calls that are never captured, so their
+  // growth leaves the world's graphs launchable.
 
   hipEvent_t take_event() {
     if (events_used == event_pool.size()) {
@@ -205,6 +201,8 @@ struct bvh_object_s {
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
       patch_of_input, counters, patch_root, root, rec64, rootbox64, rec32, rootbox32, status,
       front_a, front_b, patch_k64;
+  DevBuf frame;  // fp32-outward records: the frame origin of this object's slabs (3 doubles)
+  DevBuf slab_err;  // the slab-error statistic (one word)
   BuildBufs bb{};
   int bits = 63;
   int fp32 = 0;  // fp32-outward tree records (forms kFormFp32 and kFormMixed)
@@ -232,7 +230,6 @@ struct bvh_object_s {
   uint64_t ncandidates = 0;
   uint64_t nrecheck = 0;      // fp32 candidates re-checked in fp64
   uint64_t nrecheck_max = 0;  // the fullest re-check stripe
-  uint32_t slab_err_bits = 0;  // max fp32 slab error (ulps, float bits) of the queried objects
   const uint64_t* result_ptr = nullptr;  // device pointer to the (sorted) pairs
   bvh_object_s* result_other = nullptr;
   bool self_result = false;
@@ -266,7 +263,7 @@ struct bvh_object_s {
     DevBuf* all[] = {&input, &input_c, &gids, &centroid, &partials, &bounds, &codes0, &codes_a, &codes_b,
                      &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
                      &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
-                     &root, &rec64, &rootbox64, &rec32, &rootbox32, &status,
+                     &root, &rec64, &rootbox64, &rec32, &rootbox32, &status, &frame, &slab_err,
                      &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
                      &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
                      &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
@@ -319,7 +316,7 @@ static TreeView view_of(const bvh_object_s* o) {
   v.n = o->in.n;
   v.od = o->w->od;
   v.status = o->status.as<uint32_t>();
-  v.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
+  v.origin = o->fp32 ? o->frame.as<double>() : nullptr;
   v.elem = o->in;
   v.patch_k64 = o->patch_k64.as<unsigned long long>();
   v.splits = o->splits.as<uint32_t>();
@@ -372,6 +369,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
     BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(int32_t)));
+    BVH_TRY(o->frame.ensure(4 * sizeof(double)));
   } else {
     BVH_TRY(o->rec64.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec64)));
     BVH_TRY(o->rootbox64.ensure(kSlabs * sizeof(double)));
@@ -404,7 +402,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.rootbox64 = o->fp32 ? nullptr : o->rootbox64.as<double>();
   b.rec32 = o->fp32 ? o->rec32.as<Rec32>() : nullptr;
   b.rootbox32 = o->fp32 ? o->rootbox32.as<int32_t>() : nullptr;
-  b.origin = o->fp32 ? o->w->frame.as<double>() : nullptr;
+  b.origin = o->fp32 ? o->frame.as<double>() : nullptr;
   b.status = o->status.as<uint32_t>();
   b.front_a = o->front_a.as<FrontNode>();
   b.front_b = o->front_b.as<FrontNode>();
@@ -537,9 +535,9 @@ static bvh_status object_build(bvh_object_s* o) {
     w->stage_end();
   }
   if (o->fp32) {
-    // the world's fp32 frame: the Morton bounds minimum of its first non-empty build
+    // the object's fp32 frame: the Morton bounds minimum of this build (a
+    // refit keeps it)
     HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
-    w->frame_fixed = true;
   }
   HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
   w->stage_begin("delta");
@@ -605,10 +603,8 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   // The build runs on the object's own stream, after everything enqueued on
   // the world stream so far (inside a capture the build stream joins the
   // capture there and the graph gets a branch per build), unless builds are
-  // serial or this fp32 build would be the one that fixes the world's frame
-  // (later builds read it, so it must precede them in stream order).
+  // serial.
   const bool fork = !(w->cfg.flags & BVH_FLAG_SERIAL_BUILDS) &&
-                    !(w->cfg.precision != BVH_PREC_FP64_RECORDS && !w->frame_fixed) &&
                     !(w->capturing && !o->bs);  // no stream is created inside a capture
   if (fork) {
     if (!o->bs) {
@@ -707,7 +703,6 @@ static bvh_status read_counts(bvh_object_s* o, bvh_object_s* other, bool* overfl
   o->npatch_pairs = o->q_self ? 0 : w->h_count[kCntPatch];
   uint32_t sw[8];
   std::memcpy(sw, &w->h_count[kCntWords], sizeof(sw));
-  o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
   const uint32_t st = sw[0] | sw[4];
   if (st) return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(st) + ")");
   *overflow = o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck_max > o->q_rcap);
@@ -777,6 +772,19 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
   return BVH_OK;
 }
 
+// After a query whose lists overflowed: every list grown to what the query
+// needs.  Candidates the re-check could not hold were not decided, and each
+// of them can add one pair, so the pair list is sized for them too.
+static bvh_status grow_query_lists(bvh_object_s* o) {
+  const bool rover = o->fp32 && o->nrecheck_max > o->q_rcap;
+  const uint64_t need = o->npairs + (rover ? o->nrecheck : 0);
+  if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
+  if (need > o->q_cap) BVH_TRY(ensure_pairs(o, need + (need >> 3) + 1024));
+  if (rover)  // every stripe as large as the fullest one needs
+    BVH_TRY(o->recheck.ensure((o->nrecheck_max + (o->nrecheck_max >> 3) + 64) * kRecheckStripes * 8));
+  return BVH_OK;
+}
+
 static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow = true, bool cross = false) {
   bvh_world w = o->w;
   if (!o->built || !other->built) return fail(BVH_E_STATE, "set_entity_data must precede broadphase");
@@ -800,17 +808,13 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
                                         o->result_ptr});
     return BVH_OK;
   }
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  for (int attempt = 0; attempt < 3; ++attempt) {
     BVH_TRY(launch_query(o, other, self, narrow, cross));
     bool overflow = false;
     BVH_TRY(read_counts(o, other, &overflow));
     if (!overflow) break;
-    if (attempt == 1) return fail(BVH_E_OVERFLOW, "result lists still overflow after growing");
-    // grow to the exact counts and run again
-    if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
-    if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
-    if (o->fp32 && o->nrecheck_max > o->q_rcap)  // every stripe as large as the fullest one needs
-      BVH_TRY(o->recheck.ensure((o->nrecheck_max + (o->nrecheck_max >> 3) + 64) * kRecheckStripes * 8));
+    if (attempt == 2) return fail(BVH_E_OVERFLOW, "result lists still overflow after growing");
+    BVH_TRY(grow_query_lists(o));
   }
   if (narrow && (w->cfg.flags & BVH_FLAG_SORT_RESULTS)) {
     w->stage_begin("sort_results");
@@ -914,15 +918,11 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     o->npatch_pairs = c[kCntPatch];
     uint32_t sw[8];
     std::memcpy(sw, &c[kCntWords], sizeof(sw));
-    o->slab_err_bits = sw[2] > sw[6] ? sw[2] : sw[6];
-    if (sw[0] | sw[4])
+      if (sw[0] | sw[4])
       return fail(BVH_E_HIP, "device invariant check failed (status bits " + std::to_string(sw[0] | sw[4]) + ")");
     if (o->npairs > o->q_cap || o->npatch_pairs > o->q_pcap || (o->fp32 && o->nrecheck_max > o->q_rcap)) {
       // this pair alone again, with the lists grown to its counts
-      if (o->npatch_pairs > o->q_pcap) BVH_TRY(o->ppairs.ensure(o->npatch_pairs * 8 + 1024 * 8));
-      if (o->npairs > o->q_cap) BVH_TRY(ensure_pairs(o, o->npairs + (o->npairs >> 3) + 1024));
-      if (o->fp32 && o->nrecheck_max > o->q_rcap)
-        BVH_TRY(o->recheck.ensure((o->nrecheck_max + (o->nrecheck_max >> 3) + 64) * kRecheckStripes * 8));
+      BVH_TRY(grow_query_lists(o));
       BVH_TRY(run_query(o, others[i], false));
       continue;
     }
@@ -986,6 +986,7 @@ template <typename K>
 static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   if (!w) return fail(BVH_E_INVALID, "null world");
   BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "sort inside a capture");
   if (n == 0) return BVH_OK;
   if (!keys) return fail(BVH_E_INVALID, "null keys");
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 keys");
@@ -1069,10 +1070,6 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
     HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     w->own_stream = true;
   }
-  if (c.precision != BVH_PREC_FP64_RECORDS) {
-    BVH_TRY(w->frame.ensure(4 * sizeof(double)));
-    HIP_TRY(launch_fill_u32(w->frame.as<uint32_t>(), 8, 0u, w->stream));
-  }
   void* hc = nullptr;
   HIP_TRY(hipHostMalloc(&hc, 16 * sizeof(unsigned long long), hipHostMallocDefault));
   w->h_count = (unsigned long long*)hc;
@@ -1108,6 +1105,11 @@ bvh_status bvh_create_collision_object(bvh_world w, bvh_object* out) {
   auto o = std::make_unique<bvh_object_s>();
   o->w = w;
   o->for_each_buf([&](DevBuf& d) { d.gen = &w->alloc_gen; });
+  // outputs of calls that are never captured (records, digests, the exact
+  // filter, wire formats): their growth leaves the world's graphs launchable
+  DevBuf* outside[] = {&o->records, &o->digest, &o->slab_err, &o->ex_flag, &o->ex_out, &o->ex_lb, &o->ex_ctr,
+                       &o->wire_cent, &o->wire_range, &o->wire_src, &o->wire_out};
+  for (DevBuf* d : outside) d->gen = &g_alloc_gen;
   o->id = w->objects.size();
   *out = o.get();
   w->objects.push_back(std::move(o));
@@ -1208,6 +1210,32 @@ bvh_status bvh_object_release(bvh_object o) {
   return BVH_OK;
 }
 
+// fp32-outward slabs (ordered-integer images, relative to the frame origin
+// `org`) back in world coordinates as fp64, each moved one fp64 ulp outward
+// so that the box still contains the exact one after the rounding of the sum.
+static void decode_box32(const int32_t* ko, const double* org, double* out) {
+  static const double N[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                  {0.5773502691896258, 0.5773502691896258, 0.5773502691896258},
+                                  {0.5773502691896258, -0.5773502691896258, 0.5773502691896258},
+                                  {0.5773502691896258, 0.5773502691896258, -0.5773502691896258},
+                                  {0.5773502691896258, -0.5773502691896258, -0.5773502691896258},
+                                  {0.7071067811865475, 0.7071067811865475, 0}, {0.7071067811865475, 0, 0.7071067811865475},
+                                  {0, 0.7071067811865475, 0.7071067811865475}, {0.7071067811865475, -0.7071067811865475, 0},
+                                  {0.7071067811865475, 0, -0.7071067811865475}, {0, 0.7071067811865475, -0.7071067811865475}};
+  for (int a = 0; a < kAxes; ++a) {
+    const double po = org[0] * N[a][0] + org[1] * N[a][1] + org[2] * N[a][2];
+    for (int h = 0; h < 2; ++h) {
+      const int32_t k = ko[2 * a + h];
+      // inverse of the ordered-integer image
+      const uint32_t u = k >= 0 ? (uint32_t)k : (0x80000000u | (uint32_t)(-(int64_t)k));
+      float f;
+      std::memcpy(&f, &u, 4);
+      const double v = (double)f + po;
+      out[2 * a + h] = h ? std::nextafter(v, HUGE_VAL) : std::nextafter(v, -HUGE_VAL);
+    }
+  }
+}
+
 bvh_status bvh_object_root_kdop(bvh_object o, double* kdop) {
   if (!o || !kdop) return fail(BVH_E_INVALID, "null argument");
   BVH_TRY(join_all(o->w));
@@ -1217,27 +1245,10 @@ bvh_status bvh_object_root_kdop(bvh_object o, double* kdop) {
   hipStream_t s = o->w->stream;
   if (o->fp32) {
     int32_t ko[kSlabs];
-    float f[kSlabs];
     double org[3];
     BVH_TRY(copy_out(ko, o->rootbox32.p, sizeof(ko), BVH_MEM_HOST, s));
-    for (int i = 0; i < kSlabs; ++i) {  // inverse of the ordered-integer image
-      const uint32_t u = ko[i] >= 0 ? (uint32_t)ko[i] : (0x80000000u | (uint32_t)(-(int64_t)ko[i]));
-      std::memcpy(&f[i], &u, 4);
-    }
-    BVH_TRY(copy_out(org, o->w->frame.p, sizeof(org), BVH_MEM_HOST, s));
-    static const double N[13][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1},
-                                    {0.5773502691896258, 0.5773502691896258, 0.5773502691896258},
-                                    {0.5773502691896258, -0.5773502691896258, 0.5773502691896258},
-                                    {0.5773502691896258, 0.5773502691896258, -0.5773502691896258},
-                                    {0.5773502691896258, -0.5773502691896258, -0.5773502691896258},
-                                    {0.7071067811865475, 0.7071067811865475, 0}, {0.7071067811865475, 0, 0.7071067811865475},
-                                    {0, 0.7071067811865475, 0.7071067811865475}, {0.7071067811865475, -0.7071067811865475, 0},
-                                    {0.7071067811865475, 0, -0.7071067811865475}, {0, 0.7071067811865475, -0.7071067811865475}};
-    for (int a = 0; a < kAxes; ++a) {
-      const double po = org[0] * N[a][0] + org[1] * N[a][1] + org[2] * N[a][2];
-      kdop[2 * a] = (double)f[2 * a] + po;
-      kdop[2 * a + 1] = (double)f[2 * a + 1] + po;
-    }
+    BVH_TRY(copy_out(org, o->frame.p, sizeof(org), BVH_MEM_HOST, s));
+    decode_box32(ko, org, kdop);
     return BVH_OK;
   }
   return copy_out(kdop, o->rootbox64.p, 26 * sizeof(double), BVH_MEM_HOST, s);
@@ -1444,6 +1455,19 @@ bvh_status bvh_result_records(bvh_object o, bvh_narrowphase_record* out, uint64_
   return copy_out(out, o->records.p, n * sizeof(bvh_narrowphase_record), mem, s);
 }
 
+// Largest distance of an object's fp32-outward leaf slabs from their fp64
+// values, in fp32 ulps (float bits), recomputed from its element data (which
+// must still be valid); 0 for fp64 records.
+static bvh_status object_slab_error(bvh_object_s* o, uint32_t* bits) {
+  *bits = 0;
+  if (!o || !o->fp32 || !o->built || o->in.n == 0) return BVH_OK;
+  hipStream_t s = o->w->stream;
+  BVH_TRY(o->slab_err.ensure(sizeof(uint32_t)));
+  HIP_TRY(launch_fill_u32(o->slab_err.as<uint32_t>(), 1, 0u, s));
+  HIP_TRY(launch_slab_error(o->in, o->frame.as<double>(), o->slab_err.as<uint32_t>(), s));
+  return copy_out(bits, o->slab_err.p, sizeof(uint32_t), BVH_MEM_HOST, s);
+}
+
 bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch_pairs, uint64_t* candidates,
                             uint64_t* rechecked, double* max_slab_error_ulps) {
   if (!o) return fail(BVH_E_INVALID, "null object");
@@ -1454,9 +1478,16 @@ bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch
   if (candidates) *candidates = o->have_results ? (o->fp32 ? o->ncandidates : o->npairs) : 0;
   if (rechecked) *rechecked = o->have_results && o->fp32 ? o->nrecheck : 0;
   if (max_slab_error_ulps) {
+    // the queried objects' slabs, measured now (not part of the build)
+    uint32_t b0 = 0, b1 = 0;
+    if (o->have_results && o->fp32) {
+      BVH_TRY(object_slab_error(o, &b0));
+      if (o->result_other && o->result_other != o) BVH_TRY(object_slab_error(o->result_other, &b1));
+    }
+    const uint32_t bits = b0 > b1 ? b0 : b1;
     float f;
-    std::memcpy(&f, &o->slab_err_bits, sizeof(f));
-    *max_slab_error_ulps = o->have_results ? (double)f : 0.0;
+    std::memcpy(&f, &bits, sizeof(f));
+    *max_slab_error_ulps = (double)f;
   }
   return BVH_OK;
 }
@@ -1687,7 +1718,6 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
                            size_t* nnodes, size_t* root) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
   BVH_TRY(join_all(o->w));
-  if (o->fp32) return fail(BVH_E_UNSUPPORTED, "tree export needs fp64 records (BVH_PREC_FP64_RECORDS)");
   const uint64_t n = o->in.n;
   const size_t nn = n ? 2 * n - 1 : 0;
   if (nnodes) *nnodes = nn;
@@ -1697,11 +1727,35 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
   if (!kdops && !children && !ranges) return BVH_OK;
   if (cap < nn) return fail(BVH_E_OVERFLOW, "tree buffer too small");
   if (nn == 0) return BVH_OK;
-  // a node's slabs are in its parent's record (the root's in rootbox)
+  // a node's slabs are in its parent's record (the root's in rootbox); the
+  // fp32-outward records give their bounds in world coordinates (each
+  // contains the node's exact k-DOP)
   std::vector<Rec64> recs(n - 1);
   double rootk[kSlabs];
-  if (n > 1) BVH_TRY(copy_out(recs.data(), o->rec64.p, (n - 1) * sizeof(Rec64), BVH_MEM_HOST, o->w->stream));
-  BVH_TRY(copy_out(rootk, o->rootbox64.p, sizeof(rootk), BVH_MEM_HOST, o->w->stream));
+  if (o->fp32) {
+    std::vector<Rec32> r32(n - 1);
+    int32_t rk[kSlabs];
+    double org[3];
+    if (n > 1) BVH_TRY(copy_out(r32.data(), o->rec32.p, (n - 1) * sizeof(Rec32), BVH_MEM_HOST, o->w->stream));
+    BVH_TRY(copy_out(rk, o->rootbox32.p, sizeof(rk), BVH_MEM_HOST, o->w->stream));
+    BVH_TRY(copy_out(org, o->frame.p, sizeof(org), BVH_MEM_HOST, o->w->stream));
+    decode_box32(rk, org, rootk);
+    for (size_t x = 0; x + 1 < n; ++x) {
+      Rec64& R = recs[x];
+      const Rec32& Q = r32[x];
+      for (int side = 0; side < 2; ++side) {
+        R.child[side] = Q.child[side];
+        R.elem[side] = Q.elem[side];
+        if (kdops) decode_box32(Q.box[side], org, R.box[side]);
+      }
+      R.first = Q.first;
+      R.last = Q.last;
+      R.patch = Q.patch;
+    }
+  } else {
+    if (n > 1) BVH_TRY(copy_out(recs.data(), o->rec64.p, (n - 1) * sizeof(Rec64), BVH_MEM_HOST, o->w->stream));
+    BVH_TRY(copy_out(rootk, o->rootbox64.p, sizeof(rootk), BVH_MEM_HOST, o->w->stream));
+  }
   if (kdops && r < nn) std::memcpy(kdops + 26 * (size_t)r, rootk, sizeof(rootk));
   for (size_t i = n - 1; i < nn; ++i) {
     if (children) children[2 * i] = children[2 * i + 1] = -1;
@@ -1729,6 +1783,7 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t* vals, uint64_t n) {
   if (!w) return fail(BVH_E_INVALID, "null world");
   BVH_TRY(join_all(w));
+  if (w->capturing) return fail(BVH_E_STATE, "scan inside a capture");
   if (n == 0) return BVH_OK;
   if (!vals) return fail(BVH_E_INVALID, "null values");
   hipStream_t s = w->stream;

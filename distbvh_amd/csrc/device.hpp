@@ -392,6 +392,41 @@ __device__ __forceinline__ void ord32_slabs(const float* kf, int32_t* ko) {
   }
 }
 
+// Inverse of ord32 (NaN images excluded).
+__device__ __forceinline__ float from_ord32(int32_t k) {
+  const uint32_t u = k >= 0 ? (uint32_t)k : (0x80000000u | (uint32_t)(-(int64_t)k));
+  return __uint_as_float(u);
+}
+
+// The ordered images q of one element's outward fp32 slabs in the frame of
+// origin oq, moved to the frame of origin ot and still outward.  With pq, pt
+// the origins' fp64 projections, the exact slab x has x - pq >= f (lower
+// slab), so x - pt >= f + (pq - pt).  (s, e) is the exact TwoSum of
+// pq - pt; (f + s) + e, rounded to nearest in fp64, errs by far less than
+// half an fp32 ulp of the result, and moving its fp32 rounding one ulp out
+// keeps the bound.  Upper slabs likewise.  NaN images (INT32_MAX lower,
+// INT32_MIN upper) stay as they are.
+template <int A = 0>
+__device__ __forceinline__ void reframe32(int32_t* q, const double* oq, const double* ot) {
+  // one axis at a time, so that only this axis's projections are live
+  const double pq = project<A>(oq[0], oq[1], oq[2]);
+  const double pt = project<A>(ot[0], ot[1], ot[2]);
+  const double s = pq - pt;
+  const double bb = s - pq;
+  const double e = (pq - (s - bb)) + (-pt - bb);
+  const int32_t klo = q[2 * A], khi = q[2 * A + 1];
+  if (klo != 0x7fffffff) {
+    const float f = round_down_1ulp(((double)from_ord32(klo) + s) + e);
+    q[2 * A] = (f == f) ? ord32(f) : 0x7fffffff;
+  }
+  if (khi != (int32_t)0x80000000) {
+    const float f = round_up_1ulp(((double)from_ord32(khi) + s) + e);
+    q[2 * A + 1] = (f == f) ? ord32(f) : (int32_t)0x80000000;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (A + 1 < kAxes) reframe32<A + 1>(q, oq, ot);
+}
+
 __device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {
 #pragma unroll
   for (int a = 0; a < kAxes; ++a) {

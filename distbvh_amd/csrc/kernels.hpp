@@ -118,6 +118,9 @@ hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hip
 // parents' records
 hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
                                hipStream_t s);
+// largest fp32-outward slab error of an object's elements in fp32 ulps (atomic
+// max of the float bits into *out, which the caller zeroes)
+hipError_t launch_slab_error(const ElemInput& in, const double* org, uint32_t* out, hipStream_t s);
 // every leaf's fp64 slabs recomputed from its element, sorted order (n x 26)
 hipError_t launch_sorted_elem_kdops(const ElemInput& in, const uint32_t* perm, double* out, hipStream_t s);
 uint64_t prep_blocks(uint64_t n);

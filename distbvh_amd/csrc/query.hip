@@ -73,6 +73,28 @@ __device__ __forceinline__ void wave_flush(uint64_t* buf, uint32_t& fill, const 
   fill = 0;
 }
 
+// End of a kernel whose waves all reach this point together (every thread of
+// the block calls it): the block's waves append their staged pairs with one
+// atomic for the whole block (the pair counter is one address; its atomics
+// are serialized, about 12 ns each).
+__device__ __forceinline__ void block_flush(uint64_t* buf, uint32_t fill, const PairOut& out) {
+  __shared__ uint32_t s_fill[256 / kWave];
+  __shared__ unsigned long long s_base;
+  const uint32_t wv = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  if (lane_id() == 0) s_fill[wv] = fill;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t v = 0; v < nw; ++v) tot += s_fill[v];
+    s_base = tot ? atomicAdd(out.count, (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  unsigned long long base = s_base;
+  for (uint32_t v = 0; v < wv; ++v) base += s_fill[v];
+  for (uint32_t i = lane_id(); i < fill; i += kWave)
+    if (base + i < out.cap) out.pairs[base + i] = buf[i];
+}
+
 __device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const PairOut& out, bool pred,
                                           uint64_t key) {
   const uint64_t m = __ballot(pred);
@@ -83,9 +105,8 @@ __device__ __forceinline__ void wave_emit(uint64_t* buf, uint32_t& fill, const P
   fill += cnt;
 }
 
-// fp32-outward form.  Node slabs are fp32 in the frame of the world's origin P
-// (one frame for every object of a world), rounded outward, so an fp32 miss
-// proves an fp64 miss: with q, t the exact slabs and f(.) the rounded ones,
+// fp32-outward form.  Node slabs are fp32 in the frame of their object's
+// origin P, rounded outward, so an fp32 miss proves an fp64 miss: with q, t the exact slabs and f(.) the rounded ones,
 // q_lo < t_hi implies f(q_lo) <= q_lo - P < t_hi - P <= f(t_hi).  The query's
 // slabs are its own fp32 leaf slabs.
 //
@@ -221,6 +242,7 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
   uint32_t fill = 0, rfill = 0;
   const uint64_t p = (uint64_t)tile * blockDim.x + threadIdx.x;
   const bool valid = p < q.n;
+  if (q.origin != t.origin) inner = 0x7fffffff;
   int32_t qf[kSlabs];  // ord32 images of the query's outward fp32 slabs
   uint32_t qa = 0;
   if (valid) {
@@ -239,6 +261,10 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
       qf[2 * a + 1] = v.y;
     }
     qa = q.perm[p];
+    // another object's tree has its own frame: the slabs moved to it, still
+    // outward (their fp32 rounding then bounds nothing from the inside, so
+    // every candidate of such a query is decided in fp64)
+    if (q.origin != t.origin) reframe32(qf, q.origin, t.origin);
   } else {
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {  // the empty box overlaps nothing
@@ -752,6 +778,10 @@ __global__ __launch_bounds__(256) void k_narrow64s(TreeView q, TreeView t, int x
 // sized for it); kGeneric: any mix, chosen at run time.
 constexpr int kGeneric = -1;
 
+// blocks per re-check stripe (2,048 in all): each block strides over its
+// stripe's list, so a block stages many pairs per flush
+constexpr uint64_t kRecheckBlocksPerStripe = 32;
+
 template <int KIND, int NV>
 __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut out) {
   __shared__ uint64_t s_buf[256 / kWave][kWaveBuf];
@@ -785,7 +815,7 @@ __global__ __launch_bounds__(256) void k_recheck(TreeView q, TreeView t, PairOut
     }
     wave_emit(buf, fill, out, hit, key);
   }
-  wave_flush(buf, fill, out);
+  block_flush(buf, fill, out);
 }
 
 static void launch_recheck(const TreeView& q, const TreeView& t, const PairOut& out, unsigned blocks, hipStream_t s) {
@@ -902,9 +932,22 @@ __global__ __launch_bounds__(256) void k_digest(const uint64_t* __restrict__ pai
     s += __shfl_xor(s, o, 64);
     x ^= __shfl_xor(x, o, 64);
   }
-  if (lane_id() == 0 && (s | x) != 0) {
-    atomicAdd(out, (unsigned long long)s);
-    atomicXor(out + 1, (unsigned long long)x);
+  // one pair of atomics per block (the two addresses serialize them)
+  __shared__ uint64_t ss[256 / kWave], sx[256 / kWave];
+  if (lane_id() == 0) {
+    ss[threadIdx.x / kWave] = s;
+    sx[threadIdx.x / kWave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t v = 1; v < blockDim.x / kWave; ++v) {
+      s += ss[v];
+      x ^= sx[v];
+    }
+    if ((s | x) != 0) {
+      atomicAdd(out, (unsigned long long)s);
+      atomicXor(out + 1, (unsigned long long)x);
+    }
   }
 }
 
@@ -928,7 +971,7 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   const unsigned blocks = (unsigned)((q.n + bs - 1) / bs);
   const int xcd = shape.xcd_remap ? 1 : 0;
   if (form == kFormFp32 || form == kFormMixed) {
-    if (!q.rec32 || !q.rootbox32 || !q.delta || !t.rec32 || !t.rootbox32 || !t.origin || q.origin != t.origin)
+    if (!q.rec32 || !q.rootbox32 || !q.delta || !t.rec32 || !t.rootbox32 || !q.origin || !t.origin)
       return hipErrorInvalidValue;
     if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
@@ -941,7 +984,7 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
     // in self mode both keys index the one object (q == t)
     uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
-    if (per > 8192 / kRecheckStripes) per = 8192 / kRecheckStripes;
+    if (per > kRecheckBlocksPerStripe) per = kRecheckBlocksPerStripe;
     if (per == 0) per = 1;
     launch_recheck(q, t, out, (unsigned)(per * kRecheckStripes), s);
     return hipGetLastError();
@@ -978,7 +1021,7 @@ hipError_t launch_recheck_query(const TreeView& q, const TreeView& t, const Pair
   if (q.n == 0 || t.n == 0) return hipSuccess;
   if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
   uint64_t per = (out.rcap + 255) / 256;  // blocks per stripe
-  if (per > 8192 / kRecheckStripes) per = 8192 / kRecheckStripes;
+  if (per > kRecheckBlocksPerStripe) per = kRecheckBlocksPerStripe;
   if (per == 0) per = 1;
   launch_recheck(q, t, out, (unsigned)(per * kRecheckStripes), s);
   return hipGetLastError();
@@ -1008,7 +1051,7 @@ hipError_t launch_records(const uint64_t* pairs, uint64_t count, const uint32_t*
 hipError_t launch_digest(const uint64_t* pairs, uint64_t count, unsigned long long* out, hipStream_t s) {
   if (count == 0) return hipSuccess;
   uint64_t blocks = (count + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
+  if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_digest, dim3((unsigned)blocks), dim3(256), 0, s, pairs, count, out);
   return hipGetLastError();
 }

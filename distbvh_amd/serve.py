@@ -113,11 +113,13 @@ class Server:
             self._release(min(idle, key=lambda m: self.cache[m]["last_use"]))
         return True
 
-    def step(self, reqs, on_done=None):
+    def step(self, reqs, on_done=None, digests=False):
         """One coalesced step.  reqs: list of (mesh id, A vertices, B vertices)
         with distinct mesh ids, all admitted.  Returns each request's (k, 2)
         pairs; on_done(i, pairs) is called as request i's pairs reach the
-        host."""
+        host.  With `digests`, returns (pairs, digests) instead: each
+        request's order-independent device digest (bvh_result_digest),
+        computed after every request of the step reached the host."""
         self.steps += 1
         entries = []
         refitted = []
@@ -163,6 +165,8 @@ class Server:
             if on_done is not None:
                 on_done(i, pairs)
             out.append(pairs)
+        if digests:
+            return out, [e["a"].digest() for e in entries]
         return out
 
     def close(self):
@@ -170,14 +174,25 @@ class Server:
         self.world.close()
 
 
-def _pair_digest(pairs):
-    """Order-independent digest of a pair list: SHA-256 of the sorted rows."""
-    import hashlib
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    return z ^ (z >> np.uint64(31))
 
-    p = np.ascontiguousarray(pairs, dtype=np.uint64)
-    if len(p):
-        p = p[np.lexsort((p[:, 1], p[:, 0]))]
-    return hashlib.sha256(p.tobytes()).hexdigest()[:16]
+
+def pair_digest(pairs):
+    """Host form of the device digest bvh_result_digest of (k, 2) element
+    pairs: the sum and the xor of two 64-bit mixes (splitmix64 finalizer) of
+    every key a << 32 | b.  Order-independent, so a request's device digest
+    can be compared with the digest of pairs computed elsewhere."""
+    p = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    if len(p) == 0:
+        return 0, 0
+    keys = (p[:, 0] << np.uint64(32)) | p[:, 1]
+    with np.errstate(over="ignore"):
+        s = int(_mix64(keys).sum(dtype=np.uint64))
+        x = int(np.bitwise_xor.reduce(_mix64(keys ^ np.uint64(0x9e3779b97f4a7c15))))
+    return s, x
 
 
 def serve(rate=200.0, duration=2.0, k=8, lo=1e4, hi=2e6, variants=4, seed=0xC5, precision="fp64", od=64,
@@ -264,10 +279,11 @@ def serve(rate=200.0, duration=2.0, k=8, lo=1e4, hi=2e6, variants=4, seed=0xC5, 
                 r = batch[i]
                 lat[r] = (time.perf_counter() - t0) - arrivals[r]
                 counts[r] = len(pairs)
-                if record:
-                    digests[r] = _pair_digest(pairs)
 
-            srv.step(reqs, done)
+            res = srv.step(reqs, done, digests=record)
+            if record:
+                for r, dig in zip(batch, res[1]):
+                    digests[r] = dig
             step_sizes.append(len(batch))
             served += len(batch)
         wall = time.perf_counter() - t0

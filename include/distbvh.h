@@ -228,11 +228,13 @@ bvh_status bvh_result_digest(bvh_object o, uint64_t digest[2]);
 /* device pointer to the compact pairs (valid until the next broadphase) */
 bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_t *count);
 /* counters of the last query: tree nodes visited (summed over waves of 64
- * queries), broadphase patch pairs, candidate pairs (fp32 form: pairs whose
- * outward-rounded fp32 slabs overlap; fp64 form: the pair count), rechecked
- * pairs (fp32 form: candidates too close to decide in fp32, tested again on
- * fp64 k-DOPs; 0 in the fp64 form) and the largest fp32 slab error in ulps of
- * the objects' last builds (0 in the fp64 form).  Any pointer may be NULL. */
+ * queries), broadphase patch pairs, candidate pairs (fp32-outward records:
+ * pairs whose outward-rounded fp32 slabs overlap; fp64 records: the pair
+ * count), rechecked pairs (candidates decided on fp64 k-DOPs: every candidate
+ * in BVH_PREC_FP64, those too close for fp32 in BVH_PREC_FP32_OUTWARD, 0 with
+ * fp64 records) and the largest fp32 slab error in ulps of the two objects'
+ * leaf slabs (0 with fp64 records; measured by this call from the element
+ * data, which must still be valid).  Any pointer may be NULL. */
 bvh_status bvh_result_stats(bvh_object o, uint64_t *node_visits, uint64_t *patch_pairs,
                             uint64_t *candidate_pairs, uint64_t *rechecked_pairs,
                             double *max_slab_error_ulps);
@@ -287,7 +289,10 @@ bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node *nodes, size_t cap,
                                  uint64_t *leaf_patches, size_t leaf_cap, size_t *nleaves);
 /* the element-level tree (no reference counterpart: the reference's element
  * trees are per patch pair): node count,
- * then per node kdop (26), children (-1 for none) and leaf range [first, last]. */
+ * then per node kdop (26), children (-1 for none) and leaf range [first, last].
+ * With fp64 records the k-DOPs are the exact slab unions; with fp32-outward
+ * records (BVH_PREC_FP64, BVH_PREC_FP32_OUTWARD) they are those bounds in
+ * world coordinates, each containing the node's exact k-DOP. */
 bvh_status bvh_object_tree(bvh_object o, double *kdops, int64_t *children, uint64_t *ranges,
                            size_t cap, size_t *nnodes, size_t *root);
 

@@ -227,3 +227,39 @@ def test_batch_workload_full_size_fp32_matches_fp64(torch_cuda):
         torch.cuda.empty_cache()
     assert res["fp64"][0] > 0
     assert res["fp32"] == res["fp64"]
+
+
+# ---- object-local frames ---------------------------------------------------------------------
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_object_frames_far_apart(torch_cuda, precision):
+    """Each object's fp32 slabs are relative to its own origin.  An object
+    built first at the origin must not coarsen the slabs of two objects
+    built later 1e4 away: their pairs (A x B across frames, and A's self
+    collision) equal the oracle's brute force, and the fp32 candidates stay
+    within a few percent of the pairs (one world frame at the first object
+    would round those slabs to fp32 ulps of about 1e-3, the spheres'
+    radius)."""
+    c = W.sphere_cloud(4000, radius=1e-3, ncomp=4, sigma=0.01, shift=(0.0, 0.0, 0.0), seed=0x61)
+    a = W.sphere_cloud(12000, radius=1e-3, ncomp=8, sigma=0.01, shift=(1e4, -1e4, 1e4), seed=0x62)
+    b = W.sphere_cloud(12000, radius=1e-3, ncomp=8, sigma=0.01, shift=(1e4 + 0.0013, -1e4 + 0.0007, 1e4), seed=0x62)
+    ka, _ = O.elements_sphere(a)
+    kb, _ = O.elements_sphere(b)
+    want_ab = sort_pairs(O.brute_pairs(ka, kb))
+    want_aa = sort_pairs(O.brute_self(ka))
+    assert len(want_ab) > 0 and len(want_aa) > 0
+    with CollisionWorld(8, precision=precision, serial_builds=True) as w:
+        oc = gpu_object(w, "sphere", c, device=True)  # built first, at the origin
+        oa = gpu_object(w, "sphere", a, device=True)
+        ob = gpu_object(w, "sphere", b, device=True)
+        oa.broadphase(ob)
+        got_ab = sort_pairs(oa.pairs())
+        st_ab = oa.result_stats()
+        oa.self_collision()
+        got_aa = sort_pairs(oa.pairs())
+        st_aa = oa.result_stats()
+        oc.self_collision()
+    assert np.array_equal(got_ab, want_ab)
+    assert np.array_equal(got_aa, want_aa)
+    for st, want in ((st_ab, want_ab), (st_aa, want_aa)):
+        assert len(want) <= st["candidate_pairs"] <= 1.05 * len(want) + 16
+        assert 0.0 < st["max_slab_error_ulps"] <= 2.0

@@ -35,7 +35,7 @@ def _expected(pool, seed, variants, m, v, brute=True):
     else:
         recs = O.Object(ka, ca, od=64, nthreads=16).collide(O.Object(kb, cb, od=64, nthreads=16))
         exp = recs[:, [1, 3]]
-    return len(exp), S._pair_digest(exp)
+    return len(exp), S.pair_digest(exp)
 
 
 def test_serving_alone_and_in_the_mix(torch_cuda):

@@ -34,3 +34,24 @@ def test_poisson_arrival_rate():
     gaps = -np.log(1.0 - W.uniform(0x1, n)) / rate
     assert abs(gaps.mean() - 1 / rate) < 0.05 / rate
     assert abs(gaps.std() - 1 / rate) < 0.05 / rate
+
+
+def test_pair_digest_is_order_independent_host_form():
+    """serve.pair_digest restates the device digest (sum and xor of two
+    splitmix64 mixes of a << 32 | b): independent of the pair order, changed
+    by any changed pair, and (0, 0) for no pairs."""
+    rng = np.random.default_rng(5)
+    p = rng.integers(0, 1 << 31, size=(1000, 2), dtype=np.uint64)
+    d = S.pair_digest(p)
+    assert d == S.pair_digest(p[rng.permutation(len(p))])
+    q = p.copy()
+    q[17, 1] += np.uint64(1)
+    assert S.pair_digest(q) != d
+    assert S.pair_digest(np.zeros((0, 2), np.uint64)) == (0, 0)
+    # one key by hand: the splitmix64 finalizer of 1 << 32 | 2
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xbf58476d1ce4e5b9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94d049bb133111eb) & (2**64 - 1)
+        return z ^ (z >> 31)
+    k = (1 << 32) | 2
+    assert S.pair_digest(np.array([[1, 2]], np.uint64)) == (mix(k), mix(k ^ 0x9e3779b97f4a7c15))
