@@ -116,6 +116,7 @@ SIGNATURES = {
     "bvh_object_sorted_kdops": [_vp, _vp, _u32],
     "bvh_object_tree": [_vp, _vp, _vp, _vp, _sz, C.POINTER(_sz), C.POINTER(_sz)],
     "bvh_sort_u64": [_vp, _vp, _vp, _u64],
+    "bvh_sort_u64_noalloc": [_vp, _vp, _vp, _u64],
     "bvh_sort_u32": [_vp, _vp, _vp, _u64],
     "bvh_exclusive_scan_u32": [_vp, _vp, _u64],
     "bvh_closest_point": [_vp, _vp, _u64, _vp, _vp, _u32],

@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 #include <atomic>
+#include <mutex>
 
 #include "../../include/distbvh.h"
 #include "kernels.hpp"
@@ -1004,6 +1005,70 @@ static bvh_status prim_sort(bvh_world w, K* keys, uint32_t* vals, uint64_t n) {
   return sort_in_place<K>(w, keys, vals, n, sc);
 }
 
+// ---- the sort that allocates nothing (bvh_sort_u64_noalloc) -----------------------------------
+// Its scratch is one file-scope device array: the code object's own memory on
+// every GPU the library runs on, held until the process ends, so a call makes
+// no device allocation.  Carved, 256-byte aligned, for at most
+// kNoAllocMaxKeys 64-bit keys: key and value ping-pong buffers, the digit
+// histograms, the control block and the look-back words of 8 passes.
+// In host code `scratch` names a host-side stand-in; the device address of
+// each GPU's copy comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr uint64_t kNoAllocMaxKeys = 1ull << 20;
+constexpr uint64_t kNoAllocTiles = (kNoAllocMaxKeys + 4095) / 4096;  // the sort's 4,096-key tiles
+constexpr uint64_t na_align(uint64_t b) { return (b + 255) & ~255ull; }
+constexpr uint64_t kNaKa = 0;
+constexpr uint64_t kNaKb = kNaKa + na_align(kNoAllocMaxKeys * 8);
+constexpr uint64_t kNaVa = kNaKb + na_align(kNoAllocMaxKeys * 8);
+constexpr uint64_t kNaVb = kNaVa + na_align(kNoAllocMaxKeys * 4);
+constexpr uint64_t kNaHist = kNaVb + na_align(kNoAllocMaxKeys * 4);
+constexpr uint64_t kNaCtl = kNaHist + na_align(8 * 256 * 4);
+constexpr uint64_t kNaLb = kNaCtl + na_align(sizeof(SortCtl));
+constexpr uint64_t kNoAllocBytes = kNaLb + na_align(8 * kNoAllocTiles * 256 * 4);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_na_mutex;
+void* g_na_ptr[kNoAllocDevices] = {};
+
+// Device address of `scratch` on the current device (set up once per device,
+// with that device current).
+static bvh_status noalloc_setup_current() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(BVH_E_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lk(g_na_mutex);
+  if (g_na_ptr[dev]) return BVH_OK;
+  void* p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(BVH_E_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+  g_na_ptr[dev] = p;
+  return BVH_OK;
+}
+
+// The scratch of the world's device; set up by bvh_world_create, or here
+// with the world's device made current for the call.
+static bvh_status noalloc_scratch(bvh_world w, void** out) {
+  const int dev = (int)w->cfg.device;
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(BVH_E_UNSUPPORTED, "device index above 63");
+  {
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (g_na_ptr[dev]) {
+      *out = g_na_ptr[dev];
+      return BVH_OK;
+    }
+  }
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != dev) HIP_TRY(hipSetDevice(dev));
+  const bvh_status st = noalloc_setup_current();
+  if (cur != dev) HIP_TRY(hipSetDevice(cur));
+  BVH_TRY(st);
+  std::lock_guard<std::mutex> lk(g_na_mutex);
+  *out = g_na_ptr[dev];
+  return BVH_OK;
+}
+
 // device copy of a caller array: the array itself (device memory) or the
 // staging buffer it was uploaded into (host memory)
 template <typename T>
@@ -1070,6 +1135,7 @@ bvh_status bvh_world_create(size_t od, const bvh_world_config* cfg, bvh_world* o
     HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
     w->own_stream = true;
   }
+  BVH_TRY(noalloc_setup_current());  // the device address of bvh_sort_u64_noalloc's scratch
   void* hc = nullptr;
   HIP_TRY(hipHostMalloc(&hc, 16 * sizeof(unsigned long long), hipHostMallocDefault));
   w->h_count = (unsigned long long*)hc;
@@ -2420,6 +2486,24 @@ bvh_status bvh_graph_destroy(bvh_graph g) {
 }
 
 bvh_status bvh_sort_u64(bvh_world w, uint64_t* keys, uint32_t* vals, uint64_t n) { return prim_sort<uint64_t>(w, keys, vals, n); }
+
+bvh_status bvh_sort_u64_noalloc(bvh_world w, uint64_t* keys, uint32_t* vals, uint64_t n) {
+  if (!w) return fail(BVH_E_INVALID, "null world");
+  BVH_TRY(join_all(w));
+  if (n > kNoAllocMaxKeys)
+    return fail(BVH_E_INVALID, "bvh_sort_u64_noalloc sorts at most " + std::to_string(kNoAllocMaxKeys) +
+                                   " keys (" + std::to_string(n) + " given); use bvh_sort_u64");
+  if (w->capturing) return fail(BVH_E_STATE, "sort inside a capture");
+  if (n == 0) return BVH_OK;
+  if (!keys) return fail(BVH_E_INVALID, "null keys");
+  if (sort_tiles(n) > kNoAllocTiles) return fail(BVH_E_INVALID, "sort tiling does not fit the scratch");
+  void* base = nullptr;
+  BVH_TRY(noalloc_scratch(w, &base));
+  unsigned char* b = static_cast<unsigned char*>(base);
+  const SortScratch sc{b + kNaKa, b + kNaKb, (uint32_t*)(b + kNaVa), (uint32_t*)(b + kNaVb),
+                       (uint32_t*)(b + kNaHist), (SortCtl*)(b + kNaCtl), (uint32_t*)(b + kNaLb)};
+  return sort_in_place<uint64_t>(w, keys, vals, n, sc);
+}
 bvh_status bvh_sort_u32(bvh_world w, uint32_t* keys, uint32_t* vals, uint64_t n) { return prim_sort<uint32_t>(w, keys, vals, n); }
 
 }  // extern "C"

@@ -301,6 +301,16 @@ bvh_status bvh_object_tree(bvh_object o, double *kdops, int64_t *children, uint6
  * uint32 payload (payload may be NULL), in place. */
 bvh_status bvh_sort_u64(bvh_world w, uint64_t *keys, uint32_t *vals, uint64_t n);
 bvh_status bvh_sort_u32(bvh_world w, uint32_t *keys, uint32_t *vals, uint64_t n);
+/* bvh_sort_u64 without any device allocation; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The scratch is one static device array per GPU (26 MB, the library's own
+ * code-object memory, held until the process ends), so n is at most
+ * 1,048,576 (2^20) keys; larger n gives BVH_E_INVALID.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (two host threads,
+ * or two worlds on the same device) race and must be serialized by the
+ * caller.  Same result as bvh_sort_u64; waits for it. */
+bvh_status bvh_sort_u64_noalloc(bvh_world w, uint64_t *keys, uint32_t *vals, uint64_t n);
 /* prefix_sum, util/prefix_sum.hpp:41-54 (exclusive), in place */
 bvh_status bvh_exclusive_scan_u32(bvh_world w, uint32_t *vals, uint64_t n);
 
