@@ -550,29 +550,36 @@ __device__ __forceinline__ void rec_store_half(const BuildBufs& b, uint32_t pare
                                                uint32_t child, uint32_t elem) {
   RecT<FP32>* R = recs_of<FP32>(b) + parent;
   if constexpr (FP32) {
-    int2* d = (int2*)R->box[side];  // 104-byte halves: 8-byte aligned
+    // 128-byte half: slabs, child id and element as seven 16-byte stores
+    int4* d = (int4*)R->h[side].box;
 #pragma unroll
-    for (int q = 0; q < kAxes; ++q) d[q] = make_int2(k[2 * q], k[2 * q + 1]);
+    for (int q = 0; q < 6; ++q) d[q] = make_int4(k[4 * q], k[4 * q + 1], k[4 * q + 2], k[4 * q + 3]);
+    d[6] = make_int4(k[24], k[25], (int32_t)child, (int32_t)elem);
   } else {
     double2* d = (double2*)R->box[side];  // 208-byte halves: 16-byte aligned
 #pragma unroll
     for (int q = 0; q < kAxes; ++q) d[q] = make_double2(k[2 * q], k[2 * q + 1]);
+    R->child[side] = child;
+    R->elem[side] = elem;
   }
-  R->child[side] = child;
-  R->elem[side] = elem;
 }
 
 template <bool FP32>
 __device__ __forceinline__ void rec_load_half(const BuildBufs& b, uint32_t parent, int side, SlabT<FP32>* k) {
   const RecT<FP32>* R = recs_of<FP32>(b) + parent;
   if constexpr (FP32) {
-    const int2* d = (const int2*)R->box[side];
+    const int4* d = (const int4*)R->h[side].box;
 #pragma unroll
-    for (int q = 0; q < kAxes; ++q) {
-      const int2 v = d[q];
-      k[2 * q] = v.x;
-      k[2 * q + 1] = v.y;
+    for (int q = 0; q < 6; ++q) {
+      const int4 v = d[q];
+      k[4 * q] = v.x;
+      k[4 * q + 1] = v.y;
+      k[4 * q + 2] = v.z;
+      k[4 * q + 3] = v.w;
     }
+    const int2 v = *(const int2*)(R->h[side].box + 24);
+    k[24] = v.x;
+    k[25] = v.y;
   } else {
     const double2* d = (const double2*)R->box[side];
 #pragma unroll
@@ -692,9 +699,7 @@ __device__ __forceinline__ void climb(const BuildBufs& b, uint64_t n, uint32_t l
     }
     slab_union(kd, ks);
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    recs[parent].first = l;
-    recs[parent].last = r;
-    recs[parent].patch = mypatch;
+    rec_set_range(recs[parent], l, r, mypatch);
     if (is_patch_root(mypatch, l, r, n, dl, dr)) {
       b.patch_root[mypatch] = parent;
       store_patch_box<FP32>(b, mypatch, kd);
@@ -888,9 +893,7 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     if (is_left) r = slast;
     else l = c0 + ss;
     mypatch = (mypatch == spatch) ? mypatch : kMixed;
-    recs[parent].first = l;
-    recs[parent].last = r;
-    recs[parent].patch = mypatch;
+    rec_set_range(recs[parent], l, r, mypatch);
     const uint32_t ndl = L.delta[l - c0], ndr = L.delta[r - c0 + 1];
     if (is_patch_root(mypatch, l, r, n, ndl, ndr)) {
       b.patch_root[mypatch] = parent;
@@ -1239,7 +1242,7 @@ __global__ void k_patch_tree(BuildBufs b, uint64_t n, uint64_t od, int64_t* __re
     out[4 * k + 1] = -1;
     out[4 * k + 2] = -1;
     if (par != ~0ull) out[4 * (par >> 1) + 1 + (par & 1)] = (int64_t)k;
-    const uint32_t pv = x >= n - 1 ? b.patch_of_pos[x - (n - 1)] : recs[x].patch;
+    const uint32_t pv = x >= n - 1 ? b.patch_of_pos[x - (n - 1)] : rec_patch(recs[x]);
     if (pv != kMixed) {
       out[4 * k + 3] = pv;
     } else {
@@ -1248,10 +1251,10 @@ __global__ void k_patch_tree(BuildBufs b, uint64_t n, uint64_t od, int64_t* __re
         atomicOr(b.status, kErrTree);
         return;
       }
-      snode[sp] = recs[x].child[1];  // right below left: left comes out first
+      snode[sp] = rec_child(recs[x], 1);  // right below left: left comes out first
       spar[sp] = (k << 1) | 1;
       ++sp;
-      snode[sp] = recs[x].child[0];
+      snode[sp] = rec_child(recs[x], 0);
       spar[sp] = k << 1;
       ++sp;
     }

@@ -1810,13 +1810,13 @@ bvh_status bvh_object_tree(bvh_object o, double* kdops, int64_t* children, uint6
       Rec64& R = recs[x];
       const Rec32& Q = r32[x];
       for (int side = 0; side < 2; ++side) {
-        R.child[side] = Q.child[side];
-        R.elem[side] = Q.elem[side];
-        if (kdops) decode_box32(Q.box[side], org, R.box[side]);
+        R.child[side] = rec_child(Q, side);
+        R.elem[side] = rec_elem(Q, side);
+        if (kdops) decode_box32(rec_box(Q, side), org, R.box[side]);
       }
-      R.first = Q.first;
-      R.last = Q.last;
-      R.patch = Q.patch;
+      R.first = rec_first(Q);
+      R.last = rec_last(Q);
+      R.patch = rec_patch(Q);
     }
   } else {
     if (n > 1) BVH_TRY(copy_out(recs.data(), o->rec64.p, (n - 1) * sizeof(Rec64), BVH_MEM_HOST, o->w->stream));

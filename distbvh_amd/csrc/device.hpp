@@ -179,15 +179,22 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
 // slabs live in its parent's record (the root's in a separate array).
 // fp32 form: 256 B, read with scalar loads; fp64 form: 448 B, staged through
 // LDS by a wave-wide vector load.
+// Rec32: two 128-byte halves, one per child, each written as seven 16-byte
+// stores (slabs, child id, element) by the child that completes: a half is
+// the child's ordered fp32 slab images (104 B), its node id and, for a leaf,
+// its input index; half 0 also holds this node's range and patch (one 16-byte
+// store by the node's completer).
 struct alignas(256) Rec32 {
-  int32_t box[2][kSlabs];  // left, right child: ordered images of fp32 slabs (208 B)
-  uint32_t child[2];     // child node ids (leaf p = n-1+p)
-  uint32_t elem[2];      // input index of a leaf child
-  uint32_t first, last;  // this node's range of sorted positions
-  uint32_t patch;        // patch id, or kMixed
-  uint32_t pad;
+  struct alignas(16) Half {
+    int32_t box[kSlabs];  // ordered images of the child's fp32 slabs
+    uint32_t child;       // child node id (leaf p = n-1+p)
+    uint32_t elem;        // input index of a leaf child
+    uint32_t first, last; // half 0: this node's range of sorted positions
+    uint32_t patch;       // half 0: patch id, or kMixed
+    uint32_t pad;
+  } h[2];
 };
-static_assert(sizeof(Rec32) == 256, "record layout");
+static_assert(sizeof(Rec32) == 256 && sizeof(Rec32::Half) == 128, "record layout");
 
 struct alignas(64) Rec64 {
   double box[2][kSlabs];  // left, right child (416 B)
@@ -198,6 +205,29 @@ struct alignas(64) Rec64 {
   uint32_t pad;
 };
 static_assert(sizeof(Rec64) == 448, "record layout");
+
+// field access common to both record formats
+__host__ __device__ __forceinline__ uint32_t rec_child(const Rec64& r, int s) { return r.child[s]; }
+__host__ __device__ __forceinline__ uint32_t rec_child(const Rec32& r, int s) { return r.h[s].child; }
+__host__ __device__ __forceinline__ uint32_t rec_elem(const Rec64& r, int s) { return r.elem[s]; }
+__host__ __device__ __forceinline__ uint32_t rec_elem(const Rec32& r, int s) { return r.h[s].elem; }
+__host__ __device__ __forceinline__ uint32_t rec_patch(const Rec64& r) { return r.patch; }
+__host__ __device__ __forceinline__ uint32_t rec_patch(const Rec32& r) { return r.h[0].patch; }
+__host__ __device__ __forceinline__ uint32_t rec_first(const Rec64& r) { return r.first; }
+__host__ __device__ __forceinline__ uint32_t rec_first(const Rec32& r) { return r.h[0].first; }
+__host__ __device__ __forceinline__ uint32_t rec_last(const Rec64& r) { return r.last; }
+__host__ __device__ __forceinline__ uint32_t rec_last(const Rec32& r) { return r.h[0].last; }
+__host__ __device__ __forceinline__ const double* rec_box(const Rec64& r, int s) { return r.box[s]; }
+__host__ __device__ __forceinline__ const int32_t* rec_box(const Rec32& r, int s) { return r.h[s].box; }
+// this node's range and patch, written by its completer
+__device__ __forceinline__ void rec_set_range(Rec64& r, uint32_t first, uint32_t last, uint32_t patch) {
+  r.first = first;
+  r.last = last;
+  r.patch = patch;
+}
+__device__ __forceinline__ void rec_set_range(Rec32& r, uint32_t first, uint32_t last, uint32_t patch) {
+  *(uint4*)&r.h[0].first = make_uint4(first, last, patch, 0u);
+}
 
 // Where leaf p's slabs live: the half `side` of internal node `parent`'s
 // record.  Leaf p is the left child of internal node p when delta(p) <
@@ -405,12 +435,14 @@ __device__ __forceinline__ float from_ord32(int32_t k) {
 // pq - pt; (f + s) + e, rounded to nearest in fp64, errs by far less than
 // half an fp32 ulp of the result, and moving its fp32 rounding one ulp out
 // keeps the bound.  Upper slabs likewise.  NaN images (INT32_MAX lower,
-// INT32_MIN upper) stay as they are.
+// INT32_MIN upper) stay as they are.  (ox, oy, oz) and (tx, ty, tz) are the
+// two origins.
 template <int A = 0>
-__device__ __forceinline__ void reframe32(int32_t* q, const double* oq, const double* ot) {
+__device__ __forceinline__ void reframe32(int32_t* q, double ox, double oy, double oz, double tx, double ty,
+                                          double tz) {
   // one axis at a time, so that only this axis's projections are live
-  const double pq = project<A>(oq[0], oq[1], oq[2]);
-  const double pt = project<A>(ot[0], ot[1], ot[2]);
+  const double pq = project<A>(ox, oy, oz);
+  const double pt = project<A>(tx, ty, tz);
   const double s = pq - pt;
   const double bb = s - pq;
   const double e = (pq - (s - bb)) + (-pt - bb);
@@ -424,7 +456,7 @@ __device__ __forceinline__ void reframe32(int32_t* q, const double* oq, const do
     q[2 * A + 1] = (f == f) ? ord32(f) : (int32_t)0x80000000;
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (A + 1 < kAxes) reframe32<A + 1>(q, oq, ot);
+  if constexpr (A + 1 < kAxes) reframe32<A + 1>(q, ox, oy, oz, tx, ty, tz);
 }
 
 __device__ __forceinline__ void to_fp32_outward(const double* kd, const double* po, float* kf) {

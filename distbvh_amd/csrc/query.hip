@@ -252,7 +252,7 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
       uint32_t parent;
       int side;
       leaf_slot(q.delta, q.n, p, parent, side);
-      src = q.rec32[parent].box[side];
+      src = q.rec32[parent].h[side].box;
     }
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {
@@ -264,7 +264,8 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
     // another object's tree has its own frame: the slabs moved to it, still
     // outward (their fp32 rounding then bounds nothing from the inside, so
     // every candidate of such a query is decided in fp64)
-    if (q.origin != t.origin) reframe32(qf, q.origin, t.origin);
+    if (q.origin != t.origin)
+      reframe32(qf, q.origin[0], q.origin[1], q.origin[2], t.origin[0], t.origin[1], t.origin[2]);
   } else {
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {  // the empty box overlaps nothing
@@ -321,18 +322,20 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
       if (lane == 0) atomicOr(t.status, kErrTraverse);
       break;
     }
-    // the record's first 224 bytes (both boxes, child ids, leaf elements)
-    // and `last`: every scalar load issued before any is used, one memory
-    // round trip per visit
+    // both halves' slabs, child ids and leaf elements (the first 112 bytes of
+    // each 128-byte half) and `last`: every scalar load issued before any is
+    // used, one memory round trip per visit
     CInt* rp = (CInt*)(recs + x);
     int32_t rw[56];
 #pragma unroll
-    for (int k = 0; k < 56; ++k) rw[k] = rp[k];
-    const uint32_t rlast = SELF ? recs[x].last : 0u;
-    const uint32_t rfirst = CROSS ? recs[x].first : 0u;
+    for (int k = 0; k < 28; ++k) rw[k] = rp[k];
+#pragma unroll
+    for (int k = 0; k < 28; ++k) rw[28 + k] = rp[32 + k];
+    const uint32_t rlast = SELF ? recs[x].h[0].last : 0u;
+    const uint32_t rfirst = CROSS ? recs[x].h[0].first : 0u;
     __builtin_amdgcn_sched_barrier(0);
-    const uint32_t cl = (uint32_t)rw[52], cr = (uint32_t)rw[53];
-    const uint32_t el = (uint32_t)rw[54], er = (uint32_t)rw[55];
+    const uint32_t cl = (uint32_t)rw[26], cr = (uint32_t)rw[54];
+    const uint32_t el = (uint32_t)rw[27], er = (uint32_t)rw[55];
     // internal children: overlap only (v_cmpx chain); leaf children: the gap,
     // which also decides whether fp32 proves the fp64 overlap
     const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
@@ -345,10 +348,10 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
       hl = box_hits32(qf, rw);
     }
     if (leaf_r) {
-      gr = min_gap(qf, rw + kSlabs);
+      gr = min_gap(qf, rw + 28);
       hr = gr > 0;
     } else {
-      hr = box_hits32(qf, rw + kSlabs);
+      hr = box_hits32(qf, rw + 28);
     }
     if (SELF) {
       // position bounds (non-short-circuit &: no divergent branch)
