@@ -792,6 +792,33 @@ __device__ __forceinline__ void patch_flush(unsigned long long* pk64, const doub
   }
 }
 
+// Exact fp64 patch slabs (the union of each patch's element k-DOPs) of an
+// object whose tree holds fp32-outward records: one thread per sorted
+// position, the element's k-DOP recomputed from its data, reduced per wave
+// when the wave lies in one patch (else per lane with atomics), the block's
+// waves folded in LDS.  pk64 starts at the ordered images of the empty slabs.
+template <int KIND, int NV>
+__global__ __launch_bounds__(256) void k_patch_slabs(ElemInput in, const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ patch_of_pos,
+                                                     unsigned long long* __restrict__ pk64) {
+  __shared__ double s_part[256 / kWave][kSlabs];
+  __shared__ uint32_t s_ppatch[256 / kWave];
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t e = p < in.n ? perm[p] : kEnd;
+  const bool act = p < in.n && e < in.n;
+  double kd[kSlabs];
+  uint32_t pk = kEnd;
+  if (act) {
+    element_kdop<KIND, NV>(in, e, kd);
+    pk = patch_of_pos[p];
+  } else {
+    empty_kdop(kd);
+  }
+  patch_partial(pk64, pk, kd, s_part[threadIdx.x / kWave], &s_ppatch[threadIdx.x / kWave]);
+  __syncthreads();
+  patch_flush<256 / kWave>(pk64, s_part, s_ppatch);
+}
+
 // the object's frame origin: its Morton bounds minimum (non-finite
 // components 0)
 __global__ void k_set_frame(const Bounds* bounds, double* frame) {
@@ -847,7 +874,7 @@ __device__ __forceinline__ void publish(ChunkLds<FP32>& L, uint32_t slot, uint32
 // global memory with stores that nothing in this kernel waits for.
 template <bool FP32>
 __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint32_t l, uint32_t r, uint32_t mypatch,
-                                            uint32_t c0, uint32_t c1, ChunkLds<FP32>& L) {
+                                            uint32_t c0, uint32_t c1, ChunkLds<FP32>& L, SlabT<FP32>* own) {
   RecT<FP32>* recs = recs_of<FP32>(b);
   for (;;) {
     // delta[l - 1] = L.delta[l - c0], delta[r] = L.delta[r - c0 + 1] (0 at the ends)
@@ -882,11 +909,14 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     }
     const uint32_t ms = l - c0;
     const uint32_t slast = L.last[ss], spatch = L.patch[ss];
-    SlabT<FP32> kd[kSlabs];
+    // fp32 slabs (32-bit images) stay in registers from one level to the
+    // next; the fp64 ones are read back from the node's own slot
+    SlabT<FP32> kdl[FP32 ? 1 : kSlabs];
+    SlabT<FP32>* kd = FP32 ? own : kdl;
 #pragma unroll
     for (int a = 0; a < kAxes; ++a) {
-      const SlabT<FP32> lo0 = L.slab[2 * a][ms], lo1 = L.slab[2 * a][ss];
-      const SlabT<FP32> hi0 = L.slab[2 * a + 1][ms], hi1 = L.slab[2 * a + 1][ss];
+      const SlabT<FP32> lo0 = FP32 ? own[2 * a] : L.slab[2 * a][ms], lo1 = L.slab[2 * a][ss];
+      const SlabT<FP32> hi0 = FP32 ? own[2 * a + 1] : L.slab[2 * a + 1][ms], hi1 = L.slab[2 * a + 1][ss];
       kd[2 * a] = (lo0 < lo1) ? lo0 : lo1;
       kd[2 * a + 1] = (hi0 > hi1) ? hi0 : hi1;
     }
@@ -978,29 +1008,14 @@ __device__ __forceinline__ float slab_error_ulps(float f, double exact) {
   return (float)__builtin_ldexp(__builtin_fabs((double)f - exact), 1 - e);
 }
 
-// uni: the wave's elements lie in one patch (its partials are reduced in
-// registers, lane 0 leaves them in `part`); else each lane with an element
-// folds its slabs in with atomics.  Each axis's outward slabs go straight to
-// the leaf's LDS slot (column `slot` of slab), so no slab stays in a register
-// while the next axis is formed.
+// Each axis's outward slabs go straight to the leaf's LDS slot (column
+// `slot` of slab), so no slab stays in a register while the next axis is
+// formed.
 template <int KIND, int NV, int A = 0>
-__device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool act, bool uni, uint32_t pk,
-                                            unsigned long long* pk64, double* part, const double* org,
+__device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool act, const double* org,
                                             int32_t (*slab)[kTreeLeafChunk], uint32_t slot) {
   double lo, hi;
   leaf_axis<KIND, NV, A>(s, act, lo, hi);
-  if (uni) {
-    const double wl = wave_min(lo);
-    const double wh = wave_max(hi);
-    if (lane_id() == 0) {
-      part[2 * A] = wl;
-      part[2 * A + 1] = wh;
-    }
-  } else if (pk != kEnd) {
-    unsigned long long* d = pk64 + (uint64_t)pk * kSlabs;
-    atomicMin(d + 2 * A, (unsigned long long)ordered_u64(lo));
-    atomicMax(d + 2 * A + 1, (unsigned long long)ordered_u64(hi));
-  }
   if (act) {
     const double po = project<A>(org[0], org[1], org[2]);
     const float fl = round_down_1ulp(lo - po), fh = round_up_1ulp(hi - po);
@@ -1010,7 +1025,7 @@ __device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool 
     slab[2 * A + 1][slot] = kh;
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (A + 1 < kAxes) leaf_axes32<KIND, NV, A + 1>(s, act, uni, pk, pk64, part, org, slab, slot);
+  if constexpr (A + 1 < kAxes) leaf_axes32<KIND, NV, A + 1>(s, act, org, slab, slot);
 }
 
 // Largest distance of an object's fp32-outward leaf slabs from their fp64
@@ -1066,24 +1081,19 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   if (p < c1 && e >= n) atomicOr(b.status, kErrIndex);
   const bool act = p < c1 && e < n;
   uint32_t pk = act ? b.patch_of_pos[p] : kEnd;
-  // exact patch slabs for the patch broadphase (fp32 form; the fp64 form
-  // takes them from the patch roots)
-  __shared__ double s_part[FP32 ? kTreeChunk / kWave : 1][kSlabs];
-  __shared__ uint32_t s_ppatch[FP32 ? kTreeChunk / kWave : 1];
   const uint32_t leaf = (uint32_t)(n - 1 + p);
+  SlabT<FP32> own[FP32 ? kSlabs : 1];  // fp32: the climbing node's slabs
   // the leaf's half of its parent's record (the root's array if n = 1)
   const uint32_t dl = (act && p) ? b.delta[p - 1] : 0u, dr = (act && p != n - 1) ? b.delta[p] : 0u;
   const bool is_left = (p == 0) || (p != n - 1 && dr < dl);
   const uint32_t parent = is_left ? p : p - 1;
   const int side = is_left ? 0 : 1;
   if constexpr (FP32) {
+    // (the exact patch slabs of the fp32-outward records are reduced by
+    // k_patch_slabs when something asks for them)
     LeafSource<KIND, NV> src;
     leaf_source<KIND, NV>(in, e, act, src);
-    const uint64_t m = __ballot(pk != kEnd);
-    const uint32_t pk0 = m ? __builtin_amdgcn_readlane(pk, __ffsll((long long)m) - 1) : kEnd;
-    const bool uni = m != 0 && __ballot(pk != kEnd && pk != pk0) == 0;
-    leaf_axes32<KIND, NV>(src, act, uni, pk, b.patch_k64, s_part[t / kWave], b.origin, L.slab, t);
-    if (lane_id() == 0) s_ppatch[t / kWave] = uni ? pk0 : kEnd;
+    leaf_axes32<KIND, NV>(src, act, b.origin, L.slab, t);
   }
   if (act) {
     b.patch_of_input[e] = pk;
@@ -1094,15 +1104,15 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       L.patch[t] = pk;
       L.endslot[t] = t;
       // the leaf's half of its parent's record (the root's array if n = 1),
-      // from its LDS slot, all its stores together
-      int32_t ks[kSlabs];
+      // from its LDS slot, all its stores together; the slabs stay in
+      // registers for the climb
 #pragma unroll
-      for (int i = 0; i < kSlabs; ++i) ks[i] = L.slab[i][t];
+      for (int i = 0; i < kSlabs; ++i) own[i] = L.slab[i][t];
       if (n == 1) {
         *b.root = leaf;
-        store_rootbox<true>(b, ks);
+        store_rootbox<true>(b, own);
       } else {
-        rec_store_half<true>(b, parent, side, ks, leaf, e);
+        rec_store_half<true>(b, parent, side, own, leaf, e);
       }
     } else {
       double kd[kSlabs];
@@ -1117,9 +1127,8 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       }
     }
   }
-  __syncthreads();  // counters, deltas, leaf slots and patch partials ready
-  if constexpr (FP32) patch_flush<kTreeChunk / kWave>(b.patch_k64, s_part, s_ppatch);
-  if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L);
+  __syncthreads();  // counters, deltas and leaf slots ready
+  if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L, own);
   __syncthreads();
   // a first arrival whose sibling lies outside the chunk leaves its arrival word
   if (c0 + t + 1 < c1 && L.cnt[t] == 1)
@@ -1448,6 +1457,30 @@ hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const ui
   uint64_t blocks = (n * kSlabs + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_sorted_kdops, dim3((unsigned)blocks), dim3(256), 0, s, rec, rootbox, delta, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_slabs(const ElemInput& in, const uint32_t* perm, const uint32_t* patch_of_pos,
+                              unsigned long long* pk64, hipStream_t s) {
+  if (in.n == 0) return hipSuccess;
+  if (!perm || !patch_of_pos || !pk64 || !in.data) return hipErrorInvalidValue;
+  const dim3 g((unsigned)((in.n + 255) / 256)), th(256);
+  if (in.kind == kSphere) {
+    hipLaunchKernelGGL((k_patch_slabs<kSphere, 1>), g, th, 0, s, in, perm, patch_of_pos, pk64);
+  } else if (in.kind == kSnapshot) {
+    hipLaunchKernelGGL((k_patch_slabs<kSnapshot, 1>), g, th, 0, s, in, perm, patch_of_pos, pk64);
+  } else {
+    switch (in.nv) {
+      case 1: hipLaunchKernelGGL((k_patch_slabs<kPoly, 1>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 2: hipLaunchKernelGGL((k_patch_slabs<kPoly, 2>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 3: hipLaunchKernelGGL((k_patch_slabs<kPoly, 3>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 4: hipLaunchKernelGGL((k_patch_slabs<kPoly, 4>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 5: hipLaunchKernelGGL((k_patch_slabs<kPoly, 5>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 6: hipLaunchKernelGGL((k_patch_slabs<kPoly, 6>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      case 7: hipLaunchKernelGGL((k_patch_slabs<kPoly, 7>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+      default: hipLaunchKernelGGL((k_patch_slabs<kPoly, 8>), g, th, 0, s, in, perm, patch_of_pos, pk64); break;
+    }
+  }
   return hipGetLastError();
 }
 

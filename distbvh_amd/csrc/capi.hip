@@ -227,6 +227,11 @@ struct bvh_object_s {
   DevBuf wire_cent, wire_range, wire_src, wire_out;
   uint64_t npairs = 0;
   uint64_t npatch_pairs = 0;
+  // exact fp64 patch slabs valid for the current build (the fp64 records'
+  // build writes them; the fp32-outward records' are reduced on demand), and
+  // the last query's patch pairs computed (on demand after a narrowphase)
+  bool patch_slabs_ready = false;
+  bool patch_pairs_ready = false;
   uint64_t nvisits = 0;
   uint64_t ncandidates = 0;
   uint64_t nrecheck = 0;      // fp32 candidates re-checked in fp64
@@ -501,6 +506,7 @@ static bvh_status object_build(bvh_object_s* o) {
     HIP_TRY(launch_fill_u32(o->bb.splits, w->od > 1 ? w->od - 1 : 0, 0u, s));
     HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
     HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
+    o->patch_slabs_ready = false;  // every patch empty: the init alone
     o->built = true;
     return BVH_OK;
   }
@@ -539,8 +545,10 @@ static bvh_status object_build(bvh_object_s* o) {
     // the object's fp32 frame: the Morton bounds minimum of this build (a
     // refit keeps it)
     HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
+  } else {
+    HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));  // written at the patch roots
   }
-  HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
+  o->patch_slabs_ready = !o->fp32;
   w->stage_begin("delta");
   HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
   w->stage_end();
@@ -564,7 +572,8 @@ static bvh_status object_refit(bvh_object_s* o) {
   if (n > 1) HIP_TRY(launch_fill_u32((uint32_t*)o->bb.counters, 2 * (n - 1), 0u, s));
   HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
   HIP_TRY(launch_fill_u32(o->bb.root, 1, kEnd, s));
-  HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
+  if (!o->fp32) HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));
+  o->patch_slabs_ready = !o->fp32;
   HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
   w->stage_end();
   return BVH_OK;
@@ -740,6 +749,53 @@ static bvh_status sort_results(bvh_object_s* o) {
   return BVH_OK;
 }
 
+// The exact fp64 patch slabs of the current build, reduced from element data
+// when the tree holds fp32-outward records (stream ordered; the element data
+// must still be valid).
+static bvh_status ensure_patch_slabs(bvh_object_s* o) {
+  if (o->patch_slabs_ready) return BVH_OK;
+  hipStream_t s = o->w->stream;
+  HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, o->w->od, s));
+  if (o->in.n > 0)
+    HIP_TRY(launch_patch_slabs(o->in, o->perm.as<uint32_t>(), o->patch_of_pos.as<uint32_t>(),
+                               o->patch_k64.as<unsigned long long>(), s));
+  o->patch_slabs_ready = true;
+  return BVH_OK;
+}
+
+// The patch pairs of o's last query (its patch broadphase), computed when
+// something asks for them: the built-in narrowphase walks every element and
+// does not need them.  Synchronizes.
+static bvh_status ensure_patch_pairs(bvh_object_s* o) {
+  if (!o->have_results || o->patch_pairs_ready) return BVH_OK;
+  bvh_object_s* t = o->result_other;
+  if (o->self_result || !t) {
+    o->npatch_pairs = 0;
+    o->patch_pairs_ready = true;
+    return BVH_OK;
+  }
+  if (o->w->capturing) return fail(BVH_E_STATE, "patch pairs are read after the capture ends");
+  if (!t->built) return fail(BVH_E_STATE, "the other object was released or rebuilt");
+  hipStream_t s = o->w->stream;
+  BVH_TRY(ensure_patch_slabs(o));
+  BVH_TRY(ensure_patch_slabs(t));
+  unsigned long long* dcount = o->count.as<unsigned long long>();
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    o->q_pcap = o->ppairs.bytes / 8;
+    HIP_TRY(launch_fill_u32((uint32_t*)(dcount + kCntPatch), 2, 0u, s));
+    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+    HIP_TRY(launch_patch_broadphase(view_of(o), view_of(t), o->fp32, po, s));
+    unsigned long long cnt = 0;
+    BVH_TRY(copy_out(&cnt, dcount + kCntPatch, sizeof(cnt), BVH_MEM_HOST, s));
+    o->npatch_pairs = cnt;
+    if (cnt <= o->q_pcap) break;
+    if (attempt == 1) return fail(BVH_E_OVERFLOW, "patch pair list still overflows after growing");
+    BVH_TRY(o->ppairs.ensure(cnt * 8 + 1024 * 8));
+  }
+  o->patch_pairs_ready = true;
+  return BVH_OK;
+}
+
 // Launches a query with the current buffer sizes; no synchronization (so it
 // can be captured into a graph).  The counters stay on the device.
 static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, bool narrow, bool cross) {
@@ -754,10 +810,17 @@ static bvh_status launch_query(bvh_object_s* o, bvh_object_s* other, bool self, 
   o->q_cap = o->pairs.bytes / 8;
   o->q_rcap = o->recheck.bytes / 8 / kRecheckStripes;  // per stripe
   HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntTotal, 0u, s));
-  if (!self) {
+  // the patch broadphase runs here only for a broadphase without the element
+  // narrowphase (the patch pairs are its result); after a narrowphase it
+  // runs when its pairs are asked for (ensure_patch_pairs)
+  o->patch_pairs_ready = self;
+  if (!self && !narrow) {
     w->stage_begin("broadphase");
+    BVH_TRY(ensure_patch_slabs(o));
+    BVH_TRY(ensure_patch_slabs(other));
     PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
     HIP_TRY(launch_patch_broadphase(qv, tv, o->fp32, po, s));
+    o->patch_pairs_ready = true;
     w->stage_end();
   }
   if (narrow) {
@@ -874,8 +937,7 @@ static bvh_status run_query_batch(bvh_object_s* const* os, bvh_object_s* const* 
     o->self_result = false;
     HIP_TRY(launch_fill_u32((uint32_t*)dcount, 2 * kCntTotal, 0u, s));
     const TreeView qv = view_of(o), tv = view_of(t);
-    PairOut po{o->ppairs.as<uint64_t>(), o->q_pcap, dcount + kCntPatch, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
-    HIP_TRY(launch_patch_broadphase(qv, tv, 0, po, s));
+    o->patch_pairs_ready = false;  // computed when asked for (ensure_patch_pairs)
     if (o->in.n == 0 || t->in.n == 0) continue;
     BatchQuery& e = w->bq_host[nq++];
     e.q = qv;
@@ -1268,6 +1330,7 @@ bvh_status bvh_object_release(bvh_object o) {
   o->have_data = false;
   o->have_results = false;
   o->results_pending = false;
+  o->patch_slabs_ready = o->patch_pairs_ready = false;
   o->npairs = o->npatch_pairs = o->nvisits = o->ncandidates = o->nrecheck = o->nrecheck_max = 0;
   o->result_ptr = nullptr;
   o->result_other = nullptr;
@@ -1540,7 +1603,10 @@ bvh_status bvh_result_stats(bvh_object o, uint64_t* node_visits, uint64_t* patch
   BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
   if (node_visits) *node_visits = o->have_results ? o->nvisits : 0;
-  if (patch_pairs) *patch_pairs = o->have_results ? o->npatch_pairs : 0;
+  if (patch_pairs) {
+    BVH_TRY(ensure_patch_pairs(o));
+    *patch_pairs = o->have_results ? o->npatch_pairs : 0;
+  }
   if (candidates) *candidates = o->have_results ? (o->fp32 ? o->ncandidates : o->npairs) : 0;
   if (rechecked) *rechecked = o->have_results && o->fp32 ? o->nrecheck : 0;
   if (max_slab_error_ulps) {
@@ -1562,6 +1628,7 @@ bvh_status bvh_patch_pair_count(bvh_object o, uint64_t* count) {
   if (!o || !count) return fail(BVH_E_INVALID, "null argument");
   BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
+  BVH_TRY(ensure_patch_pairs(o));
   *count = o->have_results ? o->npatch_pairs : 0;
   return BVH_OK;
 }
@@ -1570,6 +1637,7 @@ bvh_status bvh_patch_pairs(bvh_object o, uint64_t* out, uint64_t cap) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   BVH_TRY(join_all(o->w));
   BVH_TRY(results_ready(o));
+  BVH_TRY(ensure_patch_pairs(o));
   const uint64_t n = o->have_results ? o->npatch_pairs : 0;
   if (cap < n) return fail(BVH_E_OVERFLOW, "patch pair buffer too small");
   if (n == 0) return BVH_OK;
@@ -1662,6 +1730,8 @@ bvh_status bvh_object_sorted_kdops(bvh_object o, double* kdops, uint32_t mem) {
 bvh_status bvh_local_patches(bvh_object o, bvh_patch_info* out, size_t cap) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
   BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "local_patches inside a capture");
+  BVH_TRY(ensure_patch_slabs(o));
   const size_t od = o->w->od;
   if (cap < od) return fail(BVH_E_OVERFLOW, "patch buffer too small");
   hipStream_t s = o->w->stream;
@@ -1723,6 +1793,8 @@ bvh_status bvh_object_patch_tree(bvh_object o, bvh_tree_node* nodes, size_t cap,
                                  uint64_t* leaf_patches, size_t leaf_cap, size_t* nleaves) {
   if (!o || !o->built) return fail(BVH_E_STATE, "object not built");
   BVH_TRY(join_all(o->w));
+  if (o->w->capturing) return fail(BVH_E_STATE, "patch tree export inside a capture");
+  BVH_TRY(ensure_patch_slabs(o));
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n, od = w->od;
@@ -1995,6 +2067,7 @@ bvh_status bvh_pack_patches(bvh_object o, void* out, uint64_t cap, uint64_t* byt
   if (!out) return BVH_OK;
   if (cap < *bytes) return fail(BVH_E_OVERFLOW, "patch buffer holds " + std::to_string(cap) + " of " +
                                                     std::to_string(*bytes) + " bytes");
+  BVH_TRY(ensure_patch_slabs(o));
   hipStream_t s = o->w->stream;
   BVH_TRY(o->wire_cent.ensure(3 * od * sizeof(double)));
   double* cent = o->wire_cent.as<double>();
@@ -2454,6 +2527,9 @@ bvh_status bvh_graph_launch(bvh_graph g) {
   if (g->gen != g->w->alloc_gen.load())
     return fail(BVH_E_STATE, "device buffers of this world were reallocated since the capture; capture again");
   HIP_TRY(hipGraphLaunch(g->exec, g->w->stream));
+  // the replayed builds leave no exact patch slabs in fp32-outward records
+  for (auto& ob : g->w->objects)
+    if (ob->fp32) ob->patch_slabs_ready = false;
   // the captured queries' state, whatever the objects ran since the capture
   for (const CapturedQuery& c : g->queries) {
     bvh_object_s* o = c.o;
@@ -2468,6 +2544,7 @@ bvh_status bvh_graph_launch(bvh_graph g) {
     o->result_ptr = c.result_ptr;
     o->have_results = true;
     o->results_pending = true;
+    o->patch_pairs_ready = c.self || !c.narrow;  // a patch-only broadphase captured its patch pairs
   }
   return BVH_OK;
 }

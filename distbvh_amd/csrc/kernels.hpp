@@ -118,6 +118,10 @@ hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hip
 // parents' records
 hipError_t launch_sorted_kdops(const Rec64* rec, const double* rootbox, const uint8_t* delta, uint64_t n, double* out,
                                hipStream_t s);
+// exact fp64 patch slabs of an object with fp32-outward records (pk64 first
+// set by launch_patch_k64_init), from element data in sorted order
+hipError_t launch_patch_slabs(const ElemInput& in, const uint32_t* perm, const uint32_t* patch_of_pos,
+                              unsigned long long* pk64, hipStream_t s);
 // largest fp32-outward slab error of an object's elements in fp32 ulps (atomic
 // max of the float bits into *out, which the caller zeroes)
 hipError_t launch_slab_error(const ElemInput& in, const double* org, uint32_t* out, hipStream_t s);
