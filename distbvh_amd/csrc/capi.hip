@@ -250,6 +250,12 @@ struct bvh_object_s {
   hipStream_t bs = nullptr;
   hipEvent_t ev_fork = nullptr, ev_done = nullptr;
   bool pending = false;
+  // A forked build enqueues its tree stage later, at the next call that
+  // needs a tree (tree_pending), after the sort stages of every build pending
+  // then (ev_sorted): the latency-bound sorts of several objects share the
+  // GPU, and no sort waits behind another object's tree kernel.
+  hipEvent_t ev_sorted = nullptr;
+  bool tree_pending = false;
 
   bvh_object_s() = default;
   bvh_object_s(const bvh_object_s&) = delete;
@@ -261,6 +267,7 @@ struct bvh_object_s {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_done) (void)hipEventDestroy(ev_done);
+    if (ev_sorted) (void)hipEventDestroy(ev_sorted);
   }
 
   // every device buffer of the object
@@ -293,9 +300,12 @@ static bvh_status join(bvh_object_s* o) {
   HIP_TRY(hipStreamWaitEvent(o->w->stream, o->ev_done, 0));
   return BVH_OK;
 }
-// ... for every object of the world: every call that reads objects' device
-// data or may be followed by the caller's use of it starts with this.
+static bvh_status flush_trees(bvh_world w);
+// ... for every object of the world, after enqueuing the tree stages still
+// pending: every call that reads objects' device data or may be followed by
+// the caller's use of it starts with this.
 static bvh_status join_all(bvh_world w) {
+  BVH_TRY(flush_trees(w));
   for (auto& o : w->objects) BVH_TRY(join(o.get()));
   return BVH_OK;
 }
@@ -494,8 +504,11 @@ static bvh_status split_and_order(bvh_object_s* o) {
   return BVH_OK;
 }
 
-// The whole object build, stream ordered, no host synchronization.
-static bvh_status object_build(bvh_object_s* o) {
+static bvh_status object_build_tree(bvh_object_s* o);
+
+// The whole object build, stream ordered, no host synchronization.  With
+// defer_tree the tree stage is left to flush_trees (the object's tree_pending).
+static bvh_status object_build(bvh_object_s* o, bool defer_tree) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n;
@@ -552,11 +565,47 @@ static bvh_status object_build(bvh_object_s* o) {
   w->stage_begin("delta");
   HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
   w->stage_end();
-  w->stage_begin("tree");
-  HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, s));
-  w->stage_end();
   o->built = true;
+  if (defer_tree) {
+    o->tree_pending = true;
+    return BVH_OK;
+  }
+  return object_build_tree(o);
+}
+
+// The tree stage of a build: leaf slabs and the bottom-up climb into records.
+static bvh_status object_build_tree(bvh_object_s* o) {
+  bvh_world w = o->w;
+  w->stage_begin("tree");
+  HIP_TRY(launch_tree(o->in, o->bb, w->od, o->fp32, w->stream));
+  w->stage_end();
   return BVH_OK;
+}
+
+// Enqueues the pending tree stages of the world's forked builds, each on its
+// build stream after the sort stages of all of them, and moves each build's
+// ev_done behind its tree.
+static bvh_status flush_trees(bvh_world w) {
+  std::vector<bvh_object_s*> p;
+  for (auto& o : w->objects)
+    if (o->tree_pending) p.push_back(o.get());
+  if (p.empty()) return BVH_OK;
+  for (bvh_object_s* a : p)
+    for (bvh_object_s* b : p)
+      if (a != b) HIP_TRY(hipStreamWaitEvent(a->bs, b->ev_sorted, 0));
+  bvh_status st = BVH_OK;
+  for (bvh_object_s* a : p) {
+    a->tree_pending = false;
+    bvh_status sa;
+    {
+      StreamSwap swap(w, a->bs);
+      sa = object_build_tree(a);
+    }
+    // recorded also after a failure, so that nothing enqueued is left unordered
+    HIP_TRY(hipEventRecord(a->ev_done, a->bs));
+    if (st == BVH_OK) st = sa;
+  }
+  return st;
 }
 
 // Refit: new element k-DOPs on the topology of the last build (same
@@ -581,7 +630,7 @@ static bvh_status object_refit(bvh_object_s* o) {
 
 static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                    const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
-                                   uint32_t split, hipStream_t bstream);
+                                   uint32_t split, hipStream_t bstream, bool defer_tree);
 
 static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                     const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
@@ -605,8 +654,11 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
   bvh_world w = o->w;
   if (o->pending) {
-    // the previous build may still use the buffers this one may reallocate
-    // (inside a capture nothing is reallocated, and nothing may synchronize)
+    // a tree stage still pending is superseded by this build (nothing read
+    // that tree); the previous build may still use the buffers this one may
+    // reallocate (inside a capture nothing is reallocated, and nothing may
+    // synchronize)
+    o->tree_pending = false;
     if (!w->capturing) HIP_TRY(hipStreamSynchronize(o->bs));
     BVH_TRY(join(o));
   }
@@ -621,12 +673,21 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
       HIP_TRY(hipStreamCreateWithFlags(&o->bs, hipStreamNonBlocking));
       HIP_TRY(hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&o->ev_done, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&o->ev_sorted, hipEventDisableTiming));
     }
     HIP_TRY(hipEventRecord(o->ev_fork, w->stream));
     HIP_TRY(hipStreamWaitEvent(o->bs, o->ev_fork, 0));
   }
-  const bvh_status st = set_entity_build(o, kind, nv, eps, data, cent, n, gids, mem, split, fork ? o->bs : w->stream);
+  // Inside a capture the tree stage is not deferred: deferral makes the build
+  // streams wait on each other directly, and hipStreamEndCapture crashed in
+  // the host runtime on a graph holding such waits between two builds.  The
+  // capture keeps the fork / join pattern only (a graph branch per build).
+  const bool defer = fork && !w->capturing;
+  const bvh_status st =
+      set_entity_build(o, kind, nv, eps, data, cent, n, gids, mem, split, fork ? o->bs : w->stream, defer);
   if (fork) {
+    if (st != BVH_OK) o->tree_pending = false;
+    if (o->tree_pending) HIP_TRY(hipEventRecord(o->ev_sorted, o->bs));
     // the world stream waits for this at the next call that needs it (also
     // after a failed build, so that nothing it enqueued is left unordered)
     HIP_TRY(hipEventRecord(o->ev_done, o->bs));
@@ -637,7 +698,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
 
 static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                    const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
-                                   uint32_t split, hipStream_t bstream) {
+                                   uint32_t split, hipStream_t bstream, bool defer_tree) {
   bvh_world w = o->w;
   StreamSwap swap(w, bstream);
   hipStream_t s = w->stream;
@@ -682,7 +743,8 @@ static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps
   }
   o->have_data = true;
   o->built = false;
-  return object_build(o);
+  o->tree_pending = false;
+  return object_build(o, defer_tree);
 }
 
 // Counter words of a query (device, o->count): pairs, node visits, fp32
@@ -1209,6 +1271,7 @@ static void orphan_graphs(bvh_world w);
 
 bvh_status bvh_world_destroy(bvh_world w) {
   if (!w) return BVH_OK;
+  for (auto& o : w->objects) o->tree_pending = false;  // trees nobody reads
   (void)join_all(w);
   (void)hipStreamSynchronize(w->stream);
   // graphs outlive their world only as handles to destroy
@@ -1313,6 +1376,7 @@ bvh_status bvh_object_release(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "release inside a capture");
+  o->tree_pending = false;  // nobody reads this tree any more
   BVH_TRY(join_all(w));
   // queued work of the world and of the object's build stream may still use
   // the buffers
