@@ -819,14 +819,18 @@ __global__ __launch_bounds__(256) void k_patch_slabs(ElemInput in, const uint32_
   patch_flush<256 / kWave>(pk64, s_part, s_ppatch);
 }
 
-// the object's frame origin: its Morton bounds minimum (non-finite
-// components 0)
+// the object's frame: its origin, the Morton bounds minimum (non-finite
+// components 0), then the origin's projections on the 13 axes (what the leaf
+// kernel subtracts, read once per workgroup)
 __global__ void k_set_frame(const Bounds* bounds, double* frame) {
   if (threadIdx.x != 0) return;
+  double o[3];
   for (int c = 0; c < 3; ++c) {
     const double v = bounds->min[c];
-    frame[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
+    o[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
+    frame[c] = o[c];
   }
+  proj_all(o[0], o[1], o[2], frame + 3);
 }
 
 __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
@@ -843,7 +847,7 @@ __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
 // internal node c0 + j.
 template <bool FP32>
 struct ChunkLds {
-  SlabT<FP32> slab[kSlabs][kTreeChunk];  // SoA: conflict-free stores
+  SlabT<FP32> slab[kSlabs][kTreeChunk + 1];  // SoA, rows padded: conflict-free stores, and column reads
   uint32_t cnt[kTreeChunk];      // arrivals at internal node c0 + j
   uint32_t far[kTreeChunk];
   uint32_t fpatch[kTreeChunk];
@@ -1012,20 +1016,19 @@ __device__ __forceinline__ float slab_error_ulps(float f, double exact) {
 // `slot` of slab), so no slab stays in a register while the next axis is
 // formed.
 template <int KIND, int NV, int A = 0>
-__device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool act, const double* org,
-                                            int32_t (*slab)[kTreeLeafChunk], uint32_t slot) {
+__device__ __forceinline__ void leaf_axes32(const LeafSource<KIND, NV>& s, bool act, const double* po,
+                                            int32_t (*slab)[kTreeLeafChunk + 1], uint32_t slot) {
   double lo, hi;
   leaf_axis<KIND, NV, A>(s, act, lo, hi);
   if (act) {
-    const double po = project<A>(org[0], org[1], org[2]);
-    const float fl = round_down_1ulp(lo - po), fh = round_up_1ulp(hi - po);
+    const float fl = round_down_1ulp(lo - po[A]), fh = round_up_1ulp(hi - po[A]);
     const int32_t kl = (fl == fl) ? ord32(fl) : 0x7fffffff;
     const int32_t kh = (fh == fh) ? ord32(fh) : (int32_t)0x80000000;
     slab[2 * A][slot] = kl;
     slab[2 * A + 1][slot] = kh;
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (A + 1 < kAxes) leaf_axes32<KIND, NV, A + 1>(s, act, org, slab, slot);
+  if constexpr (A + 1 < kAxes) leaf_axes32<KIND, NV, A + 1>(s, act, po, slab, slot);
 }
 
 // Largest distance of an object's fp32-outward leaf slabs from their fp64
@@ -1055,6 +1058,33 @@ __global__ __launch_bounds__(256) void k_slab_error(ElemInput in, const double* 
   if (lane_id() == 0 && err > 0.0f) atomicMax(out, __float_as_uint(err));
 }
 
+// The fp32 leaf halves of a wave (slabs from the leaves' LDS slots, leaf id,
+// input index) stored as 16-byte pieces in address order: lane l of store j
+// writes piece q = k mod 7 of the wave's leaf k / 7 (k = 64 j + l), so the 64
+// lanes of one store cover about nine consecutive halves rather than 64
+// scattered ones.  half: the lane's record half (2 x parent + side), or kEnd
+// for a lane without a leaf.  Called by all 64 lanes of the wave.
+__device__ __forceinline__ void store_leaf_halves32(const BuildBufs& b, const ChunkLds<true>& L, uint32_t t,
+                                                    uint32_t half, uint32_t e, uint32_t c0, uint64_t n) {
+  int4* base = (int4*)b.rec32;
+  const uint32_t lane = t & (kWave - 1), w0 = t - lane;
+  const uint32_t leaf0 = (uint32_t)(n - 1) + c0 + w0;
+#pragma unroll
+  for (uint32_t j = 0; j < 7; ++j) {
+    const uint32_t k = kWave * j + lane;
+    const uint32_t i = k / 7u, q = k - 7u * i;
+    const uint32_t h = (uint32_t)__shfl((int)half, (int)i, kWave);
+    const uint32_t ei = (uint32_t)__shfl((int)e, (int)i, kWave);
+    const uint32_t sl = w0 + i;
+    // piece q holds slabs 4q .. 4q+3; piece 6 slabs 24, 25, the leaf id and
+    // its input index
+    const int32_t x = L.slab[4 * q][sl], y = L.slab[4 * q + 1][sl];
+    const int32_t z = q < 6 ? L.slab[4 * q + 2][sl] : (int32_t)(leaf0 + i);
+    const int32_t w = q < 6 ? L.slab[4 * q + 3][sl] : (int32_t)ei;
+    if (h != kEnd) base[(uint64_t)h * 8u + q] = make_int4(x, y, z, w);
+  }
+}
+
 template <int KIND, int NV, bool FP32>
 __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildBufs b, uint64_t od) {
   __shared__ ChunkLds<FP32> L;
@@ -1065,6 +1095,12 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
   const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t c0 = chunk * kTreeChunk;
   const uint32_t c1 = (uint32_t)((uint64_t)c0 + kTreeChunk < n ? c0 + kTreeChunk : n);
+  // the frame origin's projections (uniform, loaded before any store)
+  double po[FP32 ? kAxes : 1];
+  if constexpr (FP32) {
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) po[a] = b.origin[3 + a];
+  }
   L.cnt[t] = 0;
   {
     // delta[c0 - 1 + t] for t in [0, 256], where the index is in [0, n - 2]
@@ -1093,7 +1129,7 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
     // k_patch_slabs when something asks for them)
     LeafSource<KIND, NV> src;
     leaf_source<KIND, NV>(in, e, act, src);
-    leaf_axes32<KIND, NV>(src, act, b.origin, L.slab, t);
+    leaf_axes32<KIND, NV>(src, act, po, L.slab, t);
   }
   if (act) {
     b.patch_of_input[e] = pk;
@@ -1111,9 +1147,8 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       if (n == 1) {
         *b.root = leaf;
         store_rootbox<true>(b, own);
-      } else {
-        rec_store_half<true>(b, parent, side, own, leaf, e);
       }
+      // (with n > 1 the wave stores the leaf halves together, below)
     } else {
       double kd[kSlabs];
       element_kdop<KIND, NV>(in, e, kd);
@@ -1126,6 +1161,9 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
         rec_store_half<false>(b, parent, side, kd, leaf, e);
       }
     }
+  }
+  if constexpr (FP32) {
+    if (n > 1) store_leaf_halves32(b, L, t, act ? 2 * parent + (uint32_t)side : kEnd, e, c0, n);
   }
   __syncthreads();  // counters, deltas and leaf slots ready
   if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L, own);

@@ -202,7 +202,7 @@ struct bvh_object_s {
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
       patch_of_input, counters, patch_root, root, rec64, rootbox64, rec32, rootbox32, status,
       front_a, front_b, patch_k64;
-  DevBuf frame;  // fp32-outward records: the frame origin of this object's slabs (3 doubles)
+  DevBuf frame;  // fp32-outward records: the frame origin of this object's slabs (3 doubles), its 13 projections
   DevBuf slab_err;  // the slab-error statistic (one word)
   BuildBufs bb{};
   int bits = 63;
@@ -385,7 +385,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   if (o->fp32) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
     BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(int32_t)));
-    BVH_TRY(o->frame.ensure(4 * sizeof(double)));
+    BVH_TRY(o->frame.ensure(16 * sizeof(double)));
   } else {
     BVH_TRY(o->rec64.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec64)));
     BVH_TRY(o->rootbox64.ensure(kSlabs * sizeof(double)));

@@ -90,7 +90,7 @@ struct BuildBufs {
   double* rootbox64;       // fp64 form: the root's slabs (26)
   Rec32* rec32;            // fp32 form: n-1 child-pair records
   int32_t* rootbox32;      // fp32 form: the root's slabs (26, ord32 images)
-  double* origin;          // fp32 form: the world frame (3 doubles)
+  double* origin;          // fp32 form: the object frame (origin, 3 doubles; its 13 axis projections)
   uint32_t* status;        // device error bits (kErr*)
   FrontNode* front_a;      // 2 per chunk of kTreeLeafChunk leaves: nodes handed to the next build phase
   FrontNode* front_b;
