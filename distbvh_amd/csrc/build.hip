@@ -857,6 +857,7 @@ struct ChunkLds {
   uint8_t delta[kTreeChunk + 1];  // delta[c0 - 1 + j]
   FrontNode front[2];
   uint32_t nfront;
+  uint32_t order[kTreeChunk];  // fp32 climb: per wave, the lanes that formed a node at this level
 };
 
 template <bool FP32>
@@ -941,6 +942,112 @@ __device__ __forceinline__ void climb_chunk(const BuildBufs& b, uint64_t n, uint
     const bool up_left = (l == 0) || (r != n - 1 && ndr < ndl);
     rec_store_half<FP32>(b, up_left ? r : l - 1, up_left ? 0 : 1, kd, parent, kEnd);
     publish<FP32>(L, l - c0, r - c0, kd, r, mypatch);
+  }
+}
+
+// The record stores of one level of the fp32 phase-1 climb, made by the
+// whole wave in address order.  A lane that formed node `node` this level
+// (else kEnd) stores 8 pieces of 16 bytes: its half `half` of its parent's
+// record (7 pieces: slabs from its LDS slot `sl`, its node id, no element;
+// half kEnd for the root) and the range piece of its own record (first,
+// last, patch).  The forming lanes are ranked in lane order in the wave's
+// part of L.order; lane l of store j writes piece q = k mod 8 of the node of
+// rank k / 8 (k = 64 j + l), so one store covers eight nodes' pieces rather
+// than one piece of up to 64 scattered nodes.  Called by all lanes of the
+// wave, after the forming lanes published their slabs in their LDS slots.
+__device__ __forceinline__ void store_level32(const BuildBufs& b, ChunkLds<true>& L, uint32_t c0, uint32_t node,
+                                              uint32_t half, uint32_t sl) {
+  const uint64_t mask = __ballot(node != kEnd);
+  if (mask == 0) return;
+  int4* base = (int4*)b.rec32;
+  const uint32_t lane = lane_id(), w0 = threadIdx.x - lane;
+  if (node != kEnd) L.order[w0 + (uint32_t)__popcll(mask & lanemask_lt())] = lane;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  const uint32_t pieces = 8u * (uint32_t)__popcll(mask);
+  for (uint32_t k0 = 0; k0 < pieces; k0 += kWave) {  // wave-uniform
+    const uint32_t k = k0 + lane;
+    const uint32_t i = k < pieces ? k >> 3 : 0u, q = k & 7u;
+    const int src = (int)L.order[w0 + i];
+    const uint32_t nd = (uint32_t)__shfl((int)node, src, kWave);
+    const uint32_t hf = (uint32_t)__shfl((int)half, src, kWave);
+    const uint32_t s = (uint32_t)__shfl((int)sl, src, kWave);
+    if (k >= pieces) continue;
+    if (q == 7) {  // this node's range and patch (its record's half 0)
+      base[(uint64_t)nd * 16u + 7u] = make_int4((int32_t)(c0 + s), (int32_t)L.last[s], (int32_t)L.patch[s], 0);
+    } else if (hf != kEnd) {
+      // piece q of the half: slabs 4q .. 4q+3; piece 6: slabs 24, 25, node, no element
+      const uint32_t r0 = 4u * q;
+      const int32_t x = L.slab[r0][s], y = L.slab[r0 + 1][s];
+      const int32_t z = q < 6 ? L.slab[q < 6 ? r0 + 2 : 0][s] : (int32_t)nd;
+      const int32_t w = q < 6 ? L.slab[q < 6 ? r0 + 3 : 0][s] : (int32_t)kEnd;
+      base[(uint64_t)hf * 8u + q] = make_int4(x, y, z, w);
+    }
+  }
+}
+
+// The fp32 phase-1 climb (climb_chunk's steps), wave-synchronous: every lane
+// of the wave runs each level, lanes whose climb has ended idle, so that the
+// level's record stores go out together (store_level32).  alive: the lane
+// holds leaf l = r (its slabs in own and in its LDS slot).
+__device__ __forceinline__ void climb_chunk32(const BuildBufs& b, uint64_t n, bool alive, uint32_t l, uint32_t r,
+                                              uint32_t mypatch, uint32_t c0, uint32_t c1, ChunkLds<true>& L,
+                                              int32_t* own) {
+  while (__ballot(alive) != 0) {
+    uint32_t node = kEnd, half = kEnd;
+    if (alive) {
+      const uint32_t dl = L.delta[l - c0], dr = L.delta[r - c0 + 1];
+      const bool is_left = (l == 0) || (r != n - 1 && dr < dl);
+      const uint32_t parent = is_left ? r : l - 1;
+      if (parent < c0 || parent + 1 >= c1) {
+        const uint32_t k = atomicAdd(&L.nfront, 1u);
+        if (k < 2) L.front[k] = FrontNode{l, r, mypatch, dl | (dr << 8)};
+        else atomicOr(b.status, kErrTree);
+        alive = false;
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        const uint32_t old =
+            __hip_atomic_fetch_add(&L.cnt[parent - c0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        // the sibling starts at r + 1 (left child) or ends at l - 1 (right child)
+        const uint32_t ss = (old == 1) ? (is_left ? (r + 1 - c0) : L.endslot[l - 1 - c0]) : 0u;
+        if (old == 0) {
+          // first arriver: the sibling finishes the parent, in this phase or,
+          // through the parent's arrival word, in a later one
+          L.far[parent - c0] = is_left ? l : r;
+          L.fpatch[parent - c0] = mypatch;
+          alive = false;
+        } else if (old != 1 || ss >= kTreeChunk) {
+          atomicOr(b.status, kErrTree);
+          alive = false;
+        } else {
+          const uint32_t slast = L.last[ss], spatch = L.patch[ss];
+#pragma unroll
+          for (int a = 0; a < kAxes; ++a) {
+            const int32_t lo1 = L.slab[2 * a][ss], hi1 = L.slab[2 * a + 1][ss];
+            own[2 * a] = (own[2 * a] < lo1) ? own[2 * a] : lo1;
+            own[2 * a + 1] = (own[2 * a + 1] > hi1) ? own[2 * a + 1] : hi1;
+          }
+          if (is_left) r = slast;
+          else l = c0 + ss;
+          mypatch = (mypatch == spatch) ? mypatch : kMixed;
+          const uint32_t ndl = L.delta[l - c0], ndr = L.delta[r - c0 + 1];
+          if (is_patch_root(mypatch, l, r, n, ndl, ndr)) b.patch_root[mypatch] = parent;
+          node = parent;
+          if (l == 0 && r == n - 1) {
+            *b.root = parent;
+            store_rootbox<true>(b, own);
+            alive = false;
+          } else {
+            const bool up_left = (l == 0) || (r != n - 1 && ndr < ndl);
+            half = 2u * (up_left ? r : l - 1) + (up_left ? 0u : 1u);
+          }
+          publish<true>(L, l - c0, r - c0, own, r, mypatch);
+        }
+      }
+    }
+    store_level32(b, L, c0, node, half, l - c0);
   }
 }
 
@@ -1166,7 +1273,8 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
     if (n > 1) store_leaf_halves32(b, L, t, act ? 2 * parent + (uint32_t)side : kEnd, e, c0, n);
   }
   __syncthreads();  // counters, deltas and leaf slots ready
-  if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L, own);
+  if constexpr (FP32) climb_chunk32(b, n, act && n > 1, p, p, pk, c0, c1, L, own);
+  else if (act && n > 1) climb_chunk<FP32>(b, n, p, p, pk, c0, c1, L, own);
   __syncthreads();
   // a first arrival whose sibling lies outside the chunk leaves its arrival word
   if (c0 + t + 1 < c1 && L.cnt[t] == 1)
