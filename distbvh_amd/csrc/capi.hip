@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -36,12 +37,17 @@ bvh_status fail(bvh_status s, const std::string& msg) {
   return s;
 }
 
+// On failure the runtime's last-error state is cleared too, so that the
+// error is reported once, by this call, and not again by the caller's next
+// runtime call (torch checks hipGetLastError after its own launches).
 #define HIP_TRY(expr)                                                                       \
   do {                                                                                      \
     hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess)                                                                   \
+    if (_e != hipSuccess) {                                                                 \
+      (void)hipGetLastError();                                                              \
       return fail(_e == hipErrorOutOfMemory ? BVH_E_OOM : BVH_E_HIP,                        \
                   std::string(#expr) + ": " + hipGetErrorString(_e));                        \
+    }                                                                                       \
   } while (0)
 
 #define BVH_TRY(expr)                    \
@@ -106,11 +112,15 @@ struct DevBuf {
 struct Stage {
   const char* name;
   hipEvent_t a, b;
+  hipStream_t s;  // the stream its events are recorded on
+  bool open;      // begun, not ended
 };
 
 }  // namespace
 
 struct bvh_object_s;
+// one or a few launches of a build, run on the object's stream
+using BuildStep = std::function<bvh_status()>;
 
 // The host-side state of a query launched inside a capture: which partner
 // and kind, the list capacities its kernels were launched with and where its
@@ -174,15 +184,22 @@ struct bvh_world_s {
   }
   void stage_begin(const char* name) {
     if (!(cfg.flags & BVH_FLAG_TIMING) || capturing || stages.size() >= 512) return;
-    Stage s{name, take_event(), take_event()};
+    Stage s{name, take_event(), take_event(), stream, true};
     if (s.a && s.b) {
       (void)hipEventRecord(s.a, stream);
       stages.push_back(s);
     }
   }
+  // ends the newest open stage of the current stream (the pending builds of
+  // several objects interleave their stages, each on its own stream)
   void stage_end() {
-    if (!(cfg.flags & BVH_FLAG_TIMING) || capturing || stages.empty()) return;
-    (void)hipEventRecord(stages.back().b, stream);
+    if (!(cfg.flags & BVH_FLAG_TIMING) || capturing) return;
+    for (size_t i = stages.size(); i-- > 0;)
+      if (stages[i].open && stages[i].s == stream) {
+        (void)hipEventRecord(stages[i].b, stream);
+        stages[i].open = false;
+        return;
+      }
   }
 };
 
@@ -250,12 +267,15 @@ struct bvh_object_s {
   hipStream_t bs = nullptr;
   hipEvent_t ev_fork = nullptr, ev_done = nullptr;
   bool pending = false;
-  // A forked build enqueues its tree stage later, at the next call that
-  // needs a tree (tree_pending), after the sort stages of every build pending
-  // then (ev_sorted): the latency-bound sorts of several objects share the
-  // GPU, and no sort waits behind another object's tree kernel.
+  // A forked build enqueues its launches later, at the next call that reads
+  // an object (build_pending, steps: the launches before the tree stage):
+  // the pending builds' steps interleaved, then each tree stage after the
+  // steps of all of them (ev_sorted), so that the latency-bound sorts of
+  // several objects share the GPU and no sort waits behind another object's
+  // tree kernel.
+  std::vector<BuildStep> steps;
   hipEvent_t ev_sorted = nullptr;
-  bool tree_pending = false;
+  bool build_pending = false;
 
   bvh_object_s() = default;
   bvh_object_s(const bvh_object_s&) = delete;
@@ -300,12 +320,12 @@ static bvh_status join(bvh_object_s* o) {
   HIP_TRY(hipStreamWaitEvent(o->w->stream, o->ev_done, 0));
   return BVH_OK;
 }
-static bvh_status flush_trees(bvh_world w);
+static bvh_status flush_builds(bvh_world w);
 // ... for every object of the world, after enqueuing the tree stages still
 // pending: every call that reads objects' device data or may be followed by
 // the caller's use of it starts with this.
 static bvh_status join_all(bvh_world w) {
-  BVH_TRY(flush_trees(w));
+  BVH_TRY(flush_builds(w));
   for (auto& o : w->objects) BVH_TRY(join(o.get()));
   return BVH_OK;
 }
@@ -506,15 +526,85 @@ static bvh_status split_and_order(bvh_object_s* o) {
 
 static bvh_status object_build_tree(bvh_object_s* o);
 
+// The launches of a build up to (not including) its tree stage, as steps run
+// in order on the object's stream: the bounds, the Morton codes, the sort's
+// plan and each of its passes, the patch splits, the frame and the split
+// heights.  Host state that later calls read is set when the steps are made.
+static void object_build_steps(bvh_object_s* o, std::vector<BuildStep>& st) {
+  bvh_world w = o->w;
+  const hipStream_t s = w->stream;
+  const uint64_t n = o->in.n;
+  st.clear();
+  st.push_back([=] {
+    HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, fp32 slab error
+    w->stage_begin("bounds");
+    HIP_TRY(launch_prep(o->in, o->bb, s));
+    w->stage_end();
+    return BVH_OK;
+  });
+  st.push_back([=] {
+    w->stage_begin("morton");
+    HIP_TRY(launch_morton(o->in, o->bb, o->bits, s));
+    w->stage_end();
+    return BVH_OK;
+  });
+  const int npass = o->bits == 63 ? 8 : 4;
+  for (int p = -1; p < npass; ++p) {
+    st.push_back([=] {
+      if (p < 0) w->stage_begin("sort");
+      if (o->bits == 63) {
+        SortArgs<uint64_t> a{};
+        a.k0 = (const uint64_t*)o->bb.codes0; a.ka = (uint64_t*)o->bb.codes_a; a.kb = (uint64_t*)o->bb.codes_b;
+        a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
+        a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
+        HIP_TRY(p < 0 ? launch_sort_plan<uint64_t>(a, npass, o->bb.hist, s) : launch_sort_pass<uint64_t>(a, p, s));
+      } else {
+        SortArgs<uint32_t> a{};
+        a.k0 = (const uint32_t*)o->bb.codes0; a.ka = (uint32_t*)o->bb.codes_a; a.kb = (uint32_t*)o->bb.codes_b;
+        a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
+        a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
+        HIP_TRY(p < 0 ? launch_sort_plan<uint32_t>(a, npass, o->bb.hist, s) : launch_sort_pass<uint32_t>(a, p, s));
+      }
+      if (p == npass - 1) w->stage_end();
+      return BVH_OK;
+    });
+  }
+  st.push_back([=] {
+    if (o->split == BVH_SPLIT_CLUSTERING) {
+      w->stage_begin("cluster");
+      HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
+      w->stage_end();
+    } else {
+      w->stage_begin("split");
+      BVH_TRY(split_and_order(o));
+      w->stage_end();
+    }
+    if (o->fp32) {
+      // the object's fp32 frame: the Morton bounds minimum of this build (a
+      // refit keeps it)
+      HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
+    } else {
+      HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));  // written at the patch roots
+    }
+    w->stage_begin("delta");
+    HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
+    w->stage_end();
+    return BVH_OK;
+  });
+  o->patch_slabs_ready = !o->fp32;
+}
+
 // The whole object build, stream ordered, no host synchronization.  With
-// defer_tree the tree stage is left to flush_trees (the object's tree_pending).
-static bvh_status object_build(bvh_object_s* o, bool defer_tree) {
+// defer its launches are left to flush_builds (the object's build_pending),
+// which interleaves them with the other pending builds'.
+static bvh_status object_build(bvh_object_s* o, bool defer) {
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   const uint64_t n = o->in.n;
   BVH_TRY(object_alloc(o));
-  HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, fp32 slab error
+  o->steps.clear();
   if (n == 0) {
+    HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));
     // splits are all 0 (split/cluster.hpp:101-105), every patch empty
     HIP_TRY(launch_fill_u32(o->bb.splits, w->od > 1 ? w->od - 1 : 0, 0u, s));
     HIP_TRY(launch_fill_u32(o->bb.patch_root, w->od, kEnd, s));
@@ -523,53 +613,14 @@ static bvh_status object_build(bvh_object_s* o, bool defer_tree) {
     o->built = true;
     return BVH_OK;
   }
-  w->stage_begin("bounds");
-  HIP_TRY(launch_prep(o->in, o->bb, s));
-  w->stage_end();
-  w->stage_begin("morton");
-  HIP_TRY(launch_morton(o->in, o->bb, o->bits, s));
-  w->stage_end();
-  w->stage_begin("sort");
-  const int npass = o->bits == 63 ? 8 : 4;
-  if (o->bits == 63) {
-    SortArgs<uint64_t> a{};
-    a.k0 = (const uint64_t*)o->bb.codes0; a.ka = (uint64_t*)o->bb.codes_a; a.kb = (uint64_t*)o->bb.codes_b;
-    a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
-    a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
-    HIP_TRY(launch_sort_passes<uint64_t>(a, npass, o->bb.hist, s));
-  } else {
-    SortArgs<uint32_t> a{};
-    a.k0 = (const uint32_t*)o->bb.codes0; a.ka = (uint32_t*)o->bb.codes_a; a.kb = (uint32_t*)o->bb.codes_b;
-    a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
-    a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
-    HIP_TRY(launch_sort_passes<uint32_t>(a, npass, o->bb.hist, s));
-  }
-  w->stage_end();
-  if (o->split == BVH_SPLIT_CLUSTERING) {
-    w->stage_begin("cluster");
-    HIP_TRY(launch_cluster(o->bb, n, o->bits, w->od - 1, s));
-    w->stage_end();
-  } else {
-    w->stage_begin("split");
-    BVH_TRY(split_and_order(o));
-    w->stage_end();
-  }
-  if (o->fp32) {
-    // the object's fp32 frame: the Morton bounds minimum of this build (a
-    // refit keeps it)
-    HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
-  } else {
-    HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));  // written at the patch roots
-  }
-  o->patch_slabs_ready = !o->fp32;
-  w->stage_begin("delta");
-  HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
-  w->stage_end();
+  object_build_steps(o, o->steps);
   o->built = true;
-  if (defer_tree) {
-    o->tree_pending = true;
+  if (defer) {
+    o->build_pending = true;
     return BVH_OK;
   }
+  for (BuildStep& f : o->steps) BVH_TRY(f());
+  o->steps.clear();
   return object_build_tree(o);
 }
 
@@ -582,28 +633,42 @@ static bvh_status object_build_tree(bvh_object_s* o) {
   return BVH_OK;
 }
 
-// Enqueues the pending tree stages of the world's forked builds, each on its
-// build stream after the sort stages of all of them, and moves each build's
-// ev_done behind its tree.
-static bvh_status flush_trees(bvh_world w) {
+// Enqueues the world's pending (forked) builds: their steps interleaved, one
+// step of each build in turn, each on its build stream, so that every build
+// has work on the GPU from the start and the latency-bound sort passes of
+// several builds run side by side; then each tree stage, after the steps of
+// all of them (no sort pass waits behind another build's tree kernel); and
+// moves each build's ev_done behind its tree.
+static bvh_status flush_builds(bvh_world w) {
   std::vector<bvh_object_s*> p;
   for (auto& o : w->objects)
-    if (o->tree_pending) p.push_back(o.get());
+    if (o->build_pending) p.push_back(o.get());
   if (p.empty()) return BVH_OK;
+  std::vector<bvh_status> sa(p.size(), BVH_OK);
+  size_t nsteps = 0;
+  for (bvh_object_s* a : p) nsteps = std::max(nsteps, a->steps.size());
+  for (size_t i = 0; i < nsteps; ++i)
+    for (size_t j = 0; j < p.size(); ++j)
+      if (i < p[j]->steps.size() && sa[j] == BVH_OK) {
+        StreamSwap swap(w, p[j]->bs);
+        sa[j] = p[j]->steps[i]();
+      }
+  for (bvh_object_s* a : p) HIP_TRY(hipEventRecord(a->ev_sorted, a->bs));
   for (bvh_object_s* a : p)
     for (bvh_object_s* b : p)
       if (a != b) HIP_TRY(hipStreamWaitEvent(a->bs, b->ev_sorted, 0));
   bvh_status st = BVH_OK;
-  for (bvh_object_s* a : p) {
-    a->tree_pending = false;
-    bvh_status sa;
-    {
+  for (size_t j = 0; j < p.size(); ++j) {
+    bvh_object_s* a = p[j];
+    a->build_pending = false;
+    a->steps.clear();
+    if (sa[j] == BVH_OK) {
       StreamSwap swap(w, a->bs);
-      sa = object_build_tree(a);
+      sa[j] = object_build_tree(a);
     }
     // recorded also after a failure, so that nothing enqueued is left unordered
     HIP_TRY(hipEventRecord(a->ev_done, a->bs));
-    if (st == BVH_OK) st = sa;
+    if (st == BVH_OK) st = sa[j];
   }
   return st;
 }
@@ -630,7 +695,7 @@ static bvh_status object_refit(bvh_object_s* o) {
 
 static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                    const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
-                                   uint32_t split, hipStream_t bstream, bool defer_tree);
+                                   uint32_t split, hipStream_t bstream, bool defer);
 
 static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                     const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
@@ -654,11 +719,13 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
   bvh_world w = o->w;
   if (o->pending) {
-    // a tree stage still pending is superseded by this build (nothing read
-    // that tree); the previous build may still use the buffers this one may
+    // a build still pending is superseded by this one (nothing read it and
+    // its launches were never made); the previous build may still use the
+    // buffers this one may
     // reallocate (inside a capture nothing is reallocated, and nothing may
     // synchronize)
-    o->tree_pending = false;
+    o->build_pending = false;
+    o->steps.clear();
     if (!w->capturing) HIP_TRY(hipStreamSynchronize(o->bs));
     BVH_TRY(join(o));
   }
@@ -678,7 +745,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
     HIP_TRY(hipEventRecord(o->ev_fork, w->stream));
     HIP_TRY(hipStreamWaitEvent(o->bs, o->ev_fork, 0));
   }
-  // Inside a capture the tree stage is not deferred: deferral makes the build
+  // Inside a capture the build is not deferred: deferral makes the build
   // streams wait on each other directly, and hipStreamEndCapture crashed in
   // the host runtime on a graph holding such waits between two builds.  The
   // capture keeps the fork / join pattern only (a graph branch per build).
@@ -686,8 +753,10 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   const bvh_status st =
       set_entity_build(o, kind, nv, eps, data, cent, n, gids, mem, split, fork ? o->bs : w->stream, defer);
   if (fork) {
-    if (st != BVH_OK) o->tree_pending = false;
-    if (o->tree_pending) HIP_TRY(hipEventRecord(o->ev_sorted, o->bs));
+    if (st != BVH_OK) {
+      o->build_pending = false;
+      o->steps.clear();
+    }
     // the world stream waits for this at the next call that needs it (also
     // after a failed build, so that nothing it enqueued is left unordered)
     HIP_TRY(hipEventRecord(o->ev_done, o->bs));
@@ -698,7 +767,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
 
 static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps, const double* data,
                                    const double* cent, uint64_t n, const uint64_t* gids, uint32_t mem,
-                                   uint32_t split, hipStream_t bstream, bool defer_tree) {
+                                   uint32_t split, hipStream_t bstream, bool defer) {
   bvh_world w = o->w;
   StreamSwap swap(w, bstream);
   hipStream_t s = w->stream;
@@ -743,8 +812,9 @@ static bvh_status set_entity_build(bvh_object_s* o, int kind, int nv, double eps
   }
   o->have_data = true;
   o->built = false;
-  o->tree_pending = false;
-  return object_build(o, defer_tree);
+  o->build_pending = false;
+  o->steps.clear();
+  return object_build(o, defer);
 }
 
 // Counter words of a query (device, o->count): pairs, node visits, fp32
@@ -1271,7 +1341,7 @@ static void orphan_graphs(bvh_world w);
 
 bvh_status bvh_world_destroy(bvh_world w) {
   if (!w) return BVH_OK;
-  for (auto& o : w->objects) o->tree_pending = false;  // trees nobody reads
+  for (auto& o : w->objects) o->build_pending = false;  // builds nobody reads
   (void)join_all(w);
   (void)hipStreamSynchronize(w->stream);
   // graphs outlive their world only as handles to destroy
@@ -1347,6 +1417,7 @@ bvh_status bvh_stage_times(bvh_world w, const char** names, double* ms, size_t c
   HIP_TRY(hipStreamSynchronize(w->stream));
   size_t k = 0;
   for (const Stage& st : w->stages) {
+    if (st.open) continue;  // never ended (a failed call)
     if (k < cap) {
       float t = 0.f;
       HIP_TRY(hipEventElapsedTime(&t, st.a, st.b));
@@ -1376,7 +1447,7 @@ bvh_status bvh_object_release(bvh_object o) {
   if (!o) return fail(BVH_E_INVALID, "null object");
   bvh_world w = o->w;
   if (w->capturing) return fail(BVH_E_STATE, "release inside a capture");
-  o->tree_pending = false;  // nobody reads this tree any more
+  o->build_pending = false;  // nobody reads this build any more
   BVH_TRY(join_all(w));
   // queued work of the world and of the object's build stream may still use
   // the buffers

@@ -41,6 +41,11 @@ template <typename K>
 hipError_t launch_sort_histogram(const K* keys, uint64_t n, int npass, uint32_t* hist, hipStream_t s);
 template <typename K>
 hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s);
+// the same launches one at a time: the plan, then pass p (0 .. npass-1) in order
+template <typename K>
+hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s);
+template <typename K>
+hipError_t launch_sort_pass(const SortArgs<K>& a, int p, hipStream_t s);
 // exclusive scan in place; lb: sort_tiles(n) zeroed 64-bit words, counter: zeroed
 hipError_t launch_scan_u32(uint32_t* v, uint64_t n, uint64_t* lb, uint32_t* counter, hipStream_t s);
 

@@ -327,19 +327,29 @@ template hipError_t launch_sort_histogram<uint32_t>(const uint32_t*, uint64_t, i
 // Sort after the histograms are in `hist` (npass * 256 counters) and the
 // look-back words are zero.  Result in a.ka (and a.va).
 template <typename K>
-hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s) {
+hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, hist, a.n, npass, a.ctl);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const unsigned tiles = (unsigned)sort_tiles(a.n);
-  for (int p = 0; p < npass; ++p) {
-    hipLaunchKernelGGL(k_sort_pass<K>, dim3(tiles), dim3(kSortBlock), 0, s, a, p);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return hipGetLastError();
 }
+
+template <typename K>
+hipError_t launch_sort_pass(const SortArgs<K>& a, int p, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_pass<K>, dim3((unsigned)sort_tiles(a.n)), dim3(kSortBlock), 0, s, a, p);
+  return hipGetLastError();
+}
+
+template <typename K>
+hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s) {
+  hipError_t e = launch_sort_plan<K>(a, npass, hist, s);
+  for (int p = 0; p < npass && e == hipSuccess; ++p) e = launch_sort_pass<K>(a, p, s);
+  return e;
+}
+template hipError_t launch_sort_plan<uint64_t>(const SortArgs<uint64_t>&, int, const uint32_t*, hipStream_t);
+template hipError_t launch_sort_plan<uint32_t>(const SortArgs<uint32_t>&, int, const uint32_t*, hipStream_t);
+template hipError_t launch_sort_pass<uint64_t>(const SortArgs<uint64_t>&, int, hipStream_t);
+template hipError_t launch_sort_pass<uint32_t>(const SortArgs<uint32_t>&, int, hipStream_t);
 template hipError_t launch_sort_passes<uint64_t>(const SortArgs<uint64_t>&, int, const uint32_t*, hipStream_t);
 template hipError_t launch_sort_passes<uint32_t>(const SortArgs<uint32_t>&, int, const uint32_t*, hipStream_t);
 
