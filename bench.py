@@ -5,7 +5,8 @@ Workload (one step): two deformed contact sheets, 1024 x 1024 quads =
 z = 0.01 sin(2 pi 8 x) cos(2 pi 8 y); sheet B = A + (0, 0, 0.005) + seeded
 N(0, 0.002^2) vertex noise.  Per step both objects are rebuilt
 (26-DOP bounds, 63-bit Morton codes, radix sort, cluster splits into 256
-patches, bottom-up tree with refit, leaf size 4) and A is collided with B
+patches, bottom-up tree with refit; one element per leaf, the CPU oracle's
+trees use the workload's leaf size 4) and A is collided with B
 (patch broadphase + element narrowphase); the step ends when the pair count is
 on the host.
 
@@ -72,9 +73,9 @@ def stage_kernel(stage, fp32, launch):
     the launch batch: one record in scalar registers, or K staged in LDS)."""
     if stage == "narrowphase":
         if fp32:
-            return "k_narrow32<false>"
+            return "k_narrow32<false, false>"
         b = launch.get("batch", 1) if launch else 1
-        return "k_narrow64s<false>" if b == 1 else "k_narrow64<false, %d>" % b
+        return "k_narrow64s<false, false>" if b == 1 else "k_narrow64<false, %d, false>" % b
     return STAGE_KERNEL.get(stage, (None, None))[1 if fp32 else 0]
 
 
