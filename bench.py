@@ -551,10 +551,11 @@ def main():
                          "trees; the GPU trees hold one element per leaf and test leaves from their "
                          "parents' records, so it does not change their work")
     ap.add_argument("--precision", choices=["fp64", "fp32", "fp64_records", "both"], default="both",
-                    help="fp64: fp32-outward bounds in the tree, every pair decided in fp64 (the "
-                         "default form); fp32: the fp32-outward form (pairs fp32 proves are kept, the "
-                         "rest decided in fp64); fp64_records: fp64 slabs in every tree record; both: "
-                         "all three (the line reports fp64 and compares the others)")
+                    help="fp64: the default form (fp32-outward bounds in the tree; a candidate is "
+                         "kept when its fp32 margin proves the fp64 overlap, else decided in fp64; "
+                         "fp64 element k-DOP exports); fp32: the fp32-outward form (the same walk, fp32 "
+                         "slabs only); fp64_records: fp64 slabs in every tree record; both: all three "
+                         "(the line reports fp64 and compares the others)")
     ap.add_argument("--split", choices=["clustering", "geom_axis", "ml_geom_axis"], default="clustering",
                     help="sheets: the split_algorithm of set_entity_data (patches)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -639,12 +640,13 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64 (pair decisions) + f32 (outward pruning bounds)",
+        "dtype": {"fp64": "f64 (pair predicate) + f32 (outward bounds, margin proofs)",
                   "fp32": "f32 (outward slabs) + f64 (undecided pairs)", "fp64_records": "f64"}[form],
         "data": "synthetic",
         "config": {"workload": workload,
-                   "precision": {"fp64": "fp64: fp64 element slabs, every pair decided in fp64; tree records "
-                                         "hold fp32 bounds rounded outward (pruning only)",
+                   "precision": {"fp64": "fp64: the fp64 pair set; tree records hold fp32 bounds rounded "
+                                         "outward, a candidate is kept when its fp32 margin proves the fp64 "
+                                         "overlap, else decided on fp64 k-DOPs",
                                  "fp32": "fp32-outward slabs, exact fp64 re-check of what fp32 cannot prove",
                                  "fp64_records": "fp64 slabs in every tree record"}[form],
                    "elements_per_object": n, "od": args.od,

@@ -292,9 +292,10 @@ struct LaunchShape {
 };
 
 // Tree forms: fp64 slabs in every record (kFormRecords64); fp32-outward
-// records whose leaf candidates fp32 cannot prove are decided in fp64
-// (kFormFp32); fp32-outward records with every leaf candidate decided in
-// fp64 (kFormMixed, the default fp64 form).
+// records (kFormFp32 and kFormMixed, the default fp64 form): the same walk,
+// which keeps the leaf candidates whose fp32 gap proves the fp64 overlap and
+// decides the rest on fp64 k-DOPs, so both give the fp64 pair set; the fp64
+// form also serves exact fp64 element k-DOPs (bvh_object_sorted_kdops).
 constexpr int kFormRecords64 = 0;
 constexpr int kFormFp32 = 1;
 constexpr int kFormMixed = 2;

@@ -979,9 +979,9 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
     if (!out.recheck || !out.rcount || !out.rtotal) return hipErrorInvalidValue;
     // the fp32 walk tests leaves from their parents' records: no leaf clusters
     // one record per round trip: it arrives in scalar registers (the batch
-    // applies to the fp64 walk).  The mixed form proves nothing in fp32: every
-    // leaf candidate goes to the fp64 re-check.
-    const int32_t inner = form == kFormMixed ? 0x7fffffff : kInnerUlps;
+    // applies to the fp64 walk).  Both forms keep the leaf candidates whose
+    // fp32 gap proves the fp64 overlap and re-check the rest in fp64.
+    const int32_t inner = kInnerUlps;
     if (self == 2) hipLaunchKernelGGL((k_narrow32<true, true>), dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
     else if (self) hipLaunchKernelGGL(k_narrow32<true>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
     else hipLaunchKernelGGL(k_narrow32<false>, dim3(blocks), dim3(bs), 0, s, q, t, xcd, out, inner);
@@ -1010,8 +1010,7 @@ hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t 
                                     hipStream_t s) {
   if (nq == 0 || blocks == 0) return hipSuccess;
   if (form == kFormFp32 || form == kFormMixed) {
-    const int32_t inner = form == kFormMixed ? 0x7fffffff : kInnerUlps;
-    hipLaunchKernelGGL(k_narrow32_batch, dim3(blocks), dim3(256), 0, s, bq, nq, inner);
+    hipLaunchKernelGGL(k_narrow32_batch, dim3(blocks), dim3(256), 0, s, bq, nq, kInnerUlps);
   } else if (batch == 4) {
     hipLaunchKernelGGL(k_narrow64_batch<4>, dim3(blocks), dim3(256), 0, s, bq, nq);
   } else {

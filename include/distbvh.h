@@ -59,11 +59,12 @@ typedef enum bvh_split_algorithm {
 typedef enum bvh_memory { BVH_MEM_HOST = 0, BVH_MEM_DEVICE = 1 } bvh_memory;
 
 typedef enum bvh_precision {
-  BVH_PREC_FP64 = 0,          /* fp64 element slabs, every pair decided in fp64 (bit-exact with
-                                 the CPU path); the tree's internal bounds are fp32 rounded
-                                 outward, so they only prune */
-  BVH_PREC_FP32_OUTWARD = 1,  /* fp32 slabs rounded outward; leaf pairs fp32 proves are kept,
-                                 the rest decided in fp64 (same pairs) */
+  BVH_PREC_FP64 = 0,          /* the fp64 pair set (bit-exact with the CPU path): tree bounds
+                                 are fp32 rounded outward; a leaf candidate is kept when its
+                                 fp32 margin proves the fp64 overlap, else decided on fp64
+                                 k-DOPs; element k-DOPs are exported in fp64 */
+  BVH_PREC_FP32_OUTWARD = 1,  /* the same walk and pairs, fp32 slabs only (no fp64 element
+                                 k-DOP export) */
   BVH_PREC_FP64_RECORDS = 2   /* fp64 slabs in every tree record (the all-fp64 walk) */
 } bvh_precision;
 
@@ -230,9 +231,10 @@ bvh_status bvh_result_pairs_device(bvh_object o, const uint64_t **pairs, uint64_
 /* counters of the last query: tree nodes visited (summed over waves of 64
  * queries), broadphase patch pairs, candidate pairs (fp32-outward records:
  * pairs whose outward-rounded fp32 slabs overlap; fp64 records: the pair
- * count), rechecked pairs (candidates decided on fp64 k-DOPs: every candidate
- * in BVH_PREC_FP64, those too close for fp32 in BVH_PREC_FP32_OUTWARD, 0 with
- * fp64 records) and the largest fp32 slab error in ulps of the two objects'
+ * count), rechecked pairs (candidates decided on fp64 k-DOPs: with
+ * fp32-outward records those whose fp32 gap proves nothing, every candidate
+ * between two objects, whose frames differ; 0 with fp64 records) and the
+ * largest fp32 slab error in ulps of the two objects'
  * leaf slabs (0 with fp64 records; measured by this call from the element
  * data, which must still be valid).  Any pointer may be NULL. */
 bvh_status bvh_result_stats(bvh_object o, uint64_t *node_visits, uint64_t *patch_pairs,

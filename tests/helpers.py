@@ -6,8 +6,9 @@ import oracle as O  # oracle/oracle.py (checker)
 from distbvh_amd import workloads as W
 
 
-# the three tree forms: fp32-outward records with every pair decided in fp64
-# (the default), the fp32-outward form, fp64 slabs in every record
+# the three tree forms: fp32-outward records (the default fp64 form, which
+# also exports fp64 element k-DOPs, and the fp32-outward form), fp64 slabs in
+# every record
 FORMS = ["fp64", "fp32", "fp64_records"]
 
 
