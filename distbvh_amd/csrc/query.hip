@@ -154,10 +154,12 @@ __device__ __forceinline__ int32_t min_gap(const int32_t* q, P b) {
 
 // The same v_cmpx chain on ordered fp32 images (int32 compares): the strict
 // overlap of the lane's query slabs with one wave-uniform box, for internal
-// children of the fp32 walk (a leaf child needs the gap, min_gap).
-__device__ __forceinline__ bool box_hits32(const int32_t* q, const int32_t* b) {
+// children of the fp32 walk (a leaf child needs the gap, min_gap).  Returns
+// the mask of the lanes that hit (EXEC after the chain; every lane of the
+// wave is active in the walk), which stays in scalar registers: an internal
+// child only needs to know whether any lane hits it.
+__device__ __forceinline__ uint64_t box_mask32(const int32_t* q, const int32_t* b) {
   unsigned long long res, save, m;
-  int v;
   asm volatile(
       "s_mov_b64 %[save], exec\n\t"
       "v_cmpx_lt_i32_e64 %[m], %[q0], %[b1]\n\t"
@@ -189,11 +191,10 @@ __device__ __forceinline__ bool box_hits32(const int32_t* q, const int32_t* b) {
       "s_nop 4\n\t"
       "s_mov_b64 %[res], exec\n\t"
       "s_mov_b64 exec, %[save]\n\t"
-      "v_cndmask_b32_e64 %[v], 0, 1, %[res]\n\t"
-      : [res] "=&s"(res), [save] "=&s"(save), [m] "=&s"(m), [v] "=&v"(v)
+      : [res] "=&s"(res), [save] "=&s"(save), [m] "=&s"(m)
       : [q0] "v"(q[0]), [q1] "v"(q[1]), [q2] "v"(q[2]), [q3] "v"(q[3]), [q4] "v"(q[4]), [q5] "v"(q[5]), [q6] "v"(q[6]), [q7] "v"(q[7]), [q8] "v"(q[8]), [q9] "v"(q[9]), [q10] "v"(q[10]), [q11] "v"(q[11]), [q12] "v"(q[12]), [q13] "v"(q[13]), [q14] "v"(q[14]), [q15] "v"(q[15]), [q16] "v"(q[16]), [q17] "v"(q[17]), [q18] "v"(q[18]), [q19] "v"(q[19]), [q20] "v"(q[20]), [q21] "v"(q[21]), [q22] "v"(q[22]), [q23] "v"(q[23]), [q24] "v"(q[24]), [q25] "v"(q[25]),
         [b0] "s"(b[0]), [b1] "s"(b[1]), [b2] "s"(b[2]), [b3] "s"(b[3]), [b4] "s"(b[4]), [b5] "s"(b[5]), [b6] "s"(b[6]), [b7] "s"(b[7]), [b8] "s"(b[8]), [b9] "s"(b[9]), [b10] "s"(b[10]), [b11] "s"(b[11]), [b12] "s"(b[12]), [b13] "s"(b[13]), [b14] "s"(b[14]), [b15] "s"(b[15]), [b16] "s"(b[16]), [b17] "s"(b[17]), [b18] "s"(b[18]), [b19] "s"(b[19]), [b20] "s"(b[20]), [b21] "s"(b[21]), [b22] "s"(b[22]), [b23] "s"(b[23]), [b24] "s"(b[24]), [b25] "s"(b[25]));
-  return v != 0;
+  return res;
 }
 
 // Per-wave LDS staging of the candidates queued for the fp64 re-check: one
@@ -339,30 +340,39 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
     // internal children: overlap only (v_cmpx chain); leaf children: the gap,
     // which also decides whether fp32 proves the fp64 overlap
     const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
+    // per lane, whether a child may hold its pairs at all: position bounds
+    // (self) and the own patch (cross), non-short-circuit &: no branch
+    bool okl = true, okr = true;
+    if (SELF) {
+      okl = x > p32;  // the left child ends at position x
+      okr = rlast > p32;
+    }
+    if (CROSS) {  // children [rfirst, x] and [x + 1, rlast] inside the lane's own patch
+      okl = okl & !((rfirst >= pb) & (x < pe));
+      okr = okr & !((x + 1 >= pb) & (rlast < pe));
+    }
+    // the lanes hitting each child, as scalar masks; a leaf child also keeps
+    // each lane's gap and hit for the candidate test
     int32_t gl = 0, gr = 0;
-    bool hl, hr;
+    bool hl = false, hr = false;
+    uint64_t ml, mr;
     if (leaf_l) {
       gl = min_gap(qf, rw);
-      hl = gl > 0;
+      hl = (gl > 0) & okl;
+      ml = __ballot(hl);
     } else {
-      hl = box_hits32(qf, rw);
+      ml = box_mask32(qf, rw);
+      if (SELF || CROSS) ml &= __ballot(okl);
     }
     if (leaf_r) {
       gr = min_gap(qf, rw + 28);
-      hr = gr > 0;
+      hr = (gr > 0) & okr;
+      mr = __ballot(hr);
     } else {
-      hr = box_hits32(qf, rw + 28);
+      mr = box_mask32(qf, rw + 28);
+      if (SELF || CROSS) mr &= __ballot(okr);
     }
-    if (SELF) {
-      // position bounds (non-short-circuit &: no divergent branch)
-      hl = hl & (x > p32);  // the left child ends at position x
-      hr = hr & (rlast > p32);
-    }
-    if (CROSS) {  // children [rfirst, x] and [x + 1, rlast] inside the lane's own patch
-      hl = hl & !((rfirst >= pb) & (x < pe));
-      hr = hr & !((x + 1 >= pb) & (rlast < pe));
-    }
-    const bool any_l = __ballot(hl) != 0, any_r = __ballot(hr) != 0;
+    const bool any_l = ml != 0, any_r = mr != 0;
     if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl & (cl - tleaf0 > p32)) : hl);
     if (any_r && leaf_r) leaf_check(gr, er, SELF ? (hr & (cr - tleaf0 > p32)) : hr);
     const bool go_l = any_l && !leaf_l, go_r = any_r && !leaf_r;
