@@ -351,26 +351,23 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
       okl = okl & !((rfirst >= pb) & (x < pe));
       okr = okr & !((x + 1 >= pb) & (rlast < pe));
     }
-    // the lanes hitting each child, as scalar masks; a leaf child also keeps
-    // each lane's gap and hit for the candidate test
+    // the lanes hitting each child, as scalar masks (the v_cmpx chain tests
+    // the same strict overlap as gap > 0); a leaf child that some lane hits
+    // also gets each lane's gap and hit for the candidate test
     int32_t gl = 0, gr = 0;
     bool hl = false, hr = false;
-    uint64_t ml, mr;
-    if (leaf_l) {
+    uint64_t ml = box_mask32(qf, rw), mr = box_mask32(qf, rw + 28);
+    if (SELF || CROSS) {
+      ml &= __ballot(okl);
+      mr &= __ballot(okr);
+    }
+    if (leaf_l && ml != 0) {
       gl = min_gap(qf, rw);
       hl = (gl > 0) & okl;
-      ml = __ballot(hl);
-    } else {
-      ml = box_mask32(qf, rw);
-      if (SELF || CROSS) ml &= __ballot(okl);
     }
-    if (leaf_r) {
+    if (leaf_r && mr != 0) {
       gr = min_gap(qf, rw + 28);
       hr = (gr > 0) & okr;
-      mr = __ballot(hr);
-    } else {
-      mr = box_mask32(qf, rw + 28);
-      if (SELF || CROSS) mr &= __ballot(okr);
     }
     const bool any_l = ml != 0, any_r = mr != 0;
     if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl & (cl - tleaf0 > p32)) : hl);
