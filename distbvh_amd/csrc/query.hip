@@ -281,6 +281,7 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
   uint32_t pb = 0, pe = 0;
   if (CROSS) own_patch_range(q, qa, valid, n, pb, pe);
   CRec32* recs = (CRec32*)t.rec32;
+  uint32_t sink = 0, pf = 0;  // record prefetch (below)
   uint32_t steps = 0;
   uint32_t ncand = 0;
   // fp32 candidates at one (wave-uniform) leaf, gap g per lane: emit what
@@ -340,6 +341,16 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
     // internal children: overlap only (v_cmpx chain); leaf children: the gap,
     // which also decides whether fp32 proves the fp64 overlap
     const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
+    // The next record the walk loads is one of these children's (or one it
+    // pushed earlier).  Lanes 0-1 load a word of each 128-byte line of the
+    // left child's record and lanes 2-3 of the right child's (internal
+    // children only), so the lines are on their way to L2 while this record
+    // is tested; the words feed a sink that nothing reads.
+    {
+      sink += pf;
+      const uint32_t c = lane < 2 ? cl : cr;
+      pf = (lane < 4 && c < tleaf0) ? ((const uint32_t*)(recs + c))[(lane & 1) * 32] : 0u;
+    }
     // per lane, whether a child may hold its pairs at all: position bounds
     // (self) and the own patch (cross), non-short-circuit &: no branch
     bool okl = true, okr = true;
@@ -387,6 +398,8 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
   }
   wave_flush(buf, fill, out);
   recheck_flush(rbuf, rfill, out, rstripe);
+  sink += pf;
+  asm volatile("; record prefetch sink %0" ::"v"(sink));  // keeps the prefetch loads
   if (lane == 0) {
     if (out.visits) atomicAdd(out.visits, (unsigned long long)steps);
     if (out.candidates) atomicAdd(out.candidates, (unsigned long long)ncand);
