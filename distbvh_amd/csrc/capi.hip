@@ -274,6 +274,7 @@ struct bvh_object_s {
   // several objects share the GPU and no sort waits behind another object's
   // tree kernel.
   std::vector<BuildStep> steps;
+  size_t steps_done = 0;  // steps already launched (the first ones go at the call)
   hipEvent_t ev_sorted = nullptr;
   bool build_pending = false;
 
@@ -526,6 +527,9 @@ static bvh_status split_and_order(bvh_object_s* o) {
 
 static bvh_status object_build_tree(bvh_object_s* o);
 
+// steps of a deferred build launched by its set_entity_data call itself
+constexpr size_t kEagerSteps = 2;
+
 // The launches of a build up to (not including) its tree stage, as steps run
 // in order on the object's stream: the bounds, the Morton codes, the sort's
 // plan and each of its passes, the patch splits, the frame and the split
@@ -616,6 +620,11 @@ static bvh_status object_build(bvh_object_s* o, bool defer) {
   object_build_steps(o, o->steps);
   o->built = true;
   if (defer) {
+    // the first steps (bounds, Morton codes) are launched at once, so the GPU
+    // works while the caller makes its next calls; the rest wait for the flush
+    o->steps_done = 0;
+    for (; o->steps_done < kEagerSteps && o->steps_done < o->steps.size(); ++o->steps_done)
+      BVH_TRY(o->steps[o->steps_done]());
     o->build_pending = true;
     return BVH_OK;
   }
@@ -646,13 +655,15 @@ static bvh_status flush_builds(bvh_world w) {
   if (p.empty()) return BVH_OK;
   std::vector<bvh_status> sa(p.size(), BVH_OK);
   size_t nsteps = 0;
-  for (bvh_object_s* a : p) nsteps = std::max(nsteps, a->steps.size());
+  for (bvh_object_s* a : p) nsteps = std::max(nsteps, a->steps.size() - a->steps_done);
   for (size_t i = 0; i < nsteps; ++i)
-    for (size_t j = 0; j < p.size(); ++j)
-      if (i < p[j]->steps.size() && sa[j] == BVH_OK) {
+    for (size_t j = 0; j < p.size(); ++j) {
+      const size_t k = p[j]->steps_done + i;
+      if (k < p[j]->steps.size() && sa[j] == BVH_OK) {
         StreamSwap swap(w, p[j]->bs);
-        sa[j] = p[j]->steps[i]();
+        sa[j] = p[j]->steps[k]();
       }
+    }
   for (bvh_object_s* a : p) HIP_TRY(hipEventRecord(a->ev_sorted, a->bs));
   for (bvh_object_s* a : p)
     for (bvh_object_s* b : p)
@@ -719,8 +730,8 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   if (n >= (1ull << 30)) return fail(BVH_E_INVALID, "at most 2^30 - 1 elements per object");
   bvh_world w = o->w;
   if (o->pending) {
-    // a build still pending is superseded by this one (nothing read it and
-    // its launches were never made); the previous build may still use the
+    // a build still pending is superseded by this one (nothing read it; its
+    // launches not made yet are dropped); the previous build may still use the
     // buffers this one may
     // reallocate (inside a capture nothing is reallocated, and nothing may
     // synchronize)
@@ -833,9 +844,7 @@ static bvh_status read_counts(bvh_object_s* o, bvh_object_s* other, bool* overfl
   bvh_world w = o->w;
   hipStream_t s = w->stream;
   const unsigned long long* dcount = o->count.as<unsigned long long>();
-  HIP_TRY(hipMemcpyAsync(&w->h_count[0], dcount, kCntWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[kCntWords], o->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&w->h_count[kCntWords + 2], other->status.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(launch_readback(dcount, kCntWords, o->status.as<uint32_t>(), other->status.as<uint32_t>(), w->h_count, s));
   HIP_TRY(hipStreamSynchronize(s));
   o->npairs = o->q_narrow ? w->h_count[kCntPairs] : 0;
   o->nvisits = w->h_count[kCntVisits];

@@ -317,6 +317,10 @@ struct BatchQuery {
 // `blocks` blocks of 256 threads; bq in device memory; batch 2 or 4 for the
 // walk over fp64 records.  The fp32-outward forms leave their re-check
 // lists to launch_recheck_query, one launch per query.
+// a query's ncount counter words, then the two objects' status words (4 x
+// u32 each, 8-byte aligned), into host (pinned, device-accessible) memory
+hipError_t launch_readback(const unsigned long long* count, int ncount, const uint32_t* st_a, const uint32_t* st_b,
+                           unsigned long long* host, hipStream_t s);
 hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, int form,
                                     hipStream_t s);
 hipError_t launch_recheck_query(const TreeView& q, const TreeView& t, const PairOut& out, hipStream_t s);

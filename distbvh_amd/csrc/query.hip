@@ -1026,6 +1026,27 @@ hipError_t launch_narrowphase(const TreeView& q, const TreeView& t, int leaf_siz
   return hipGetLastError();
 }
 
+// The counters of a query and both objects' status words (4 x u32 each) in
+// one launch, straight into pinned host memory: one small kernel instead of
+// three device-to-host copies.
+__global__ void k_readback(const unsigned long long* __restrict__ count, int ncount,
+                           const unsigned long long* __restrict__ st_a, const unsigned long long* __restrict__ st_b,
+                           unsigned long long* __restrict__ host) {
+  const int i = (int)threadIdx.x;
+  if (i < ncount) host[i] = count[i];
+  else if (i < ncount + 2) host[i] = st_a[i - ncount];
+  else if (i < ncount + 4) host[i] = st_b[i - ncount - 2];
+  __threadfence_system();  // the words are in host memory before the kernel ends
+}
+
+hipError_t launch_readback(const unsigned long long* count, int ncount, const uint32_t* st_a, const uint32_t* st_b,
+                           unsigned long long* host, hipStream_t s) {
+  if (ncount < 0 || ncount + 4 > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(64), 0, s, count, ncount, (const unsigned long long*)st_a,
+                     (const unsigned long long*)st_b, host);
+  return hipGetLastError();
+}
+
 hipError_t launch_narrowphase_batch(const BatchQuery* bq, uint32_t nq, uint32_t blocks, int batch, int form,
                                     hipStream_t s) {
   if (nq == 0 || blocks == 0) return hipSuccess;
