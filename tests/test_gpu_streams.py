@@ -1,8 +1,8 @@
 """Concurrent object builds (each set_entity_data builds on the object's own
-stream, joined into the world stream at the next call that is not a
-set_entity_data / init_broadphase) against serial builds
-(BVH_FLAG_SERIAL_BUILDS): the same trees, patches and pairs, in both
-precision forms, and the stream order the caller relies on."""
+stream; the pending builds' launches are interleaved and joined into the
+world stream at the next call that reads an object) against serial builds
+(BVH_FLAG_SERIAL_BUILDS): the same trees, patches and pairs, in every
+precision form, and the stream order the caller relies on."""
 import numpy as np
 import pytest
 
@@ -226,3 +226,37 @@ def test_concurrent_builds_edge_sizes(torch_cuda, precision):
                 out.append((objs[i].result_count(), objs[i].digest()))
         res[serial] = out
     assert res[True] == res[False]
+
+
+@pytest.mark.parametrize("precision", FORMS)
+def test_superseded_pending_build(torch_cuda, precision):
+    """A set_entity_data on an object whose build is still pending (its first
+    steps launched, the rest waiting for the next call that reads an object)
+    replaces that build: the object's pairs, patches and stage timings are
+    those of the second data alone, and so are the pairs of every object built
+    beside it."""
+    torch = torch_cuda
+    a, b = W.wavy_sheets(96)
+    a2 = a.copy()
+    a2[:, 2, :] += 0.004  # A moved towards B
+    ta, ta2, tb = (torch.from_numpy(x).cuda() for x in (a, a2, b))
+    res = {}
+    for mode in ("serial", "superseded"):
+        with CollisionWorld(8, precision=precision, timing=True, serial_builds=(mode == "serial")) as w:
+            oa = w.create_collision_object()
+            ob = w.create_collision_object()
+            if mode == "superseded":
+                oa.set_entity_data_poly(ta)  # never read
+            ob.set_entity_data_poly(tb)
+            oa.set_entity_data_poly(ta2)
+            oa.init_broadphase()
+            ob.init_broadphase()
+            oa.broadphase(ob)
+            pairs = sort_pairs(oa.pairs())
+            names = [n for n, _ in w.stage_times()]
+            res[mode] = (oa.result_count(), oa.digest(), pairs.tobytes(),
+                         [p["size"] for p in oa.local_patches()])
+            # every stage ended (a dropped build leaves no open stage behind)
+            assert names.count("tree") == 2 and names.count("sort") >= 2
+    assert res["serial"] == res["superseded"]
+    assert res["serial"][0] > 0
