@@ -610,6 +610,10 @@ __device__ __forceinline__ void store_patch_box(const BuildBufs& b, uint32_t pat
     unsigned long long* d = b.patch_k64 + (uint64_t)patch * kSlabs;
 #pragma unroll
     for (int i = 0; i < kSlabs; ++i) d[i] = ordered_u64(k[i]);
+  } else {
+    // the outward fp32 slabs, as the patch root's record half holds them
+#pragma unroll
+    for (int i = 0; i < kSlabs; ++i) b.patch_box32[(uint64_t)i * b.od + patch] = k[i];
   }
 }
 
@@ -1033,7 +1037,10 @@ __device__ __forceinline__ void climb_chunk32(const BuildBufs& b, uint64_t n, bo
           else l = c0 + ss;
           mypatch = (mypatch == spatch) ? mypatch : kMixed;
           const uint32_t ndl = L.delta[l - c0], ndr = L.delta[r - c0 + 1];
-          if (is_patch_root(mypatch, l, r, n, ndl, ndr)) b.patch_root[mypatch] = parent;
+          if (is_patch_root(mypatch, l, r, n, ndl, ndr)) {
+            b.patch_root[mypatch] = parent;
+            store_patch_box<true>(b, mypatch, own);
+          }
           node = parent;
           if (l == 0 && r == n - 1) {
             *b.root = parent;
@@ -1251,6 +1258,7 @@ __global__ __launch_bounds__(kTreeChunk) void k_tree_leaves(ElemInput in, BuildB
       // registers for the climb
 #pragma unroll
       for (int i = 0; i < kSlabs; ++i) own[i] = L.slab[i][t];
+      if (single) store_patch_box<true>(b, pk, own);
       if (n == 1) {
         *b.root = leaf;
         store_rootbox<true>(b, own);

@@ -218,7 +218,7 @@ struct bvh_object_s {
   DevBuf centroid, partials, bounds, codes0, codes_a, codes_b, perm, perm_b, hist, ctl,
       lookback, depth0, depth_a, depth_b, depth_idx, depth_idx_b, splits, delta, patch_of_pos,
       patch_of_input, counters, patch_root, root, rec64, rootbox64, rec32, rootbox32, status,
-      front_a, front_b, patch_k64;
+      front_a, front_b, patch_k64, patch_box32;
   DevBuf frame;  // fp32-outward records: the frame origin of this object's slabs (3 doubles), its 13 projections
   DevBuf slab_err;  // the slab-error statistic (one word)
   BuildBufs bb{};
@@ -298,7 +298,7 @@ struct bvh_object_s {
                      &perm, &perm_b, &hist, &ctl, &lookback, &depth0, &depth_a, &depth_b, &depth_idx,
                      &depth_idx_b, &splits, &delta, &patch_of_pos, &patch_of_input, &counters, &patch_root,
                      &root, &rec64, &rootbox64, &rec32, &rootbox32, &status, &frame, &slab_err,
-                     &front_a, &front_b, &patch_k64, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
+                     &front_a, &front_b, &patch_k64, &patch_box32, &pairs, &pairs_a, &pairs_b, &records, &ppairs,
                      &sort_hist, &sort_ctl, &sort_lb, &count, &recheck, &digest, &sp_rp, &sp_rp_b,
                      &sp_scan, &sp_tmpf, &sp_tmpt, &sp_seg, &sp_seg_next, &sp_k64, &sp_acc, &sp_axis,
                      &sp_flag, &sp_lb, &sp_ctr, &sp_key, &key0, &key_a, &key_b, &sp_hist, &sp_x,
@@ -356,6 +356,7 @@ static TreeView view_of(const bvh_object_s* o) {
   v.origin = o->fp32 ? o->frame.as<double>() : nullptr;
   v.elem = o->in;
   v.patch_k64 = o->patch_k64.as<unsigned long long>();
+  v.patch_box32 = o->fp32 ? o->patch_box32.as<int32_t>() : nullptr;
   v.splits = o->splits.as<uint32_t>();
   return v;
 }
@@ -407,6 +408,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
     BVH_TRY(o->rec32.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec32)));
     BVH_TRY(o->rootbox32.ensure(kSlabs * sizeof(int32_t)));
     BVH_TRY(o->frame.ensure(16 * sizeof(double)));
+    BVH_TRY(o->patch_box32.ensure(od * kSlabs * sizeof(int32_t)));
   } else {
     BVH_TRY(o->rec64.ensure((n1 > 1 ? n1 - 1 : 1) * sizeof(Rec64)));
     BVH_TRY(o->rootbox64.ensure(kSlabs * sizeof(double)));
@@ -444,6 +446,8 @@ static bvh_status object_alloc(bvh_object_s* o) {
   b.front_a = o->front_a.as<FrontNode>();
   b.front_b = o->front_b.as<FrontNode>();
   b.patch_k64 = o->patch_k64.as<unsigned long long>();
+  b.patch_box32 = o->fp32 ? o->patch_box32.as<int32_t>() : nullptr;
+  b.od = od;
   if (o->split != BVH_SPLIT_CLUSTERING) {
     BVH_TRY(o->sp_rp.ensure(n1 * 4));
     BVH_TRY(o->sp_rp_b.ensure(n1 * 4));

@@ -100,6 +100,11 @@ struct BuildBufs {
   FrontNode* front_a;      // 2 per chunk of kTreeLeafChunk leaves: nodes handed to the next build phase
   FrontNode* front_b;
   unsigned long long* patch_k64;  // od x 26 exact (fp64) patch slabs, ordered_u64 encoding
+  // fp32 form: the patch roots' slabs as their records hold them (ord32
+  // images), slab-major: slab a of patch k at a * od + k; written where the
+  // climb finds each patch root, read by the walks that start at the patches
+  int32_t* patch_box32;
+  uint64_t od;
 };
 
 // device error bits: set instead of faulting when an invariant breaks
@@ -255,6 +260,7 @@ struct TreeView {
   ElemInput elem;        // the object's element data (fp32 form re-checks in fp64)
   const unsigned long long* patch_k64;  // exact patch slabs (ordered_u64)
   const uint32_t* splits;               // od - 1 patch starts in tree-order positions
+  const int32_t* patch_box32;           // fp32 form: the patch roots' slabs, slab-major (a * od + k)
 };
 
 struct PairOut {

@@ -135,6 +135,32 @@ constexpr int32_t kInnerUlps = 6;
 // augmented keys (14 patch bits + 63 code bits + 30 position bits)
 constexpr uint32_t kStack = 128;
 
+// A walk of the fp32 records over a tree of 2..kFrontierMax patches starts at
+// the patch roots whose slabs overlap the union of the wave's query slabs
+// (one 26-compare test per patch, each lane testing every 64th patch) instead
+// of at the root: the visits above the patch roots are saved (on 2 x 2M
+// triangles in 256 patches, 32 of 200 per wave).  A wave's
+// queries are 64 consecutive positions of their own tree, so the union is
+// small and overlaps few patches.
+constexpr uint32_t kFrontierMax = 256;
+constexpr uint32_t kFrontierWords = kFrontierMax / kWave;
+constexpr uint32_t kFrontierPush = 32;
+
+// Minimum / maximum of v over the wave (every lane active), in a scalar
+// register: DPP within each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1],
+// row_ror 4, 8), then row_bcast 15 and 31 carry the rows' results to lane 63.
+template <bool MAX>
+__device__ __forceinline__ int32_t wave_reduce_i32(int32_t v) {
+  auto op = [](int32_t x, int32_t y) { return MAX ? max(x, y) : min(x, y); };
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 
 // Smallest gap over the 13 slab pairs between query q and box b, in ordered
 // fp32 images: min(b_hi - q_lo, q_hi - b_lo) with saturating subtraction
@@ -305,7 +331,82 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
   // the node to visit next (kEnd: pop the stack)
   uint32_t x = kEnd;
   uint32_t sp = 0;
-  {
+  const uint32_t od = (uint32_t)t.od;
+  bool from_root = true;
+  // (not in the cross-patch self walk: its workloads have many patches, and
+  // the start's registers would cost that kernel a wave per SIMD)
+  if (!CROSS && t.patch_box32 != nullptr && n > 1 && od > 1 && od <= kFrontierMax) {
+    // self: patches that end at or before the wave's first position hold no
+    // pair of it
+    const uint32_t pmin = __builtin_amdgcn_readlane(p32, 0);
+    // lane l tests patches l, l + 64, ... (clamped to the last patch, with
+    // the test off); the union is reduced one axis at a time, so that it
+    // takes two scalar registers and each axis's loads go out together
+    uint32_t kk[kFrontierWords];
+    bool h[kFrontierWords];
+#pragma unroll
+    for (uint32_t w = 0; w < kFrontierWords; ++w) {
+      const uint32_t k = w * kWave + lane;
+      kk[w] = (k < od ? k : od - 1) & (kFrontierMax - 1);  // (the mask: a 32-bit load offset)
+      h[w] = (k < od) & (t.patch_root[kk[w]] != kEnd);
+      if (SELF) h[w] = h[w] & ((kk[w] + 1 >= od ? n : t.splits[kk[w]]) > pmin + 1);
+    }
+#pragma unroll
+    for (int a = 0; a < kAxes; ++a) {
+      const int32_t ulo = wave_reduce_i32<false>(qf[2 * a]), uhi = wave_reduce_i32<true>(qf[2 * a + 1]);
+      const int32_t* plo = t.patch_box32 + (uint64_t)(2 * a) * od;  // this axis's slabs of every patch
+      const int32_t* phi = plo + od;
+#pragma unroll
+      for (uint32_t w = 0; w < kFrontierWords; ++w) {
+        if (w * kWave >= od) break;
+        h[w] = h[w] & (ulo < phi[kk[w]]) & (plo[kk[w]] < uhi);
+      }
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kFrontierWords; ++w) cnt += (uint32_t)__popcll(__ballot(h[w]));
+    // the patch roots hit go on the stack; within one patch's subtree the
+    // walk stacks fewer than 96 entries (its height is below the 93 key bits
+    // under the patch bits), so kFrontierPush roots leave the stack room; a
+    // wave hitting more patches starts at the root
+    if (cnt <= kFrontierPush) {
+      from_root = false;
+      uint32_t leafm = 0;  // bit w: this lane's patch of round w is one element, hit
+#pragma unroll
+      for (uint32_t w = 0; w < kFrontierWords; ++w) {
+        const uint32_t r = h[w] ? t.patch_root[kk[w]] : kEnd;
+        const bool in = h[w] & (r < tleaf0);
+        const uint64_t mi = __ballot(in);
+        if (in) stk[sp + (uint32_t)__popcll(mi & lanemask_lt())] = r;
+        sp += (uint32_t)__popcll(mi);
+        leafm |= (uint32_t)(h[w] & (r >= tleaf0)) << w;
+      }
+      // a patch of one element: its leaf is tested here
+      uint64_t lm = __ballot(leafm != 0);
+      while (lm != 0) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(lm);
+        lm &= lm - 1;
+        uint32_t bits = __builtin_amdgcn_readlane(leafm, j);
+        while (bits != 0) {
+          const uint32_t k = (uint32_t)__builtin_ctz(bits) * kWave + j;
+          bits &= bits - 1;
+          const uint32_t pos = uniform(t.patch_root[k]) - tleaf0;
+          if (pos >= n) {
+            if (lane == 0) atomicOr(t.status, kErrTraverse);
+            continue;
+          }
+          int32_t bx[kSlabs];
+#pragma unroll
+          for (int a = 0; a < kSlabs; ++a) bx[a] = t.patch_box32[(uint64_t)a * od + k];
+          const int32_t g = min_gap(qf, bx);
+          bool hit = g > 0;
+          if (SELF) hit = hit & (pos > p32);
+          leaf_check(g, uniform(t.perm[pos]), hit);
+        }
+      }
+    }
+  }
+  if (from_root) {
     const int32_t g = min_gap(qf, (CInt*)t.rootbox32);
     bool h = g > 0;
     if (SELF) h = h & ((n - 1) > p32);  // the root's last position
