@@ -443,14 +443,16 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
     // which also decides whether fp32 proves the fp64 overlap
     const bool leaf_l = cl >= tleaf0, leaf_r = cr >= tleaf0;
     // The next record the walk loads is one of these children's (or one it
-    // pushed earlier).  Lanes 0-1 load a word of each 128-byte line of the
-    // left child's record and lanes 2-3 of the right child's (internal
-    // children only), so the lines are on their way to L2 while this record
-    // is tested; the words feed a sink that nothing reads.
+    // pushed earlier).  The lanes load a word of each 128-byte line of both
+    // children's records (lanes 4j + 0, 1: the left child, 4j + 2, 3: the
+    // right; a leaf child's lanes read this record again), so the lines are
+    // on their way to L2 while this record is tested; the words feed a sink
+    // that nothing reads.  Every lane loads: no branch around the load.
     {
       sink += pf;
-      const uint32_t c = lane < 2 ? cl : cr;
-      pf = (lane < 4 && c < tleaf0) ? ((const uint32_t*)(recs + c))[(lane & 1) * 32] : 0u;
+      uint32_t c = (lane & 2) ? cr : cl;
+      c = c < tleaf0 ? c : x;
+      pf = ((const uint32_t*)(recs + c))[(lane & 1) * 32];
     }
     // per lane, whether a child may hold its pairs at all: position bounds
     // (self) and the own patch (cross), non-short-circuit &: no branch
@@ -465,25 +467,19 @@ __device__ __forceinline__ void walk32(const TreeView& q, const TreeView& t, uin
     }
     // the lanes hitting each child, as scalar masks (the v_cmpx chain tests
     // the same strict overlap as gap > 0); a leaf child that some lane hits
-    // also gets each lane's gap and hit for the candidate test
+    // also gets each lane's gap for the candidate test (both gaps before
+    // either test, so the record's registers are free by then)
     int32_t gl = 0, gr = 0;
-    bool hl = false, hr = false;
     uint64_t ml = box_mask32(qf, rw), mr = box_mask32(qf, rw + 28);
     if (SELF || CROSS) {
       ml &= __ballot(okl);
       mr &= __ballot(okr);
     }
-    if (leaf_l && ml != 0) {
-      gl = min_gap(qf, rw);
-      hl = (gl > 0) & okl;
-    }
-    if (leaf_r && mr != 0) {
-      gr = min_gap(qf, rw + 28);
-      hr = (gr > 0) & okr;
-    }
+    if (leaf_l && ml != 0) gl = min_gap(qf, rw);
+    if (leaf_r && mr != 0) gr = min_gap(qf, rw + 28);
     const bool any_l = ml != 0, any_r = mr != 0;
-    if (any_l && leaf_l) leaf_check(gl, el, SELF ? (hl & (cl - tleaf0 > p32)) : hl);
-    if (any_r && leaf_r) leaf_check(gr, er, SELF ? (hr & (cr - tleaf0 > p32)) : hr);
+    if (any_l && leaf_l) leaf_check(gl, el, (gl > 0) & okl & (!SELF || (cl - tleaf0 > p32)));
+    if (any_r && leaf_r) leaf_check(gr, er, (gr > 0) & okr & (!SELF || (cr - tleaf0 > p32)));
     const bool go_l = any_l && !leaf_l, go_r = any_r && !leaf_r;
     // depth first: continue with the left child, keep the right one
     if (go_l && go_r) {
