@@ -177,6 +177,18 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep_final(BuildBufs b, uint32_t
     }
     bd.pad = 0.0;
     *b.bounds = bd;
+    if (b.origin) {
+      // fp32-outward records: the object's frame, its origin (the Morton
+      // bounds minimum, non-finite components 0) and the origin's projections
+      // on the 13 axes (what the leaf kernel subtracts); a refit keeps it
+      double o[3];
+      for (int c = 0; c < 3; ++c) {
+        const double v = bd.min[c];
+        o[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
+        b.origin[c] = o[c];
+      }
+      proj_all(o[0], o[1], o[2], b.origin + 3);
+    }
   }
 }
 
@@ -821,20 +833,6 @@ __global__ __launch_bounds__(256) void k_patch_slabs(ElemInput in, const uint32_
   patch_partial(pk64, pk, kd, s_part[threadIdx.x / kWave], &s_ppatch[threadIdx.x / kWave]);
   __syncthreads();
   patch_flush<256 / kWave>(pk64, s_part, s_ppatch);
-}
-
-// the object's frame: its origin, the Morton bounds minimum (non-finite
-// components 0), then the origin's projections on the 13 axes (what the leaf
-// kernel subtracts, read once per workgroup)
-__global__ void k_set_frame(const Bounds* bounds, double* frame) {
-  if (threadIdx.x != 0) return;
-  double o[3];
-  for (int c = 0; c < 3; ++c) {
-    const double v = bounds->min[c];
-    o[c] = (v - v == 0.0) ? v : 0.0;  // finite, else 0
-    frame[c] = o[c];
-  }
-  proj_all(o[0], o[1], o[2], frame + 3);
 }
 
 __global__ void k_patch_k64_init(unsigned long long* pk64, uint64_t count) {
@@ -1589,11 +1587,6 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     chunk = next;
   }
   return hipSuccess;
-}
-
-hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(64), 0, s, bounds, frame);
-  return hipGetLastError();
 }
 
 hipError_t launch_patch_k64_init(unsigned long long* pk64, uint64_t od, hipStream_t s) {

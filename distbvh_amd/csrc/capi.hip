@@ -587,13 +587,9 @@ static void object_build_steps(bvh_object_s* o, std::vector<BuildStep>& st) {
       BVH_TRY(split_and_order(o));
       w->stage_end();
     }
-    if (o->fp32) {
-      // the object's fp32 frame: the Morton bounds minimum of this build (a
-      // refit keeps it)
-      HIP_TRY(launch_set_frame(o->bb.bounds, o->bb.origin, s));
-    } else {
-      HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));  // written at the patch roots
-    }
+    // (the fp32 frame, the Morton bounds minimum of this build, is written
+    // with the bounds)
+    if (!o->fp32) HIP_TRY(launch_patch_k64_init(o->bb.patch_k64, w->od, s));  // written at the patch roots
     w->stage_begin("delta");
     HIP_TRY(launch_delta(o->in, o->bb, o->bits, w->od, s));
     w->stage_end();

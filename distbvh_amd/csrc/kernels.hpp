@@ -121,7 +121,6 @@ hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint6
 hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s);
 // fp32 form: set the world frame (3 doubles, then a flag word) from the
 // bounds minimum unless already set; non-finite components become 0
-hipError_t launch_set_frame(const Bounds* bounds, double* frame, hipStream_t s);
 // patch_k64 set to the empty k-DOP before the leaf kernel
 hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hipStream_t s);
 // fp64 form: the leaves' slabs in sorted order (n x 26), read from their
