@@ -1542,7 +1542,7 @@ hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint6
   return hipGetLastError();
 }
 
-hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
+hipError_t launch_tree_leaves(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
   const uint64_t n = in.n;
   if (n == 0) return hipSuccess;
   // every buffer the kernels write must exist for this form
@@ -1567,8 +1567,11 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
   } else {
     tree_dispatch<kSnapshot, 1>(in, b, od, fp32, s);
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_up(uint64_t n, const BuildBufs& b, int fp32, hipStream_t s) {
+  hipError_t e = hipSuccess;
   // upper phases: chunks kTreeChunk times larger until one chunk holds every leaf
   uint64_t chunk = kTreeChunk;
   FrontNode* fin = b.front_a;
@@ -1587,6 +1590,11 @@ hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int
     chunk = next;
   }
   return hipSuccess;
+}
+
+hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s) {
+  const hipError_t e = launch_tree_leaves(in, b, od, fp32, s);
+  return e != hipSuccess ? e : launch_tree_up(in.n, b, fp32, s);
 }
 
 hipError_t launch_patch_k64_init(unsigned long long* pk64, uint64_t od, hipStream_t s) {

@@ -277,6 +277,12 @@ struct bvh_object_s {
   size_t steps_done = 0;  // steps already launched (the first ones go at the call)
   hipEvent_t ev_sorted = nullptr;
   bool build_pending = false;
+  // The flush runs the pending builds' leaf kernels one after another (each
+  // object's upper tree phases overlap the next one's leaf kernel); ev_leaves
+  // follows this build's leaf kernel (leaves_event: recorded for the pending
+  // build), which is all a query walk needs of its query object.
+  hipEvent_t ev_leaves = nullptr;
+  bool leaves_event = false;
 
   bvh_object_s() = default;
   bvh_object_s(const bvh_object_s&) = delete;
@@ -289,6 +295,7 @@ struct bvh_object_s {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_done) (void)hipEventDestroy(ev_done);
     if (ev_sorted) (void)hipEventDestroy(ev_sorted);
+    if (ev_leaves) (void)hipEventDestroy(ev_leaves);
   }
 
   // every device buffer of the object
@@ -321,7 +328,7 @@ static bvh_status join(bvh_object_s* o) {
   HIP_TRY(hipStreamWaitEvent(o->w->stream, o->ev_done, 0));
   return BVH_OK;
 }
-static bvh_status flush_builds(bvh_world w);
+static bvh_status flush_builds(bvh_world w, const bvh_object_s* first = nullptr);
 // ... for every object of the world, after enqueuing the tree stages still
 // pending: every call that reads objects' device data or may be followed by
 // the caller's use of it starts with this.
@@ -645,14 +652,20 @@ static bvh_status object_build_tree(bvh_object_s* o) {
 // Enqueues the world's pending (forked) builds: their steps interleaved, one
 // step of each build in turn, each on its build stream, so that every build
 // has work on the GPU from the start and the latency-bound sort passes of
-// several builds run side by side; then each tree stage, after the steps of
+// several builds run side by side; then the tree stages, after the steps of
 // all of them (no sort pass waits behind another build's tree kernel); and
-// moves each build's ev_done behind its tree.
-static bvh_status flush_builds(bvh_world w) {
+// moves each build's ev_done behind its tree.  The trees' leaf kernels run
+// one after another (two side by side took longer than one after the other,
+// the vector memory units being their bound), `first` (a query's target)
+// first, and each build's upper tree phases, latency-bound and narrow,
+// overlap the next build's leaf kernel.
+static bvh_status flush_builds(bvh_world w, const bvh_object_s* first) {
   std::vector<bvh_object_s*> p;
   for (auto& o : w->objects)
     if (o->build_pending) p.push_back(o.get());
   if (p.empty()) return BVH_OK;
+  for (size_t j = 1; j < p.size(); ++j)
+    if (p[j] == first) std::swap(p[0], p[j]);
   std::vector<bvh_status> sa(p.size(), BVH_OK);
   size_t nsteps = 0;
   for (bvh_object_s* a : p) nsteps = std::max(nsteps, a->steps.size() - a->steps_done);
@@ -669,19 +682,51 @@ static bvh_status flush_builds(bvh_world w) {
     for (bvh_object_s* b : p)
       if (a != b) HIP_TRY(hipStreamWaitEvent(a->bs, b->ev_sorted, 0));
   bvh_status st = BVH_OK;
+  const bvh_object_s* prev = nullptr;  // the build whose leaf kernel runs before the next one's
   for (size_t j = 0; j < p.size(); ++j) {
     bvh_object_s* a = p[j];
     a->build_pending = false;
     a->steps.clear();
     if (sa[j] == BVH_OK) {
       StreamSwap swap(w, a->bs);
-      sa[j] = object_build_tree(a);
+      if (prev) HIP_TRY(hipStreamWaitEvent(a->bs, prev->ev_leaves, 0));
+      w->stage_begin("tree");
+      hipError_t e = launch_tree_leaves(a->in, a->bb, w->od, a->fp32, a->bs);
+      if (e == hipSuccess) e = hipEventRecord(a->ev_leaves, a->bs);
+      if (e == hipSuccess) e = launch_tree_up(a->in.n, a->bb, a->fp32, a->bs);
+      w->stage_end();
+      if (e != hipSuccess) {
+        sa[j] = fail(BVH_E_HIP, std::string("tree stage: ") + hipGetErrorString(e));
+        (void)hipGetLastError();
+      } else {
+        a->leaves_event = true;
+        prev = a;
+      }
     }
     // recorded also after a failure, so that nothing enqueued is left unordered
     HIP_TRY(hipEventRecord(a->ev_done, a->bs));
     if (st == BVH_OK) st = sa[j];
   }
   return st;
+}
+
+// The world stream before a query of o against another object: every pending
+// build flushed (the target's tree first), the target's and every other
+// build joined, and o's joined only up to its leaf kernel when that was
+// flushed (the walk reads o's leaf slabs, positions and permutation, all in
+// place by then, and nothing its upper tree phases write): those phases run
+// beside the walk, and run_query joins them before it reads the counters.
+static bvh_status join_for_query(bvh_object_s* o, bvh_object_s* other) {
+  bvh_world w = o->w;
+  BVH_TRY(flush_builds(w, other));
+  for (auto& x : w->objects) {
+    bvh_object_s* a = x.get();
+    if (a == o && a != other && a->pending && a->leaves_event)
+      HIP_TRY(hipStreamWaitEvent(w->stream, a->ev_leaves, 0));  // o stays pending
+    else
+      BVH_TRY(join(a));
+  }
+  return BVH_OK;
 }
 
 // Refit: new element k-DOPs on the topology of the last build (same
@@ -752,6 +797,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
       HIP_TRY(hipEventCreateWithFlags(&o->ev_fork, hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&o->ev_done, hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&o->ev_sorted, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&o->ev_leaves, hipEventDisableTiming));
     }
     HIP_TRY(hipEventRecord(o->ev_fork, w->stream));
     HIP_TRY(hipStreamWaitEvent(o->bs, o->ev_fork, 0));
@@ -761,6 +807,7 @@ static bvh_status set_entity_common(bvh_object_s* o, int kind, int nv, double ep
   // the host runtime on a graph holding such waits between two builds.  The
   // capture keeps the fork / join pattern only (a graph branch per build).
   const bool defer = fork && !w->capturing;
+  o->leaves_event = false;  // recorded again by the flush of a deferred build
   const bvh_status st =
       set_entity_build(o, kind, nv, eps, data, cent, n, gids, mem, split, fork ? o->bs : w->stream, defer);
   if (fork) {
@@ -1015,6 +1062,7 @@ static bvh_status run_query(bvh_object_s* o, bvh_object_s* other, bool self, boo
   }
   for (int attempt = 0; attempt < 3; ++attempt) {
     BVH_TRY(launch_query(o, other, self, narrow, cross));
+    BVH_TRY(join(o));  // (a query object's upper tree phases may have run beside the walk)
     bool overflow = false;
     BVH_TRY(read_counts(o, other, &overflow));
     if (!overflow) break;
@@ -1601,9 +1649,12 @@ bvh_status bvh_init_broadphase(bvh_object o) {
 
 bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
   if (!o || !other) return fail(BVH_E_INVALID, "null object");
-  BVH_TRY(join_all(o->w));
-  if (o->w != other->w) return fail(BVH_E_INVALID, "objects of different worlds");
+  if (o->w != other->w) {
+    BVH_TRY(join_all(o->w));
+    return fail(BVH_E_INVALID, "objects of different worlds");
+  }
   if (o == other) {
+    BVH_TRY(join_all(o->w));
     // the reference skips self collisions in the narrowphase
     // (collision_object/impl.cpp:229-235): no results
     o->npairs = 0;
@@ -1614,6 +1665,7 @@ bvh_status bvh_broadphase(bvh_object o, bvh_object other) {
     o->self_result = true;
     return BVH_OK;
   }
+  BVH_TRY(join_for_query(o, other));
   return run_query(o, other, false);
 }
 

@@ -117,10 +117,13 @@ hipError_t launch_morton(const ElemInput& in, const BuildBufs& b, int bits, hipS
 hipError_t launch_cluster(const BuildBufs& b, uint64_t n, int bits, uint64_t nsplits, hipStream_t s);
 // k_delta (patch ids, split heights, cleared counters)
 hipError_t launch_delta(const ElemInput& in, const BuildBufs& b, int bits, uint64_t od, hipStream_t s);
-// k_tree (leaf slabs in sorted order + bottom-up build with refit)
+// k_tree_leaves + k_tree_up (leaf slabs in sorted order + bottom-up build
+// with refit); the two parts on their own: the leaf kernel writes every
+// leaf's half of its parent's record, the upper phases the nodes above the
+// leaf kernel's chunks
 hipError_t launch_tree(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s);
-// fp32 form: set the world frame (3 doubles, then a flag word) from the
-// bounds minimum unless already set; non-finite components become 0
+hipError_t launch_tree_leaves(const ElemInput& in, const BuildBufs& b, uint64_t od, int fp32, hipStream_t s);
+hipError_t launch_tree_up(uint64_t n, const BuildBufs& b, int fp32, hipStream_t s);
 // patch_k64 set to the empty k-DOP before the leaf kernel
 hipError_t launch_patch_k64_init(unsigned long long* patch_k64, uint64_t od, hipStream_t s);
 // fp64 form: the leaves' slabs in sorted order (n x 26), read from their
