@@ -145,12 +145,14 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_pass(SortArgs<K> a, int pas
   __shared__ uint32_t s_texcl[256];
   __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_hist[256];
   __shared__ K s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
 
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   if (tid == 0) s_tile = atomicAdd(&a.ctl->tile_counter[pass], 1u);
   for (int i = tid; i < 4 * 256; i += kSortBlock) (&s_wcount[0][0])[i] = 0;
+  s_hist[tid] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
   if (tile >= a.ntiles) return;  // uniform per block; no tile waits on it
@@ -167,6 +169,18 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_pass(SortArgs<K> a, int pas
     key[j] = valid ? kin[pos] : (K)0;
     if (has_vals) val[j] = valid ? (vin ? vin[pos] : (uint32_t)pos) : 0u;
     dig[j] = valid ? (uint32_t)((key[j] >> shift) & 255) : 256u;
+  }
+  // the tile's digit counts first, published as its aggregate before the
+  // ranking, so that later tiles' look-backs find it sooner
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j)
+    if (dig[j] < 256u) atomicAdd(&s_hist[dig[j]], 1u);
+  __syncthreads();
+  uint32_t* lb = a.lookback + ((uint64_t)pass * a.ntiles + tile) * 256;
+  {
+    const uint32_t tot = s_hist[tid];
+    if (tile == 0) lb_store(&lb[tid], kFlagInc | tot);
+    else lb_store(&lb[tid], kFlagAgg | tot);
   }
   const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -200,12 +214,6 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_pass(SortArgs<K> a, int pas
   s_wcount[2][t] = c0 + c1;
   s_wcount[3][t] = c0 + c1 + c2;
 
-  uint32_t* lb = a.lookback + ((uint64_t)pass * a.ntiles + tile) * 256;
-  if (tile == 0) {
-    lb_store(&lb[t], kFlagInc | total);
-  } else {
-    lb_store(&lb[t], kFlagAgg | total);
-  }
   const uint32_t texcl = block_excl_scan256(total, s_tmp);
   s_texcl[t] = texcl;
 
