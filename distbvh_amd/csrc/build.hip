@@ -193,6 +193,7 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep_final(BuildBufs b, uint32_t
 }
 
 // Morton codes in input order + digit histograms of every sort pass.
+constexpr int kMortonBatch = 4;
 template <int KIND, int BITS>
 __global__ __launch_bounds__(256) void k_morton(ElemInput in, BuildBufs b) {
   constexpr int NP = (BITS == 63) ? 8 : 4;
@@ -201,43 +202,60 @@ __global__ __launch_bounds__(256) void k_morton(ElemInput in, BuildBufs b) {
   __syncthreads();
   const Bounds bd = *b.bounds;
   const uint64_t n = in.n;
-  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (uint64_t)gridDim.x * blockDim.x) {
-    double x, y, z;
-    if constexpr (KIND == kPoly) {
-      x = b.centroid[e]; y = b.centroid[n + e]; z = b.centroid[2 * n + e];
-    } else if constexpr (KIND == kSphere) {
-      x = in.data[e]; y = in.data[n + e]; z = in.data[2 * n + e];
-    } else {
-      x = in.centroids[3 * e]; y = in.centroids[3 * e + 1]; z = in.centroids[3 * e + 2];
-    }
-    uint64_t code;
-    if constexpr (BITS == 63) {
-      code = morton63(x, y, z, bd);
-      ((uint64_t*)b.codes0)[e] = code;
-    } else {
-      uint32_t c32 = morton30(x, y, z, bd);
-      ((uint32_t*)b.codes0)[e] = c32;
-      code = c32;
-    }
-    // a wave whose lanes share the digit adds once (the active-lane mask is
-    // taken before the leader branch)
-    const uint64_t active = __ballot(1);
-    const uint32_t leader = (uint32_t)(__ffsll((long long)active) - 1);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // kMortonBatch elements per lane and round, their centroid loads issued
+  // together (one memory round trip per round, not per element)
+  for (uint64_t e0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < n; e0 += kMortonBatch * stride) {
+    double xs[kMortonBatch], ys[kMortonBatch], zs[kMortonBatch];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const uint32_t d = (uint32_t)((code >> (8 * p)) & 255);
-      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-      if (__all(d == d0)) {
-        if (lane_id() == leader) atomicAdd(&sh[p * 256 + d0], (uint32_t)__popcll(active));
-      } else {
-        atomicAdd(&sh[p * 256 + d], 1u);
+    for (int u = 0; u < kMortonBatch; ++u) {
+      const uint64_t e = e0 + u * stride;
+      xs[u] = ys[u] = zs[u] = 0.0;
+      if (e < n) {
+        if constexpr (KIND == kPoly) {
+          xs[u] = b.centroid[e]; ys[u] = b.centroid[n + e]; zs[u] = b.centroid[2 * n + e];
+        } else if constexpr (KIND == kSphere) {
+          xs[u] = in.data[e]; ys[u] = in.data[n + e]; zs[u] = in.data[2 * n + e];
+        } else {
+          xs[u] = in.centroids[3 * e]; ys[u] = in.centroids[3 * e + 1]; zs[u] = in.centroids[3 * e + 2];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMortonBatch; ++u) {
+      const uint64_t e = e0 + u * stride;
+      const bool valid = e < n;
+      uint64_t code = 0;
+      if (valid) {
+        if constexpr (BITS == 63) {
+          code = morton63(xs[u], ys[u], zs[u], bd);
+          ((uint64_t*)b.codes0)[e] = code;
+        } else {
+          const uint32_t c32 = morton30(xs[u], ys[u], zs[u], bd);
+          ((uint32_t*)b.codes0)[e] = c32;
+          code = c32;
+        }
+      }
+      // a wave whose valid lanes share the digit adds once
+      const uint64_t active = __ballot(valid);
+      if (active == 0) continue;
+      const uint32_t leader = (uint32_t)(__ffsll((long long)active) - 1);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const uint32_t d = (uint32_t)((code >> (8 * p)) & 255);
+        const uint32_t d0 = __builtin_amdgcn_readlane(d, leader);
+        if (__ballot(valid & (d != d0)) == 0) {
+          if (lane_id() == leader) atomicAdd(&sh[p * 256 + d0], (uint32_t)__popcll(active));
+        } else if (valid) {
+          atomicAdd(&sh[p * 256 + d], 1u);
+        }
       }
     }
   }
   __syncthreads();
+  uint32_t* hc = b.hist + (blockIdx.x % kHistCopies) * kHistWords;
   for (int i = threadIdx.x; i < NP * 256; i += blockDim.x)
-    if (sh[i]) atomicAdd(&b.hist[i], sh[i]);
+    if (sh[i]) atomicAdd(&hc[i], sh[i]);
 }
 
 // depth(i) = clz(h[i] ^ h[i+1]) (255 for equal codes; the reference uses
@@ -249,17 +267,33 @@ __global__ __launch_bounds__(256) void k_cluster_depth(const K* __restrict__ cod
   sh[threadIdx.x] = 0;
   __syncthreads();
   const uint64_t m = n - 1;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const K x = codes[i] ^ codes[i + 1];
-    uint32_t d;
-    if constexpr (sizeof(K) == 8) d = x ? (uint32_t)__clzll((long long)x) : 255u;
-    else d = x ? (uint32_t)__clz((int)x) : 255u;
-    depth[i] = d;
-    atomicAdd(&sh[d], 1u);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // kMortonBatch boundaries per lane and round, their code loads issued together
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < m; i0 += kMortonBatch * stride) {
+    K lo[kMortonBatch], hi[kMortonBatch];
+#pragma unroll
+    for (int u = 0; u < kMortonBatch; ++u) {
+      const uint64_t i = i0 + u * stride;
+      lo[u] = hi[u] = 0;
+      if (i < m) {
+        lo[u] = codes[i];
+        hi[u] = codes[i + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMortonBatch; ++u) {
+      const uint64_t i = i0 + u * stride;
+      if (i >= m) break;
+      const K x = lo[u] ^ hi[u];
+      uint32_t d;
+      if constexpr (sizeof(K) == 8) d = x ? (uint32_t)__clzll((long long)x) : 255u;
+      else d = x ? (uint32_t)__clz((int)x) : 255u;
+      depth[i] = d;
+      atomicAdd(&sh[d], 1u);
+    }
   }
   __syncthreads();
-  if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+  if (sh[threadIdx.x]) atomicAdd(&hist[(blockIdx.x % kHistCopies) * kHistWords + threadIdx.x], sh[threadIdx.x]);
 }
 
 // The first c entries of the (depth, index) order, sorted by position
@@ -281,7 +315,9 @@ struct ClusterCut {
 __device__ __forceinline__ ClusterCut cluster_cut(const uint32_t* __restrict__ hist, uint64_t c) {
   __shared__ uint32_t h[256];
   __shared__ ClusterCut cut;
-  h[threadIdx.x] = hist[threadIdx.x];
+  uint32_t v = 0;
+  for (int c = 0; c < kHistCopies; ++c) v += hist[c * kHistWords + threadIdx.x];
+  h[threadIdx.x] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint64_t cum = 0;
@@ -1441,9 +1477,9 @@ hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s) {
   (void)in;
   ZeroList zl{};
   zl.count = 0;
-  // b.hist holds the Morton histograms (8*256) followed by the depth
-  // histogram (256)
-  zl.r[zl.count++] = {b.hist, 9 * 256, 0u};
+  // b.hist holds kHistCopies copies of the Morton histograms (8*256)
+  // followed by the depth histogram (256)
+  zl.r[zl.count++] = {b.hist, kHistCopies * kHistWords, 0u};
   // look-back words of the Morton sort (8 passes) and of the depth sort
   const uint64_t lb_words = (8 + 1) * sort_tiles(in.n > 0 ? in.n : 1) * 256;
   zl.r[zl.count++] = {b.lookback, lb_words, 0u};

@@ -391,7 +391,7 @@ static bvh_status object_alloc(bvh_object_s* o) {
   BVH_TRY(o->codes_b.ensure(n1 * kw));
   BVH_TRY(o->perm.ensure(n1 * 4));
   BVH_TRY(o->perm_b.ensure(n1 * 4));
-  BVH_TRY(o->hist.ensure(9 * 256 * 4));
+  BVH_TRY(o->hist.ensure(kHistCopies * kHistWords * 4));
   BVH_TRY(o->ctl.ensure(2 * sizeof(SortCtl)));
   BVH_TRY(o->lookback.ensure(9 * tiles * 256 * 4));
   BVH_TRY(o->depth0.ensure(n1 * 4));
@@ -572,13 +572,15 @@ static void object_build_steps(bvh_object_s* o, std::vector<BuildStep>& st) {
         a.k0 = (const uint64_t*)o->bb.codes0; a.ka = (uint64_t*)o->bb.codes_a; a.kb = (uint64_t*)o->bb.codes_b;
         a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
         a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
-        HIP_TRY(p < 0 ? launch_sort_plan<uint64_t>(a, npass, o->bb.hist, s) : launch_sort_pass<uint64_t>(a, p, s));
+        HIP_TRY(p < 0 ? launch_sort_plan<uint64_t>(a, npass, o->bb.hist, s, kHistCopies, kHistWords)
+                      : launch_sort_pass<uint64_t>(a, p, s));
       } else {
         SortArgs<uint32_t> a{};
         a.k0 = (const uint32_t*)o->bb.codes0; a.ka = (uint32_t*)o->bb.codes_a; a.kb = (uint32_t*)o->bb.codes_b;
         a.v0 = nullptr; a.va = o->bb.perm; a.vb = o->bb.perm_b;
         a.n = n; a.ntiles = sort_tiles(n); a.ctl = o->bb.ctl; a.lookback = o->bb.lookback;
-        HIP_TRY(p < 0 ? launch_sort_plan<uint32_t>(a, npass, o->bb.hist, s) : launch_sort_pass<uint32_t>(a, p, s));
+        HIP_TRY(p < 0 ? launch_sort_plan<uint32_t>(a, npass, o->bb.hist, s, kHistCopies, kHistWords)
+                      : launch_sort_pass<uint32_t>(a, p, s));
       }
       if (p == npass - 1) w->stage_end();
       return BVH_OK;

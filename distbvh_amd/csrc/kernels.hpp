@@ -42,8 +42,15 @@ hipError_t launch_sort_histogram(const K* keys, uint64_t n, int npass, uint32_t*
 template <typename K>
 hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s);
 // the same launches one at a time: the plan, then pass p (0 .. npass-1) in order
+// The build's digit histograms (the Morton kernel's 8 x 256 words of sort
+// digits, then the 256 split depths of the cluster kernel): kHistCopies
+// copies, block b adding into copy b mod kHistCopies, so that the blocks'
+// global adds spread over more addresses; their readers sum the copies.
+constexpr int kHistCopies = 8;
+constexpr uint32_t kHistWords = 9 * 256;
 template <typename K>
-hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s);
+hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s, int copies = 1,
+                            uint32_t copy_stride = 0);
 template <typename K>
 hipError_t launch_sort_pass(const SortArgs<K>& a, int p, hipStream_t s);
 // exclusive scan in place; lb: sort_tiles(n) zeroed 64-bit words, counter: zeroed

@@ -78,19 +78,49 @@ __global__ __launch_bounds__(256) void k_sort_histogram(const K* __restrict__ ke
 // One block: digit offsets (exclusive scan per pass), pass activity and the
 // ping-pong buffer selectors (the last active pass writes buffer A).
 __global__ __launch_bounds__(256) void k_sort_plan(const uint32_t* __restrict__ hist, uint64_t n, int npass,
-                                                   SortCtl* __restrict__ ctl) {
-  __shared__ uint32_t s_tmp[4];
+                                                   SortCtl* __restrict__ ctl, int copies, uint32_t copy_stride) {
+  __shared__ uint32_t s_wtot[8][4];
   __shared__ uint32_t s_active[8];
   const int t = threadIdx.x;
-  for (int p = 0; p < npass; ++p) {
-    uint32_t h = hist[p * 256 + t];
-    uint32_t ex = block_excl_scan256(h, s_tmp);
-    ctl->offsets[p * 256 + t] = ex;
-    // a pass is needed unless one bin holds every key
-    int full = (uint64_t)h == n;
-    int any_full = __syncthreads_or(full);
-    if (t == 0) s_active[p] = any_full ? 0u : 1u;
-    __syncthreads();
+  // every pass's count of digit t, summed over the copies (all loads issued
+  // before any is used)
+  uint32_t hs[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    hs[p] = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (p < npass && c < copies) hs[p] += hist[(uint64_t)c * copy_stride + p * 256 + t];
+  }
+  // the exclusive scans of all passes side by side (wave scans, then the
+  // four wave totals of each pass through LDS: two barriers for all passes);
+  // a pass is needed unless one bin holds every key
+  const int lane = t & 63, w = t >> 6;
+  uint32_t xs[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) xs[p] = hs[p];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const uint32_t y = __shfl_up(xs[p], o, 64);
+      if (lane >= o) xs[p] += y;
+    }
+  }
+  if (t < 8) s_active[t] = 1u;
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    if (lane == 63) s_wtot[p][w] = xs[p];
+    if (p < npass && (uint64_t)hs[p] == n) s_active[p] = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    if (p >= npass) break;  // (uniform)
+    uint32_t wp = 0;
+    for (int i = 0; i < w; ++i) wp += s_wtot[p][i];
+    ctl->offsets[p * 256 + t] = wp + xs[p] - hs[p];
   }
   if (t == 0) {
     int nact = 0;
@@ -335,9 +365,11 @@ template hipError_t launch_sort_histogram<uint32_t>(const uint32_t*, uint64_t, i
 // Sort after the histograms are in `hist` (npass * 256 counters) and the
 // look-back words are zero.  Result in a.ka (and a.va).
 template <typename K>
-hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s) {
+hipError_t launch_sort_plan(const SortArgs<K>& a, int npass, const uint32_t* hist, hipStream_t s, int copies,
+                            uint32_t copy_stride) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, hist, a.n, npass, a.ctl);
+  if (npass < 1 || npass > 8 || copies < 1 || copies > 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, hist, a.n, npass, a.ctl, copies, copy_stride);
   return hipGetLastError();
 }
 
@@ -354,8 +386,10 @@ hipError_t launch_sort_passes(const SortArgs<K>& a, int npass, const uint32_t* h
   for (int p = 0; p < npass && e == hipSuccess; ++p) e = launch_sort_pass<K>(a, p, s);
   return e;
 }
-template hipError_t launch_sort_plan<uint64_t>(const SortArgs<uint64_t>&, int, const uint32_t*, hipStream_t);
-template hipError_t launch_sort_plan<uint32_t>(const SortArgs<uint32_t>&, int, const uint32_t*, hipStream_t);
+template hipError_t launch_sort_plan<uint64_t>(const SortArgs<uint64_t>&, int, const uint32_t*, hipStream_t, int,
+                                              uint32_t);
+template hipError_t launch_sort_plan<uint32_t>(const SortArgs<uint32_t>&, int, const uint32_t*, hipStream_t, int,
+                                              uint32_t);
 template hipError_t launch_sort_pass<uint64_t>(const SortArgs<uint64_t>&, int, hipStream_t);
 template hipError_t launch_sort_pass<uint32_t>(const SortArgs<uint32_t>&, int, hipStream_t);
 template hipError_t launch_sort_passes<uint64_t>(const SortArgs<uint64_t>&, int, const uint32_t*, hipStream_t);
