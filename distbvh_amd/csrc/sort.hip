@@ -84,13 +84,24 @@ __global__ __launch_bounds__(256) void k_sort_plan(const uint32_t* __restrict__ 
   const int t = threadIdx.x;
   // every pass's count of digit t, summed over the copies (all loads issued
   // before any is used)
+  // (every load is made, at a clamped address, and the unwanted ones count
+  // zero: no branch separates them)
+  uint32_t hv[8][8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const int pp = p < npass ? p : 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int cc = c < copies ? c : 0;
+      hv[p][c] = hist[(uint64_t)cc * copy_stride + pp * 256 + t];
+    }
+  }
   uint32_t hs[8];
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
     hs[p] = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-      if (p < npass && c < copies) hs[p] += hist[(uint64_t)c * copy_stride + p * 256 + t];
+    for (int c = 0; c < 8; ++c) hs[p] += (p < npass && c < copies) ? hv[p][c] : 0u;
   }
   // the exclusive scans of all passes side by side (wave scans, then the
   // four wave totals of each pass through LDS: two barriers for all passes);
