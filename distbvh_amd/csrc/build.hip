@@ -141,6 +141,7 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep(ElemInput in, BuildBufs b, 
 // Fold of the block partials into min / inv_diag / max (one block).  A
 // separate launch instead of a last-block fold: on gfx950 the device-scope
 // fence that fold needs writes back the whole L2 in every block.
+constexpr int kPrepFold = 8;
 __global__ __launch_bounds__(kPrepBlock) void k_prep_final(BuildBufs b, uint32_t nblocks) {
   __shared__ double s_part[kPrepBlock / 64][6];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -149,11 +150,24 @@ __global__ __launch_bounds__(kPrepBlock) void k_prep_final(BuildBufs b, uint32_t
     r[c] = 1.7976931348623157e308;
     r[3 + c] = -1.7976931348623157e308;
   }
-  for (unsigned blk = threadIdx.x; blk < nblocks; blk += blockDim.x)
-    for (int c = 0; c < 3; ++c) {
-      r[c] = sel_min(r[c], b.partials[blk * 6 + c]);
-      r[3 + c] = sel_max(r[3 + c], b.partials[blk * 6 + 3 + c]);
+  // kPrepFold partials per thread and round, their loads issued together
+  for (unsigned blk0 = threadIdx.x; blk0 < nblocks; blk0 += kPrepFold * blockDim.x) {
+    double pv[kPrepFold][6];
+#pragma unroll
+    for (int u = 0; u < kPrepFold; ++u) {
+      const unsigned blk = blk0 + u * blockDim.x;
+      const unsigned bl = blk < nblocks ? blk : blk0;  // (blk0 is in range)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) pv[u][c] = b.partials[bl * 6 + c];
     }
+#pragma unroll
+    for (int u = 0; u < kPrepFold; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[c] = sel_min(r[c], pv[u][c]);
+        r[3 + c] = sel_max(r[3 + c], pv[u][3 + c]);
+      }
+  }
   for (int c = 0; c < 3; ++c) {
     r[c] = wave_min(r[c]);
     r[3 + c] = wave_max(r[3 + c]);
@@ -315,8 +329,12 @@ struct ClusterCut {
 __device__ __forceinline__ ClusterCut cluster_cut(const uint32_t* __restrict__ hist, uint64_t c) {
   __shared__ uint32_t h[256];
   __shared__ ClusterCut cut;
+  uint32_t hv[kHistCopies];  // (all loads issued before the sum)
+#pragma unroll
+  for (int c = 0; c < kHistCopies; ++c) hv[c] = hist[c * kHistWords + threadIdx.x];
   uint32_t v = 0;
-  for (int c = 0; c < kHistCopies; ++c) v += hist[c * kHistWords + threadIdx.x];
+#pragma unroll
+  for (int c = 0; c < kHistCopies; ++c) v += hv[c];
   h[threadIdx.x] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
