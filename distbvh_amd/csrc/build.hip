@@ -1501,6 +1501,9 @@ hipError_t launch_prep(const ElemInput& in, const BuildBufs& b, hipStream_t s) {
   // look-back words of the Morton sort (8 passes) and of the depth sort
   const uint64_t lb_words = (8 + 1) * sort_tiles(in.n > 0 ? in.n : 1) * 256;
   zl.r[zl.count++] = {b.lookback, lb_words, 0u};
+  // the build's status words (error bits, fp32 slab error); nothing in this
+  // launch sets them
+  zl.r[zl.count++] = {b.status, 4, 0u};
   if (in.kind == kPoly) {
     switch (in.nv) {
       case 1: prep_dispatch<kPoly, 1>(in, b, zl, s); break;

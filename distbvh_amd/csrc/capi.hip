@@ -551,7 +551,7 @@ static void object_build_steps(bvh_object_s* o, std::vector<BuildStep>& st) {
   const uint64_t n = o->in.n;
   st.clear();
   st.push_back([=] {
-    HIP_TRY(launch_fill_u32(o->bb.status, 4, 0u, s));  // error bits, fp32 slab error
+    // (the bounds kernel also clears the status words)
     w->stage_begin("bounds");
     HIP_TRY(launch_prep(o->in, o->bb, s));
     w->stage_end();
