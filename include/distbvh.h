@@ -13,8 +13,10 @@
  * work enqueued on the world stream before the call, so that the builds of
  * several objects run concurrently; the world stream waits for it at the
  * next call on the world that is not a set_entity_data or init_broadphase
- * (BVH_FLAG_SERIAL_BUILDS keeps every build on the world stream).  Until
- * then the caller's own work on the world stream is not ordered after it.
+ * (bvh_broadphase(a, b) lets a's upper tree levels run beside its walk and
+ * waits for them before it returns; BVH_FLAG_SERIAL_BUILDS keeps every build
+ * on the world stream).  Until then the caller's own work on the world
+ * stream is not ordered after it.
  *
  * Element data: polytope vertices are SoA planes, plane (v*3 + c) of length
  * n holds coordinate c (x, y, z) of vertex v of every element; spheres are
