@@ -680,9 +680,11 @@ static bvh_status flush_builds(bvh_world w, const bvh_object_s* first) {
       }
     }
   for (bvh_object_s* a : p) HIP_TRY(hipEventRecord(a->ev_sorted, a->bs));
+  // (a query's target starts its tree as soon as its own steps are done: the
+  // other builds' leaf kernels come after its leaf kernel anyway)
   for (bvh_object_s* a : p)
     for (bvh_object_s* b : p)
-      if (a != b) HIP_TRY(hipStreamWaitEvent(a->bs, b->ev_sorted, 0));
+      if (a != b && a != first) HIP_TRY(hipStreamWaitEvent(a->bs, b->ev_sorted, 0));
   bvh_status st = BVH_OK;
   const bvh_object_s* prev = nullptr;  // the build whose leaf kernel runs before the next one's
   for (size_t j = 0; j < p.size(); ++j) {
